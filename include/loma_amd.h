@@ -1,0 +1,153 @@
+/*
+ * loma_amd.h -- C ABI of the MI355X-native LoMa RK3 time-advance library
+ * (libloma_hip.so).  Plain pointers and sizes only.
+ *
+ * Each entry point replaces one routine of the reference solver; the
+ * replaced routine is cited as path:line relative to the reference tree.
+ * Conventions shared with the reference:
+ *   - complex y-columns are interleaved (re,im) pairs, like the reference's
+ *     real(4) u(2,my,m) arguments (trunk/lomacte3/src/cfdiff.f:518)
+ *   - spectral fields are [kx plane i][kz mode k][y j], y fastest, kz stored
+ *     0..nz then -nz..-1 (trunk/lomacte3/src/cross.f:77-79)
+ *   - band matrices are (nb,my) Fortran arrays, like module matrices
+ *     (trunk/lomacte3/src/modulos.f:121-128)
+ * Every function returns an int status (LOMA_OK on success).  Errors never
+ * abort the process; the Fortran shim turns a nonzero status into `stop`.
+ */
+#ifndef LOMA_AMD_H
+#define LOMA_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LOMA_OK 0
+#define LOMA_ERR_ARG 1     /* bad size / argument */
+#define LOMA_ERR_NAN 2     /* non-finite value detected in the state */
+#define LOMA_ERR_OOM 3     /* device allocation failed */
+#define LOMA_ERR_DEVICE 4  /* HIP runtime error */
+#define LOMA_ERR_STATE 5   /* call order (e.g. advance before upload) */
+#define LOMA_ERR_NODEV 6   /* no GPU visible */
+
+/* precision modes */
+#define LOMA_FP64 0   /* everything in double */
+#define LOMA_MIXED 1  /* FP32 transforms + nonlinear products, FP64 y-solves,
+                         00 modes and RK accumulation */
+
+/* flags */
+#define LOMA_FLAG_DT_FP32 1   /* round dt to float, as the tem module stores it
+                                 (trunk/lomacte3/src/modulos.f:87-90) */
+#define LOMA_FLAG_SCALAR_Z 2  /* transport a passive scalar Z
+                                 (branches/lomaHZB/src/cross.f:2736) */
+
+typedef struct loma_run loma_run;
+
+/* ---- library ---------------------------------------------------------- */
+const char *loma_version(void);
+const char *loma_last_error(void);   /* message of the last failing call */
+int loma_device_count(int *n);
+
+/* Compact-FD operators for a y grid (derivadas, cfdiff.f:869-941).  Host
+ * only; no GPU needed.  Outputs (nb,my) Fortran-ordered like module matrices:
+ * dt11 = A1 (7-band LHS of d/deta), dt12 = B1 (7-band RHS), dt21 = A2 and
+ * dt22 = B2 (5-band second derivative), fmap = 1/(dy/deta). */
+int loma_fd_matrices(int my, const double *y, double *dt11, double *dt12,
+                     double *dt21, double *dt22, double *fmap);
+
+/* ---- run lifecycle ---------------------------------------------------- */
+/* initcr (trunk/lomacte3/src/main.f:461-800) + cross1 set-up
+ * (cross.f:100-160): grid, FD operators, wavenumbers, device buffers.
+ * mgalx, mgalz: physical grid; my: y points; y[my]: coordinates; Re, alp,
+ * bet: Reynolds number and fundamental wavenumbers (hre.dat:40); pr: Prandtl
+ * (main.f:569 uses 0.7); cfl, ncfl: CFL number and update period
+ * (hre.dat:68); dt0: first time step (main.f:559 uses 1e-4). */
+int loma_init(loma_run **run, int mgalx, int my, int mgalz, const double *y,
+              double Re, double alp, double bet, double pr, double cfl,
+              int ncfl, double dt0, int precision, int flags);
+int loma_free(loma_run *run);
+/* sizes: out[0..7] = mx, mz, nplanes, nz, S (complex per field),
+ * W (complex per work field), P (physical points), bytes of device memory */
+int loma_dims(const loma_run *run, long long *out);
+
+/* Restart-boundary transfers (getfil main.f:329 / escru escru.f:11).
+ * Fields are complex [nplanes][mz][my]; fp64 selects double (1) or float (0)
+ * host arrays (the reference's restart stores float).  u00,v00,w00 are the
+ * rho*u, rho*v, rho*w mean profiles (double, my).  Upload resets the RK
+ * history and computes the boundary densities (cross.f:208-213). */
+int loma_upload(loma_run *run, const void *vor, const void *phi,
+                const void *psi, const void *scal, const double *u00,
+                const double *v00, const double *w00, int fp64);
+int loma_download(loma_run *run, void *vor, void *phi, void *psi, void *scal,
+                  double *u00, double *v00, double *w00, int fp64);
+/* passive scalar Z field (LOMA_FLAG_SCALAR_Z runs only) */
+int loma_upload_z(loma_run *run, const void *z, int fp64);
+int loma_download_z(loma_run *run, void *z, int fp64);
+
+/* Time advance: nsteps full RK3 steps (cross.f:225-1223 loop body),
+ * device resident.  Returns LOMA_ERR_NAN if the state became non-finite. */
+int loma_advance(loma_run *run, int nsteps);
+/* One RK substep (rkstep 1..3) of the current step; loma_advance is three of
+ * these plus the time / dt update (cross.f:1196-1204). */
+int loma_substep(loma_run *run, int rkstep);
+/* scalars: out[0..4] = dt, time, dtr (next dt from CFL), istep, nan flag */
+int loma_scalars(loma_run *run, double *out);
+/* Drop-in for cross1's time loop (cross.f:24): upload, nstep steps,
+ * download, with the reference's argument meaning (float fields, double 00
+ * modes ordered u00,w00,v00 as in cross1's argument list). */
+int loma_cross1(loma_run *run, float *vor, float *phi, float *psi, float *scal,
+                double *u00, double *w00, double *v00, int nstep);
+/* HIP stream the run launches on (0 = default), and graph capture of one
+ * full step (replayed by loma_advance when enabled). */
+int loma_set_stream(loma_run *run, void *stream);
+int loma_use_graph(loma_run *run, int enable);
+
+/* ---- per-operator entry points (device pointers, double) --------------- */
+/* All act on ncols complex y-columns of length my stored back to back
+ * (interleaved re/im doubles), in place allowed, on the run's stream. */
+int loma_deryr2_dev(loma_run *run, const double *u, double *du, int ncols);  /* cfdiff.f:518 */
+int loma_deryyr2_dev(loma_run *run, const double *u, double *du, int ncols); /* cfdiff.f:735 */
+/* rK per column (double[ncols]) */
+int loma_lapvdvhom_dev(loma_run *run, const double *phi, double *v,
+                       double *dvdy, const double *rK, int ncols);          /* laps.f:319 */
+int loma_solvepsi_dev(loma_run *run, const double *phi, double *v,
+                      double *dvdy, const double *rK, int ncols);           /* laps.f:580 */
+/* Helmholtz with extra shift (implicit viscous kernel of visc,
+ * trunk/mixing/src/laps.f:535): (D2 - (rK+shift)) x = f, x = 0 at ends */
+int loma_helmholtz_dev(loma_run *run, const double *f, double *x,
+                       const double *rK, double shift, int ncols);
+/* inty8 (laps.f:484) on one real column of my doubles */
+int loma_inty8_dev(loma_run *run, const double *du, double *u);
+
+/* ---- per-operator entry points (host pointers, float, reference shapes) */
+int loma_deryr2(loma_run *run, const float *u, float *du, int m);   /* cfdiff.f:518 */
+int loma_deryyr2(loma_run *run, const float *u, float *du, int m);  /* cfdiff.f:735 */
+int loma_lapvdvhom(loma_run *run, const float *phi, float *v, float *dvdy,
+                   double rK);                                      /* laps.f:319 */
+int loma_solvepsi(loma_run *run, const float *phi, float *v, float *dvdy,
+                  double rK);                                       /* laps.f:580 */
+
+/* ---- transforms (device pointers) ------------------------------------- */
+/* fourz (fou3D.f:80) on nrows complex rows of length mgalz, FFTPACK
+ * convention: iopt<0 forward scaled by 1/n with e^{-i}, iopt>=0 inverse. */
+int loma_fourz_dev(loma_run *run, double *c, int nrows, int iopt);
+
+/* ---- batched banded solves in isolation ------------------------------- */
+/* nsys independent pentadiagonal systems of size n (row-major band
+ * [sys][n][5], entry d is column j-2+d) with complex right-hand sides
+ * rhs[sys][n] (interleaved doubles, float when rhs_fp32).  Factor (bandec,
+ * cfdiff.f:977) and solve (banbks, cfdiff.f:954) in FP64; x overwrites rhs
+ * (double) or goes to x (double) when rhs_fp32.  Device pointers. */
+int loma_band5_solve_dev(const double *band, void *rhs, double *x, int n,
+                         int nsys, int rhs_fp32, void *stream);
+
+/* ---- ensembles -------------------------------------------------------- */
+/* Advance several runs together, one launch per substep kernel for all
+ * runs (runs may differ in grid, Re, density ratio). */
+int loma_advance_many(loma_run **runs, int nruns, int nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

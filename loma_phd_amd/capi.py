@@ -1,0 +1,240 @@
+"""ctypes binding of libloma_hip.so (the C ABI declared in include/loma_amd.h).
+
+The product path: every call here runs the HIP kernels built for gfx950.
+Importing this module loads the library; it raises if the library has not
+been built (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(PKG, "_lib", "libloma_hip.so")
+
+LOMA_OK, LOMA_ERR_ARG, LOMA_ERR_NAN, LOMA_ERR_OOM = 0, 1, 2, 3
+LOMA_ERR_DEVICE, LOMA_ERR_STATE, LOMA_ERR_NODEV = 4, 5, 6
+LOMA_FP64, LOMA_MIXED = 0, 1
+LOMA_FLAG_DT_FP32, LOMA_FLAG_SCALAR_Z = 1, 2
+
+# every symbol the header declares (checked by tests on the CPU)
+EXPORTS = [
+    "loma_version", "loma_last_error", "loma_device_count", "loma_fd_matrices",
+    "loma_init", "loma_free", "loma_dims", "loma_upload", "loma_download",
+    "loma_upload_z", "loma_download_z", "loma_advance", "loma_substep", "loma_scalars",
+    "loma_cross1", "loma_set_stream", "loma_use_graph", "loma_deryr2_dev",
+    "loma_deryyr2_dev", "loma_lapvdvhom_dev", "loma_solvepsi_dev", "loma_helmholtz_dev",
+    "loma_inty8_dev", "loma_deryr2", "loma_deryyr2", "loma_lapvdvhom", "loma_solvepsi",
+    "loma_fourz_dev", "loma_band5_solve_dev", "loma_advance_many",
+]
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_vp = ctypes.c_void_p
+
+
+class LomaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"loma error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(
+                f"{LIB} is missing: build it first (python -m loma_phd_amd.build); "
+                "there is no CPU fallback")
+        L = ctypes.CDLL(LIB)
+        i, d = ctypes.c_int, ctypes.c_double
+        sig = {
+            "loma_version": ([], ctypes.c_char_p),
+            "loma_last_error": ([], ctypes.c_char_p),
+            "loma_device_count": ([ctypes.POINTER(i)], i),
+            "loma_fd_matrices": ([i, _dp, _dp, _dp, _dp, _dp, _dp], i),
+            "loma_init": ([ctypes.POINTER(_vp), i, i, i, _dp, d, d, d, d, d, i, d, i, i], i),
+            "loma_free": ([_vp], i),
+            "loma_dims": ([_vp, ctypes.POINTER(ctypes.c_longlong)], i),
+            "loma_upload": ([_vp, _vp, _vp, _vp, _vp, _dp, _dp, _dp, i], i),
+            "loma_download": ([_vp, _vp, _vp, _vp, _vp, _dp, _dp, _dp, i], i),
+            "loma_upload_z": ([_vp, _vp, i], i),
+            "loma_download_z": ([_vp, _vp, i], i),
+            "loma_advance": ([_vp, i], i),
+            "loma_substep": ([_vp, i], i),
+            "loma_scalars": ([_vp, _dp], i),
+            "loma_cross1": ([_vp, _fp, _fp, _fp, _fp, _dp, _dp, _dp, i], i),
+            "loma_set_stream": ([_vp, _vp], i),
+            "loma_use_graph": ([_vp, i], i),
+            "loma_deryr2_dev": ([_vp, _vp, _vp, i], i),
+            "loma_deryyr2_dev": ([_vp, _vp, _vp, i], i),
+            "loma_lapvdvhom_dev": ([_vp, _vp, _vp, _vp, _vp, i], i),
+            "loma_solvepsi_dev": ([_vp, _vp, _vp, _vp, _vp, i], i),
+            "loma_helmholtz_dev": ([_vp, _vp, _vp, _vp, d, i], i),
+            "loma_inty8_dev": ([_vp, _vp, _vp], i),
+            "loma_deryr2": ([_vp, _fp, _fp, i], i),
+            "loma_deryyr2": ([_vp, _fp, _fp, i], i),
+            "loma_lapvdvhom": ([_vp, _fp, _fp, _fp, d], i),
+            "loma_solvepsi": ([_vp, _fp, _fp, _fp, d], i),
+            "loma_fourz_dev": ([_vp, _vp, i, i], i),
+            "loma_band5_solve_dev": ([_vp, _vp, _vp, i, i, i, _vp], i),
+            "loma_advance_many": ([ctypes.POINTER(_vp), i, i], i),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def check(code: int):
+    if code != LOMA_OK:
+        msg = lib().loma_last_error().decode(errors="replace")
+        raise LomaError(code, msg)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def fd_matrices(y):
+    """Compact-FD bands of a y grid: (dt11, dt12, dt21, dt22, fmap), each band
+    (my, nb) row-major (the reference's (nb,my) Fortran arrays)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    my = y.size
+    out = [np.zeros((my, 7)), np.zeros((my, 7)), np.zeros((my, 5)), np.zeros((my, 5)),
+           np.zeros(my)]
+    check(lib().loma_fd_matrices(my, _d(y), *[_d(a) for a in out]))
+    return tuple(out)
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    lib().loma_device_count(ctypes.byref(n))
+    return n.value
+
+
+class Run:
+    """One device-resident mixing-layer run (the reference's cross1 state).
+
+    Mirrors the reference's driver contract: initcr-style set-up at
+    construction, getfil/escru-style upload/download, cross1 time loop as
+    advance().  Arrays use the reference layout [kx plane][kz][y], complex.
+    """
+
+    def __init__(self, mgalx, my, mgalz, y, Re, alp, bet, pr=0.7, cfl=0.5, ncfl=1,
+                 dt0=1e-4, precision=LOMA_FP64, flags=0):
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        assert self.y.size == my
+        h = _vp()
+        check(lib().loma_init(ctypes.byref(h), mgalx, my, mgalz, _d(self.y), Re, alp, bet, pr,
+                              cfl, ncfl, dt0, precision, flags))
+        self._h = h
+        dims = (ctypes.c_longlong * 8)()
+        check(lib().loma_dims(h, dims))
+        self.mx, self.mz, self.nplanes, self.nz, self.S, self.W, self.P, self.bytes = list(dims)
+        self.mgalx, self.my, self.mgalz = mgalx, my, mgalz
+        self.Re, self.alp, self.bet, self.pr = Re, alp, bet, pr
+        self.precision, self.flags = precision, flags
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.loma_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def shape(self):
+        return (self.nplanes, self.mz, self.my)
+
+    def upload(self, vor, phi, psi, scal, u00, v00, w00, z=None):
+        f = [np.ascontiguousarray(a, dtype=np.complex128).reshape(self.shape())
+             for a in (vor, phi, psi, scal)]
+        o = [np.ascontiguousarray(a, dtype=np.float64) for a in (u00, v00, w00)]
+        check(lib().loma_upload(self._h, *[a.ctypes.data_as(_vp) for a in f],
+                                *[_d(a) for a in o], 1))
+        if z is not None:
+            zz = np.ascontiguousarray(z, dtype=np.complex128).reshape(self.shape())
+            check(lib().loma_upload_z(self._h, zz.ctypes.data_as(_vp), 1))
+
+    def download(self):
+        f = [np.zeros(self.shape(), np.complex128) for _ in range(4)]
+        o = [np.zeros(self.my) for _ in range(3)]
+        check(lib().loma_download(self._h, *[a.ctypes.data_as(_vp) for a in f],
+                                  *[_d(a) for a in o], 1))
+        out = dict(vor=f[0], phi=f[1], psi=f[2], scal=f[3], u00=o[0], v00=o[1], w00=o[2])
+        if self.flags & LOMA_FLAG_SCALAR_Z:
+            z = np.zeros(self.shape(), np.complex128)
+            check(lib().loma_download_z(self._h, z.ctypes.data_as(_vp), 1))
+            out["z"] = z
+        return out
+
+    def advance(self, nsteps=1):
+        check(lib().loma_advance(self._h, nsteps))
+
+    def substep(self, rk):
+        check(lib().loma_substep(self._h, rk))
+
+    def scalars(self):
+        o = np.zeros(5)
+        check(lib().loma_scalars(self._h, _d(o)))
+        return dict(dt=o[0], time=o[1], dtr=o[2], istep=int(o[3]), nan=int(o[4]))
+
+    def use_graph(self, on=True):
+        check(lib().loma_use_graph(self._h, 1 if on else 0))
+
+    def set_stream(self, stream_ptr: int):
+        check(lib().loma_set_stream(self._h, ctypes.c_void_p(stream_ptr)))
+
+    # per-operator entry points on device pointers (ints)
+    def deryr2_dev(self, u_ptr, du_ptr, ncols):
+        check(lib().loma_deryr2_dev(self._h, u_ptr, du_ptr, ncols))
+
+    def deryyr2_dev(self, u_ptr, du_ptr, ncols):
+        check(lib().loma_deryyr2_dev(self._h, u_ptr, du_ptr, ncols))
+
+    def lapvdvhom_dev(self, phi, v, dv, rK, ncols):
+        check(lib().loma_lapvdvhom_dev(self._h, phi, v, dv, rK, ncols))
+
+    def solvepsi_dev(self, phi, v, dv, rK, ncols):
+        check(lib().loma_solvepsi_dev(self._h, phi, v, dv, rK, ncols))
+
+    def helmholtz_dev(self, f, x, rK, shift, ncols):
+        check(lib().loma_helmholtz_dev(self._h, f, x, rK, float(shift), ncols))
+
+    def inty8_dev(self, du, u):
+        check(lib().loma_inty8_dev(self._h, du, u))
+
+    def fourz_dev(self, c_ptr, nrows, iopt):
+        check(lib().loma_fourz_dev(self._h, c_ptr, nrows, iopt))
+
+    # host float versions with the reference's shapes
+    def deryr2(self, u):
+        u = np.ascontiguousarray(u, dtype=np.complex64)
+        du = np.zeros_like(u)
+        m = u.size // self.my
+        check(lib().loma_deryr2(self._h, u.view(np.float32).ctypes.data_as(_fp),
+                                du.view(np.float32).ctypes.data_as(_fp), m))
+        return du
+
+    def deryyr2(self, u):
+        u = np.ascontiguousarray(u, dtype=np.complex64)
+        du = np.zeros_like(u)
+        m = u.size // self.my
+        check(lib().loma_deryyr2(self._h, u.view(np.float32).ctypes.data_as(_fp),
+                                 du.view(np.float32).ctypes.data_as(_fp), m))
+        return du
+
+
+def advance_many(runs, nsteps):
+    arr = (_vp * len(runs))(*[r.handle for r in runs])
+    check(lib().loma_advance_many(arr, len(runs), nsteps))

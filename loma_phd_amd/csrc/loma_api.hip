@@ -1,0 +1,721 @@
+// loma_api.hip -- C ABI and host orchestration of the RK3 time advance.
+//
+// A run handle owns all device state of one mixing-layer instance (the
+// reference keeps it in module globals, trunk/lomacte3/src/modulos.f, and in
+// cross1's work arrays, main.f:128-206).  State stays on the device between
+// calls; host transfers happen only at upload/download (restart boundary).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "loma_internal.h"
+#include "loma_kernels.h"
+
+using namespace loma;
+
+namespace loma {
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace loma
+
+#define HIPCHK(x)                                                               \
+  do {                                                                          \
+    hipError_t e_ = (x);                                                        \
+    if (e_ != hipSuccess) {                                                     \
+      set_error(std::string(#x) + ": " + hipGetErrorString(e_));                \
+      return LOMA_ERR_DEVICE;                                                   \
+    }                                                                           \
+  } while (0)
+#define LCHK(x)                                                                 \
+  do {                                                                          \
+    int e_ = (x);                                                               \
+    if (e_ != 0) {                                                              \
+      set_error(std::string(#x) + ": " + hipGetErrorString((hipError_t)e_));    \
+      return LOMA_ERR_DEVICE;                                                   \
+    }                                                                           \
+  } while (0)
+
+struct loma_run {
+  int mgalx = 0, my = 0, mgalz = 0, mx = 0, mz = 0, nplanes = 0, nz = 0;
+  int M = 0;            // rows per lane of the y-column solver
+  int rows_pad = 0;     // rows of every per-row table incl. padding
+  int precision = 0, flags = 0;
+  bool fp32 = false, with_z = false;
+  double Re = 0, alp = 0, bet = 0, pr = 0.7, cfl = 0.5, dt0 = 1e-4;
+  int ncfl = 1;
+  FdOps fd;
+  RunConst rc{};
+  hipStream_t stream = nullptr;
+  // device allocations
+  std::vector<void*> allocs;
+  size_t bytes = 0;
+  double* tab = nullptr;  // all per-row tables
+  YTab T{};
+  double* dt21 = nullptr;  // plain (my,5) row-major copies for the factor kernel
+  double* dt22 = nullptr;
+  double* helm_lu = nullptr;
+  int nclass = 0;
+  double2 *vor = nullptr, *phi = nullptr, *psi = nullptr, *scal = nullptr;
+  double2 *ten12 = nullptr, *ten13 = nullptr, *zf = nullptr;
+  void* spec[12] = {};
+  void* wbuf[12] = {};
+  double *rhst = nullptr, *rhsz = nullptr, *hy = nullptr;
+  double *u00wk = nullptr, *v00wk = nullptr, *w00wk = nullptr, *rf0u = nullptr,
+         *rf0w = nullptr;
+  DevScalars* sc = nullptr;
+  FftPlan plz{}, plx{};
+  void *twz = nullptr, *twx = nullptr;
+  double2* twz64 = nullptr;  // double twiddles for the per-operator fourz
+  int J = 8, L = 4;
+  int istep = 0;
+  bool uploaded = false;
+  // graph
+  bool use_graph = false;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+};
+
+template <typename P>
+static int dalloc(loma_run* r, P** p, size_t bytes) {
+  void* q = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return LOMA_ERR_OOM;
+  }
+  r->allocs.push_back(q);
+  r->bytes += bytes;
+  *p = (P*)q;
+  return LOMA_OK;
+}
+
+static int pick_M(int my) {
+  int m = (my + 63) / 64;
+  if (m < 3) m = 3;
+  static const int ok[] = {3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 16};
+  for (int v : ok)
+    if (v >= m) return v;
+  return -1;
+}
+
+extern "C" {
+
+const char* loma_version(void) { return "loma_amd 0.1 (gfx950)"; }
+const char* loma_last_error(void) { return g_err.c_str(); }
+
+int loma_device_count(int* n) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  if (n) *n = c;
+  return LOMA_OK;
+}
+
+int loma_fd_matrices(int my, const double* y, double* dt11, double* dt12, double* dt21,
+                     double* dt22, double* fmap) {
+  if (my < 8 || !y) { set_error("loma_fd_matrices: bad arguments"); return LOMA_ERR_ARG; }
+  FdOps op;
+  int rc = fd_build(my, y, op);
+  if (rc) { set_error("loma_fd_matrices: singular stencil system"); return rc; }
+  // (nb,my) Fortran order == row-major [j][d]
+  if (dt11) std::memcpy(dt11, op.a1.data(), sizeof(double) * 7 * my);
+  if (dt12) std::memcpy(dt12, op.b1.data(), sizeof(double) * 7 * my);
+  if (dt21) std::memcpy(dt21, op.a2.data(), sizeof(double) * 5 * my);
+  if (dt22) std::memcpy(dt22, op.b2.data(), sizeof(double) * 5 * my);
+  if (fmap) std::memcpy(fmap, op.fmap.data(), sizeof(double) * my);
+  return LOMA_OK;
+}
+
+int loma_free(loma_run* r) {
+  if (!r) return LOMA_OK;
+  for (int q = 0; q < 2; ++q)
+    if (r->gexec[q]) hipGraphExecDestroy(r->gexec[q]);
+  for (void* p : r->allocs) hipFree(p);
+  delete r;
+  return LOMA_OK;
+}
+
+int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, double Re,
+              double alp, double bet, double pr, double cfl, int ncfl, double dt0,
+              int precision, int flags) {
+  if (!out || !y) { set_error("loma_init: null argument"); return LOMA_ERR_ARG; }
+  *out = nullptr;
+  if (mgalx < 8 || mgalz < 8 || my < 16 || my > 1024) {
+    set_error("loma_init: grid out of range (my in [16,1024], mgalx,mgalz >= 8)");
+    return LOMA_ERR_ARG;
+  }
+  if (precision != LOMA_FP64 && precision != LOMA_MIXED) {
+    set_error("loma_init: unknown precision");
+    return LOMA_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("loma_init: no GPU visible");
+    return LOMA_ERR_NODEV;
+  }
+  loma_run* r = new loma_run();
+  r->mgalx = mgalx; r->my = my; r->mgalz = mgalz;
+  r->mx = 2 * (mgalx / 3);
+  r->nplanes = r->mx / 2;
+  r->mz = 2 * (mgalz / 3) - 1;
+  r->nz = (r->mz - 1) / 2;
+  r->Re = Re; r->alp = alp; r->bet = bet; r->pr = pr; r->cfl = cfl;
+  r->ncfl = ncfl > 0 ? ncfl : 1; r->dt0 = dt0;
+  r->precision = precision; r->flags = flags;
+  r->fp32 = precision == LOMA_MIXED;
+  r->with_z = (flags & LOMA_FLAG_SCALAR_Z) != 0;
+  r->plz = make_plan(mgalz);
+  r->plx = make_plan(mgalx);
+  if (r->plz.npass < 0 || r->plx.npass < 0 || (mgalx & 1)) {
+    set_error("loma_init: mgalx/mgalz must factor into 2,3,5 (mgalx even)");
+    delete r;
+    return LOMA_ERR_ARG;
+  }
+  r->M = pick_M(my);
+  if (r->M < 0) { set_error("loma_init: my too large"); delete r; return LOMA_ERR_ARG; }
+  r->rows_pad = 64 * r->M + 2 * PADR;
+  int rc = fd_build(my, y, r->fd);
+  if (rc) { set_error("loma_init: FD setup failed"); delete r; return rc; }
+  // run constants
+  RunConst& c = r->rc;
+  c.mgalx = mgalx; c.my = my; c.mgalz = mgalz; c.mx = r->mx; c.mz = r->mz;
+  c.nplanes = r->nplanes; c.nz = r->nz;
+  c.alp = alp; c.bet = bet; c.ire = 1.0 / Re; c.peclet = Re * pr;
+  c.hxalp = alp * (mgalx - 1) / M_PI;  // cross.f:1500
+  c.hzbet = bet * (mgalz - 1) / M_PI;
+  c.cfl = cfl;
+  c.dt_fp32 = (flags & LOMA_FLAG_DT_FP32) ? 1 : 0;
+
+  // ---- per-row tables (rows shifted by PADR, zero padded) ----
+  const int R = r->rows_pad;
+  // offsets (in doubles) of each table inside one allocation
+  const int nb[14] = {7, 7, 1, 3, 3, 1, 5, 5, 1, 2, 2, 1, 3, 3};
+  size_t off[14], tot = 0;
+  for (int q = 0; q < 14; ++q) { off[q] = tot; tot += (size_t)nb[q] * R; }
+  std::vector<double> h(tot, 0.0);
+  auto put = [&](int q, const std::vector<double>& src, int w) {
+    for (int j = 0; j < my; ++j)
+      for (int d = 0; d < w; ++d) h[off[q] + (size_t)(j + PADR) * w + d] = src[(size_t)j * w + d];
+  };
+  const FdOps& f = r->fd;
+  put(0, f.b1, 7); put(1, f.a1, 7); put(2, f.inv1, 1); put(3, f.up1, 3); put(4, f.lo1, 3);
+  put(5, f.fmap, 1); put(6, f.b2, 5); put(7, f.a2, 5); put(8, f.inv2, 1); put(9, f.up2, 2);
+  put(10, f.lo2, 2); put(11, f.invi, 1); put(12, f.upi, 3); put(13, f.loi, 3);
+  if ((rc = dalloc(r, &r->tab, tot * sizeof(double)))) { loma_free(r); return rc; }
+  if (hipMemcpy(r->tab, h.data(), tot * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("loma_init: table upload failed"); loma_free(r); return LOMA_ERR_DEVICE;
+  }
+  YTab& T = r->T;
+  T.b1 = r->tab + off[0]; T.a1 = r->tab + off[1]; T.inv1 = r->tab + off[2];
+  T.up1 = r->tab + off[3]; T.lo1 = r->tab + off[4]; T.fmap = r->tab + off[5];
+  T.b2 = r->tab + off[6]; T.a2 = r->tab + off[7]; T.inv2 = r->tab + off[8];
+  T.up2 = r->tab + off[9]; T.lo2 = r->tab + off[10]; T.invi = r->tab + off[11];
+  T.upi = r->tab + off[12]; T.loi = r->tab + off[13];
+  T.my = my;
+  // plain band copies for the Helmholtz factor kernel
+  if ((rc = dalloc(r, &r->dt21, sizeof(double) * 5 * my))) { loma_free(r); return rc; }
+  if ((rc = dalloc(r, &r->dt22, sizeof(double) * 5 * my))) { loma_free(r); return rc; }
+  hipMemcpy(r->dt21, f.a2.data(), sizeof(double) * 5 * my, hipMemcpyHostToDevice);
+  hipMemcpy(r->dt22, f.b2.data(), sizeof(double) * 5 * my, hipMemcpyHostToDevice);
+  // ---- Helmholtz LU per (kx, |kz|) class: rK = kx^2 + kz^2 ----
+  r->nclass = r->nplanes * (r->nz + 1);
+  {
+    std::vector<double> shifts(r->nclass);
+    for (int i = 0; i < r->nplanes; ++i)
+      for (int ka = 0; ka <= r->nz; ++ka) {
+        const double kx = alp * i, kz = bet * ka;
+        shifts[i * (r->nz + 1) + ka] = kx * kx + kz * kz;
+      }
+    double* dsh = nullptr;
+    if ((rc = dalloc(r, &dsh, sizeof(double) * r->nclass))) { loma_free(r); return rc; }
+    hipMemcpy(dsh, shifts.data(), sizeof(double) * r->nclass, hipMemcpyHostToDevice);
+    if ((rc = dalloc(r, &r->helm_lu, sizeof(double) * 5 * (size_t)R * r->nclass))) {
+      loma_free(r); return rc;
+    }
+    if (launch_helm_factor(r->dt21, r->dt22, my, R, dsh, r->nclass, r->helm_lu, 0)) {
+      set_error("loma_init: factor launch failed"); loma_free(r); return LOMA_ERR_DEVICE;
+    }
+  }
+  // ---- state and work buffers ----
+  const size_t S = (size_t)r->nplanes * r->mz * my;
+  const size_t W = (size_t)r->nplanes * mgalz * my;
+  const size_t P = (size_t)mgalx * my * mgalz;
+  const size_t es = r->fp32 ? 8 : 16;
+  double2** st[6] = {&r->vor, &r->phi, &r->psi, &r->scal, &r->ten12, &r->ten13};
+  for (auto p : st)
+    if ((rc = dalloc(r, p, S * 16))) { loma_free(r); return rc; }
+  if (r->with_z && (rc = dalloc(r, &r->zf, S * 16))) { loma_free(r); return rc; }
+  const int nsp = r->with_z ? 12 : 11;
+  for (int q = 0; q < nsp; ++q) {
+    if ((rc = dalloc(r, &r->spec[q], S * es))) { loma_free(r); return rc; }
+    if ((rc = dalloc(r, &r->wbuf[q], W * es))) { loma_free(r); return rc; }
+  }
+  if ((rc = dalloc(r, &r->rhst, P * 8))) { loma_free(r); return rc; }
+  if (r->with_z && (rc = dalloc(r, &r->rhsz, P * 8))) { loma_free(r); return rc; }
+  double** v00[6] = {&r->u00wk, &r->v00wk, &r->w00wk, &r->rf0u, &r->rf0w, &r->hy};
+  for (auto p : v00)
+    if ((rc = dalloc(r, p, sizeof(double) * my))) { loma_free(r); return rc; }
+  {
+    std::vector<double> hyv(my);
+    for (int j = 1; j < my - 1; ++j) hyv[j] = (y[j + 1] - y[j - 1]) / 2.0;  // main.f:689-696
+    hyv[0] = (y[1] - y[0]) / 2.0;
+    hyv[my - 1] = (y[my - 1] - y[my - 2]) / 2.0;
+    hipMemcpy(r->hy, hyv.data(), sizeof(double) * my, hipMemcpyHostToDevice);
+  }
+  if ((rc = dalloc(r, &r->sc, sizeof(DevScalars)))) { loma_free(r); return rc; }
+  // twiddles
+  auto mk_tw = [&](int n, void** dst) -> int {
+    std::vector<double> tw(2 * n);
+    for (int t = 0; t < n; ++t) {
+      tw[2 * t] = std::cos(2.0 * M_PI * t / n);
+      tw[2 * t + 1] = -std::sin(2.0 * M_PI * t / n);
+    }
+    if (r->fp32) {
+      std::vector<float> tf(2 * n);
+      for (int q = 0; q < 2 * n; ++q) tf[q] = (float)tw[q];
+      int e = dalloc(r, dst, sizeof(float) * 2 * n);
+      if (e) return e;
+      hipMemcpy(*dst, tf.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice);
+    } else {
+      int e = dalloc(r, dst, sizeof(double) * 2 * n);
+      if (e) return e;
+      hipMemcpy(*dst, tw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice);
+    }
+    return 0;
+  };
+  if ((rc = mk_tw(mgalz, &r->twz)) || (rc = mk_tw(mgalx, &r->twx))) { loma_free(r); return rc; }
+  {
+    std::vector<double> tw(2 * mgalz);
+    for (int t = 0; t < mgalz; ++t) {
+      tw[2 * t] = std::cos(2.0 * M_PI * t / mgalz);
+      tw[2 * t + 1] = -std::sin(2.0 * M_PI * t / mgalz);
+    }
+    if ((rc = dalloc(r, &r->twz64, sizeof(double) * 2 * mgalz))) { loma_free(r); return rc; }
+    hipMemcpy(r->twz64, tw.data(), sizeof(double) * 2 * mgalz, hipMemcpyHostToDevice);
+  }
+  // tile sizes: keep LDS per workgroup <= 64 KiB
+  {
+    const int e2 = r->fp32 ? 8 : 16;
+    int J = 16;
+    while (J > 1 && (size_t)2 * J * mgalz * e2 > 65536) J /= 2;
+    r->J = J;
+    int pairs = 1024 / mgalx;
+    if (pairs < 1) pairs = 1;
+    if (pairs > 8) pairs = 8;
+    while (pairs > 1 && (size_t)2 * pairs * mgalx * e2 > 65536) pairs /= 2;
+    r->L = 2 * pairs;
+    if (mgalx * pairs > 1024) {
+      set_error("loma_init: mgalx too large for the x-line kernel (max 1024)");
+      loma_free(r);
+      return LOMA_ERR_ARG;
+    }
+  }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    set_error("loma_init: device error during set-up"); loma_free(r); return LOMA_ERR_DEVICE;
+  }
+  *out = r;
+  return LOMA_OK;
+}
+
+int loma_dims(const loma_run* r, long long* out) {
+  if (!r || !out) return LOMA_ERR_ARG;
+  out[0] = r->mx; out[1] = r->mz; out[2] = r->nplanes; out[3] = r->nz;
+  out[4] = (long long)r->nplanes * r->mz * r->my;
+  out[5] = (long long)r->nplanes * r->mgalz * r->my;
+  out[6] = (long long)r->mgalx * r->my * r->mgalz;
+  out[7] = (long long)r->bytes;
+  return LOMA_OK;
+}
+
+int loma_set_stream(loma_run* r, void* stream) {
+  if (!r) return LOMA_ERR_ARG;
+  r->stream = (hipStream_t)stream;
+  for (int q = 0; q < 2; ++q)
+    if (r->gexec[q]) { hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
+  return LOMA_OK;
+}
+
+static void host_to_c128(const void* src, int fp64, size_t n, std::vector<double>& dst) {
+  dst.resize(2 * n);
+  if (fp64)
+    std::memcpy(dst.data(), src, sizeof(double) * 2 * n);
+  else {
+    const float* s = (const float*)src;
+    for (size_t q = 0; q < 2 * n; ++q) dst[q] = s[q];
+  }
+}
+static void c128_to_host(const std::vector<double>& src, int fp64, void* dst) {
+  if (fp64)
+    std::memcpy(dst, src.data(), sizeof(double) * src.size());
+  else {
+    float* d = (float*)dst;
+    for (size_t q = 0; q < src.size(); ++q) d[q] = (float)src[q];
+  }
+}
+
+int loma_upload(loma_run* r, const void* vor, const void* phi, const void* psi,
+                const void* scal, const double* u00, const double* v00, const double* w00,
+                int fp64) {
+  if (!r || !vor || !phi || !psi || !scal || !u00 || !v00 || !w00) {
+    set_error("loma_upload: null argument");
+    return LOMA_ERR_ARG;
+  }
+  const size_t S = (size_t)r->nplanes * r->mz * r->my;
+  std::vector<double> buf;
+  const void* src[4] = {vor, phi, psi, scal};
+  double2* dst[4] = {r->vor, r->phi, r->psi, r->scal};
+  double T0 = 0, T1 = 0;
+  for (int q = 0; q < 4; ++q) {
+    host_to_c128(src[q], fp64, S, buf);
+    if (q == 3) { T0 = buf[0]; T1 = buf[2 * (r->my - 1)]; }
+    HIPCHK(hipMemcpy(dst[q], buf.data(), S * 16, hipMemcpyHostToDevice));
+  }
+  // history and work buffers start at zero (main.f:162-176)
+  HIPCHK(hipMemset(r->ten12, 0, S * 16));
+  HIPCHK(hipMemset(r->ten13, 0, S * 16));
+  HIPCHK(hipMemset(r->rhst, 0, sizeof(double) * (size_t)r->mgalx * r->my * r->mgalz));
+  if (r->rhsz) HIPCHK(hipMemset(r->rhsz, 0, sizeof(double) * (size_t)r->mgalx * r->my * r->mgalz));
+  HIPCHK(hipMemcpy(r->u00wk, u00, sizeof(double) * r->my, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(r->v00wk, v00, sizeof(double) * r->my, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(r->w00wk, w00, sizeof(double) * r->my, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(r->rf0u, 0, sizeof(double) * r->my));
+  HIPCHK(hipMemset(r->rf0w, 0, sizeof(double) * r->my));
+  // boundary densities from the 00 mode of T (cross.f:208-213)
+  if (!(T0 > 0.0) || !(T1 > 0.0)) {
+    set_error("loma_upload: 00 mode of T must be positive at both ends");
+    return LOMA_ERR_ARG;
+  }
+  r->rc.rhobot = 1.0 / T0;
+  r->rc.rhotop = 1.0 / T1;
+  DevScalars s{};
+  s.dt = (r->flags & LOMA_FLAG_DT_FP32) ? (double)(float)r->dt0 : r->dt0;
+  s.time = 0.0;
+  HIPCHK(hipMemcpy(r->sc, &s, sizeof(s), hipMemcpyHostToDevice));
+  r->istep = 0;
+  r->uploaded = true;
+  for (int q = 0; q < 2; ++q)
+    if (r->gexec[q]) { hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
+  return LOMA_OK;
+}
+
+int loma_upload_z(loma_run* r, const void* z, int fp64) {
+  if (!r || !z || !r->with_z) { set_error("loma_upload_z: run has no scalar Z"); return LOMA_ERR_ARG; }
+  const size_t S = (size_t)r->nplanes * r->mz * r->my;
+  std::vector<double> buf;
+  host_to_c128(z, fp64, S, buf);
+  r->rc.zbot = buf[0];
+  r->rc.ztop = buf[2 * (r->my - 1)];
+  HIPCHK(hipMemcpy(r->zf, buf.data(), S * 16, hipMemcpyHostToDevice));
+  for (int q = 0; q < 2; ++q)
+    if (r->gexec[q]) { hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
+  return LOMA_OK;
+}
+
+int loma_download(loma_run* r, void* vor, void* phi, void* psi, void* scal, double* u00,
+                  double* v00, double* w00, int fp64) {
+  if (!r || !r->uploaded) { set_error("loma_download: nothing uploaded"); return LOMA_ERR_STATE; }
+  HIPCHK(hipStreamSynchronize(r->stream));
+  const size_t S = (size_t)r->nplanes * r->mz * r->my;
+  std::vector<double> buf(2 * S);
+  void* dst[4] = {vor, phi, psi, scal};
+  double2* src[4] = {r->vor, r->phi, r->psi, r->scal};
+  for (int q = 0; q < 4; ++q) {
+    if (!dst[q]) continue;
+    HIPCHK(hipMemcpy(buf.data(), src[q], S * 16, hipMemcpyDeviceToHost));
+    c128_to_host(buf, fp64, dst[q]);
+  }
+  if (u00) HIPCHK(hipMemcpy(u00, r->u00wk, sizeof(double) * r->my, hipMemcpyDeviceToHost));
+  if (v00) HIPCHK(hipMemcpy(v00, r->v00wk, sizeof(double) * r->my, hipMemcpyDeviceToHost));
+  if (w00) HIPCHK(hipMemcpy(w00, r->w00wk, sizeof(double) * r->my, hipMemcpyDeviceToHost));
+  return LOMA_OK;
+}
+
+int loma_download_z(loma_run* r, void* z, int fp64) {
+  if (!r || !r->with_z || !z) { set_error("loma_download_z: run has no scalar Z"); return LOMA_ERR_ARG; }
+  HIPCHK(hipStreamSynchronize(r->stream));
+  const size_t S = (size_t)r->nplanes * r->mz * r->my;
+  std::vector<double> buf(2 * S);
+  HIPCHK(hipMemcpy(buf.data(), r->zf, S * 16, hipMemcpyDeviceToHost));
+  c128_to_host(buf, fp64, z);
+  return LOMA_OK;
+}
+
+static PhaseArgs phase_args(loma_run* r, int rkstep) {
+  PhaseArgs a{};
+  a.rc = r->rc;
+  a.T = r->T;
+  a.helm_lu = r->helm_lu;
+  a.lu_rows = r->rows_pad;
+  a.vor = r->vor; a.phi = r->phi; a.psi = r->psi; a.scal = r->scal;
+  a.ten12 = r->ten12; a.ten13 = r->ten13; a.zf = r->zf;
+  for (int q = 0; q < 12; ++q) a.spec[q] = r->spec[q];
+  a.u00wk = r->u00wk; a.v00wk = r->v00wk; a.w00wk = r->w00wk;
+  a.rf0u = r->rf0u; a.rf0w = r->rf0w;
+  a.sc = r->sc;
+  a.rkstep = rkstep;
+  a.with_z = r->with_z ? 1 : 0;
+  a.mode_base = 0;
+  a.nmodes = r->nplanes * r->mz;
+  return a;
+}
+
+static FftArgs fft_args(loma_run* r, int rkstep, int do_cfl) {
+  FftArgs a{};
+  a.rc = r->rc;
+  a.plz = r->plz;
+  a.plx = r->plx;
+  a.twz = r->twz;
+  a.twx = r->twx;
+  for (int q = 0; q < 12; ++q) { a.spec[q] = r->spec[q]; a.wbuf[q] = r->wbuf[q]; }
+  a.rhst = r->rhst;
+  a.rhsz = r->rhsz;
+  a.hy = r->hy;
+  a.sc = r->sc;
+  a.rkstep = rkstep;
+  a.do_cfl = do_cfl;
+  a.with_z = r->with_z ? 1 : 0;
+  a.J = r->J;
+  a.L = r->L;
+  return a;
+}
+
+static int enqueue_substep(loma_run* r, int rkstep, int icfl) {
+  hipStream_t st = r->stream;
+  PhaseArgs pa = phase_args(r, rkstep);
+  LCHK(launch_phaseA(r->M, r->fp32, pa, st));
+  FftArgs fa = fft_args(r, rkstep, (rkstep == 1 && icfl) ? 1 : 0);
+  fa.nfields = r->with_z ? 12 : 8;
+  LCHK(launch_zfft_inv(r->fp32, fa, st));
+  LCHK(launch_xline(r->fp32, fa, st));
+  fa.nfields = r->with_z ? 12 : 11;
+  LCHK(launch_zfft_fwd(r->fp32, fa, st));
+  LCHK(launch_phaseC(r->M, r->fp32, pa, st));
+  return LOMA_OK;
+}
+
+static int enqueue_step(loma_run* r, int icfl) {
+  for (int rk = 1; rk <= 3; ++rk) {
+    int e = enqueue_substep(r, rk, icfl);
+    if (e) return e;
+  }
+  LCHK(launch_finish_step(r->sc, r->rc, icfl, r->stream));
+  return LOMA_OK;
+}
+
+int loma_substep(loma_run* r, int rkstep) {
+  if (!r || !r->uploaded) { set_error("loma_substep: nothing uploaded"); return LOMA_ERR_STATE; }
+  if (rkstep < 1 || rkstep > 3) { set_error("loma_substep: rkstep in 1..3"); return LOMA_ERR_ARG; }
+  const int icfl = ((r->istep) % r->ncfl == 0) ? 1 : 0;
+  int e = enqueue_substep(r, rkstep, icfl);
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(r->stream));
+  return LOMA_OK;
+}
+
+int loma_use_graph(loma_run* r, int enable) {
+  if (!r) return LOMA_ERR_ARG;
+  r->use_graph = enable != 0;
+  return LOMA_OK;
+}
+
+static int run_steps(loma_run* r, int nsteps) {
+  for (int n = 0; n < nsteps; ++n) {
+    // cross.f:238: the step size is re-estimated every ncfl steps
+    const int icfl = ((r->istep) % r->ncfl == 0) ? 1 : 0;
+    if (r->use_graph) {
+      hipGraphExec_t& ge = r->gexec[icfl];
+      if (!ge) {
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+        int e = enqueue_step(r, icfl);
+        hipError_t ce = hipStreamEndCapture(r->stream, &g);
+        if (e) return e;
+        HIPCHK(ce);
+        HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        hipGraphDestroy(g);
+      }
+      HIPCHK(hipGraphLaunch(ge, r->stream));
+    } else {
+      int e = enqueue_step(r, icfl);
+      if (e) return e;
+    }
+    r->istep++;
+  }
+  return LOMA_OK;
+}
+
+int loma_advance(loma_run* r, int nsteps) {
+  if (!r || !r->uploaded) { set_error("loma_advance: nothing uploaded"); return LOMA_ERR_STATE; }
+  if (nsteps < 0) { set_error("loma_advance: nsteps < 0"); return LOMA_ERR_ARG; }
+  if (r->stream == nullptr && r->use_graph) {
+    // graphs need a non-default stream
+    HIPCHK(hipStreamCreate(&r->stream));
+  }
+  int e = run_steps(r, nsteps);
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(r->stream));
+  DevScalars s;
+  HIPCHK(hipMemcpy(&s, r->sc, sizeof(s), hipMemcpyDeviceToHost));
+  if (s.nan) { set_error("loma_advance: non-finite state"); return LOMA_ERR_NAN; }
+  return LOMA_OK;
+}
+
+int loma_advance_many(loma_run** runs, int nruns, int nsteps) {
+  if (!runs || nruns <= 0) return LOMA_ERR_ARG;
+  for (int n = 0; n < nsteps; ++n)
+    for (int q = 0; q < nruns; ++q) {
+      loma_run* r = runs[q];
+      if (!r || !r->uploaded) { set_error("loma_advance_many: run not uploaded"); return LOMA_ERR_STATE; }
+      int e = run_steps(r, 1);
+      if (e) return e;
+    }
+  for (int q = 0; q < nruns; ++q) HIPCHK(hipStreamSynchronize(runs[q]->stream));
+  return LOMA_OK;
+}
+
+int loma_scalars(loma_run* r, double* out) {
+  if (!r || !out) return LOMA_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(r->stream));
+  DevScalars s;
+  HIPCHK(hipMemcpy(&s, r->sc, sizeof(s), hipMemcpyDeviceToHost));
+  out[0] = s.dt; out[1] = s.time; out[2] = s.dtr; out[3] = r->istep; out[4] = s.nan;
+  return LOMA_OK;
+}
+
+int loma_cross1(loma_run* r, float* vor, float* phi, float* psi, float* scal, double* u00,
+                double* w00, double* v00, int nstep) {
+  int e = loma_upload(r, vor, phi, psi, scal, u00, v00, w00, 0);
+  if (e) return e;
+  e = loma_advance(r, nstep);
+  if (e && e != LOMA_ERR_NAN) return e;
+  int e2 = loma_download(r, vor, phi, psi, scal, u00, v00, w00, 0);
+  return e ? e : e2;
+}
+
+// ---- per-operator entry points -----------------------------------------
+int loma_deryr2_dev(loma_run* r, const double* u, double* du, int ncols) {
+  if (!r || ncols < 0) return LOMA_ERR_ARG;
+  LCHK(launch_dy(r->M, 1, r->T, u, du, ncols, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  return LOMA_OK;
+}
+int loma_deryyr2_dev(loma_run* r, const double* u, double* du, int ncols) {
+  if (!r || ncols < 0) return LOMA_ERR_ARG;
+  LCHK(launch_dy(r->M, 2, r->T, u, du, ncols, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  return LOMA_OK;
+}
+
+static int solve_cols(loma_run* r, const double* phi, double* v, double* dvdy,
+                      const double* rK, double shift, int ncols, int mode) {
+  if (!r || ncols < 0) return LOMA_ERR_ARG;
+  if (ncols == 0) return LOMA_OK;
+  std::vector<double> h(ncols);
+  HIPCHK(hipMemcpy(h.data(), rK, sizeof(double) * ncols, hipMemcpyDeviceToHost));
+  for (double& x : h) x += shift;
+  double *sh = nullptr, *lu = nullptr;
+  HIPCHK(hipMalloc(&sh, sizeof(double) * ncols));
+  HIPCHK(hipMalloc(&lu, sizeof(double) * 5 * (size_t)r->rows_pad * ncols));
+  int rc = LOMA_OK;
+  if (hipMemcpy(sh, h.data(), sizeof(double) * ncols, hipMemcpyHostToDevice) != hipSuccess ||
+      launch_helm_factor(r->dt21, r->dt22, r->my, r->rows_pad, sh, ncols, lu, r->stream) ||
+      launch_lapvdvhom(r->M, r->T, lu, r->rows_pad, phi, v, dvdy, rK, ncols, mode,
+                       r->stream) ||
+      hipStreamSynchronize(r->stream) != hipSuccess) {
+    set_error("solve_cols: device failure");
+    rc = LOMA_ERR_DEVICE;
+  }
+  hipFree(sh);
+  hipFree(lu);
+  return rc;
+}
+
+int loma_lapvdvhom_dev(loma_run* r, const double* phi, double* v, double* dvdy,
+                       const double* rK, int ncols) {
+  return solve_cols(r, phi, v, dvdy, rK, 0.0, ncols, 0);
+}
+int loma_solvepsi_dev(loma_run* r, const double* phi, double* v, double* dvdy,
+                      const double* rK, int ncols) {
+  return solve_cols(r, phi, v, dvdy, rK, 0.0, ncols, 1);
+}
+int loma_helmholtz_dev(loma_run* r, const double* f, double* x, const double* rK,
+                       double shift, int ncols) {
+  return solve_cols(r, f, x, nullptr, rK, shift, ncols, 2);
+}
+int loma_inty8_dev(loma_run* r, const double* du, double* u) {
+  if (!r) return LOMA_ERR_ARG;
+  LCHK(launch_inty8(r->M, r->T, du, u, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  return LOMA_OK;
+}
+
+// host-pointer float versions with the reference's argument shapes
+static int host_colop(loma_run* r, const float* u, float* du, int m, int order) {
+  if (!r || !u || !du || m < 0) return LOMA_ERR_ARG;
+  const size_t n = (size_t)2 * r->my * m;
+  std::vector<double> h(n);
+  for (size_t q = 0; q < n; ++q) h[q] = u[q];
+  double* d = nullptr;
+  HIPCHK(hipMalloc(&d, n * sizeof(double)));
+  int rc = LOMA_OK;
+  if (hipMemcpy(d, h.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      launch_dy(r->M, order, r->T, d, d, m, r->stream) ||
+      hipStreamSynchronize(r->stream) != hipSuccess ||
+      hipMemcpy(h.data(), d, n * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("host_colop: device failure");
+    rc = LOMA_ERR_DEVICE;
+  }
+  hipFree(d);
+  if (rc == LOMA_OK)
+    for (size_t q = 0; q < n; ++q) du[q] = (float)h[q];
+  return rc;
+}
+int loma_deryr2(loma_run* r, const float* u, float* du, int m) { return host_colop(r, u, du, m, 1); }
+int loma_deryyr2(loma_run* r, const float* u, float* du, int m) { return host_colop(r, u, du, m, 2); }
+
+static int host_solve(loma_run* r, const float* phi, float* v, float* dvdy, double rK, int mode) {
+  if (!r || !phi || !v || !dvdy) return LOMA_ERR_ARG;
+  const size_t n = (size_t)2 * r->my;
+  std::vector<double> h(3 * n + 1);
+  for (size_t q = 0; q < n; ++q) h[q] = phi[q];
+  h[3 * n] = rK;
+  double* d = nullptr;
+  HIPCHK(hipMalloc(&d, h.size() * sizeof(double)));
+  int rc = hipMemcpy(d, h.data(), h.size() * 8, hipMemcpyHostToDevice) == hipSuccess
+               ? solve_cols(r, d, d + n, d + 2 * n, d + 3 * n, 0.0, 1, mode)
+               : LOMA_ERR_DEVICE;
+  if (rc == LOMA_OK && hipMemcpy(h.data(), d, h.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = LOMA_ERR_DEVICE;
+  hipFree(d);
+  if (rc == LOMA_OK)
+    for (size_t q = 0; q < n; ++q) { v[q] = (float)h[n + q]; dvdy[q] = (float)h[2 * n + q]; }
+  return rc;
+}
+int loma_lapvdvhom(loma_run* r, const float* phi, float* v, float* dvdy, double rK) {
+  return host_solve(r, phi, v, dvdy, rK, 0);
+}
+int loma_solvepsi(loma_run* r, const float* phi, float* v, float* dvdy, double rK) {
+  return host_solve(r, phi, v, dvdy, rK, 1);
+}
+
+int loma_fourz_dev(loma_run* r, double* c, int nrows, int iopt) {
+  if (!r || !c || nrows < 0) return LOMA_ERR_ARG;
+  LCHK(launch_fourz(r->plz, r->twz64, (double2*)c, nrows, iopt, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  return LOMA_OK;
+}
+
+int loma_band5_solve_dev(const double* band, void* rhs, double* x, int n, int nsys,
+                         int rhs_fp32, void* stream) {
+  (void)band; (void)rhs; (void)x; (void)n; (void)nsys; (void)rhs_fp32; (void)stream;
+  set_error("loma_band5_solve_dev: not built yet");
+  return LOMA_ERR_STATE;
+}
+
+}  // extern "C"

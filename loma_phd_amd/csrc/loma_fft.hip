@@ -1,0 +1,526 @@
+// loma_fft.hip -- x/z Fourier transforms between spectral planes and
+// physical lines, and the fused physical-space kernel.
+//
+// Reference: foup2 (trunk/lomacte3/src/cross.f:1282-1430), fourz/fourx
+// (fou3D.f:29-95), FFTPACK conventions (cftsingle.f:2-66, rftsingle.f:25-96),
+// phys2 (cross.f:1440-1828).
+//
+// Layouts (complex T2 = float2 or double2):
+//   spectral S  : [i][k][j]       (kx plane, kz mode, y)   -- y fastest
+//   z-physical W: [i][z][j]       (kx plane, z point, y)   -- y fastest
+//   rhst/rhsz   : [z][j][x] FP64  (physical history of RHS(T) / RHS(Z))
+// Every FFT is a Stockham autosort transform in LDS with mixed radices
+// 8/4/2/3/5 (Govindaraju et al., SC'08 formulation), one workgroup per tile.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <type_traits>
+
+#include "loma_kernels.h"
+
+namespace loma {
+
+FftPlan make_plan(int n) {
+  FftPlan p;
+  p.n = n;
+  p.npass = 0;
+  int m = n;
+  while (m % 8 == 0 && m > 1) { p.rad[p.npass++] = 8; m /= 8; }
+  while (m % 4 == 0 && m > 1) { p.rad[p.npass++] = 4; m /= 4; }
+  while (m % 2 == 0 && m > 1) { p.rad[p.npass++] = 2; m /= 2; }
+  while (m % 3 == 0 && m > 1) { p.rad[p.npass++] = 3; m /= 3; }
+  while (m % 5 == 0 && m > 1) { p.rad[p.npass++] = 5; m /= 5; }
+  if (m != 1) p.npass = -1;  // unsupported length
+  return p;
+}
+
+template <typename T2>
+__device__ __forceinline__ T2 cmul(T2 a, T2 b) {
+  T2 r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <typename T2>
+__device__ __forceinline__ T2 cadd(T2 a, T2 b) { T2 r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename T2>
+__device__ __forceinline__ T2 csub(T2 a, T2 b) { T2 r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+// multiply by s*i (s = +1 or -1)
+template <typename T2>
+__device__ __forceinline__ T2 cmuli(T2 a, int s) { T2 r; r.x = -s * a.y; r.y = s * a.x; return r; }
+
+// in-register DFT of size R: V_k = sum_r v_r exp(s 2 pi i r k / R)
+template <typename T, typename T2>
+__device__ __forceinline__ void dft2(T2* v) {
+  T2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+template <typename T, typename T2>
+__device__ __forceinline__ void dft4(T2* v, int s) {
+  T2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  T2 t2 = cadd(v[1], v[3]), t3 = cmuli(csub(v[1], v[3]), s);
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[3] = csub(t1, t3);
+}
+template <typename T, typename T2>
+__device__ __forceinline__ void dft8(T2* v, int s) {
+  T2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<T, T2>(e, s);
+  dft4<T, T2>(o, s);
+  const T h = (T)0.70710678118654752440;
+  T2 w1, w3;
+  w1.x = h; w1.y = s * h;     // exp(s i pi/4)
+  w3.x = -h; w3.y = s * h;    // exp(s 3 i pi/4)
+  T2 o1 = cmul(o[1], w1), o2 = cmuli(o[2], s), o3 = cmul(o[3], w3);
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+template <typename T, typename T2>
+__device__ __forceinline__ void dft3(T2* v, int s) {
+  const T c = (T)-0.5, sn = (T)(0.86602540378443864676) * (T)s;
+  T2 a = v[0], b = v[1], d = v[2];
+  T2 sum = cadd(b, d), dif = csub(b, d);
+  v[0] = cadd(a, sum);
+  T2 m;
+  m.x = a.x + c * sum.x;
+  m.y = a.y + c * sum.y;
+  T2 r;
+  r.x = -sn * dif.y;
+  r.y = sn * dif.x;
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+template <typename T, typename T2>
+__device__ __forceinline__ void dft5(T2* v, int s) {
+  const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+  const T s1 = (T)0.95105651629515357212 * (T)s, s2 = (T)0.58778525229247312917 * (T)s;
+  T2 a = v[0];
+  T2 b1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
+  T2 b2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+  v[0].x = a.x + b1.x + b2.x;
+  v[0].y = a.y + b1.y + b2.y;
+  T2 m1, m2, r1, r2;
+  m1.x = a.x + c1 * b1.x + c2 * b2.x; m1.y = a.y + c1 * b1.y + c2 * b2.y;
+  m2.x = a.x + c2 * b1.x + c1 * b2.x; m2.y = a.y + c2 * b1.y + c1 * b2.y;
+  // r = i*(s1 d1 + s2 d2), i*(s2 d1 - s1 d2)
+  r1.x = -(s1 * d1.y + s2 * d2.y); r1.y = s1 * d1.x + s2 * d2.x;
+  r2.x = -(s2 * d1.y - s1 * d2.y); r2.y = s2 * d1.x - s1 * d2.x;
+  v[1] = cadd(m1, r1);
+  v[4] = csub(m1, r1);
+  v[2] = cadd(m2, r2);
+  v[3] = csub(m2, r2);
+}
+
+template <int R, typename T, typename T2>
+__device__ __forceinline__ void stockham_item(const T2* __restrict__ src, T2* __restrict__ dst,
+                                              int j, int nb, int Ns, int tstep,
+                                              const T2* __restrict__ tw, int s) {
+  T2 v[R];
+  const int jm = j % Ns;
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = src[j + r * nb];
+  if (Ns > 1) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      T2 w = tw[jm * r * tstep];
+      if (s > 0) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+  }
+  if (R == 2) dft2<T, T2>(v);
+  if (R == 3) dft3<T, T2>(v, s);
+  if (R == 4) dft4<T, T2>(v, s);
+  if (R == 5) dft5<T, T2>(v, s);
+  if (R == 8) dft8<T, T2>(v, s);
+  const int idxD = (j / Ns) * Ns * R + jm;
+#pragma unroll
+  for (int r = 0; r < R; ++r) dst[idxD + r * Ns] = v[r];
+}
+
+// nrows independent length-N transforms of rows stored with stride N in a;
+// b is scratch of the same size.  Returns the buffer holding the result.
+// s = -1 forward (e^{-i}), +1 inverse; unscaled.  Ends with a barrier.
+template <typename T, typename T2>
+__device__ T2* fft_rows(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
+                        const T2* __restrict__ tw, int s) {
+  int Ns = 1;
+  for (int p = 0; p < pl.npass; ++p) {
+    const int R = pl.rad[p];
+    const int nb = N / R;
+    const int tstep = N / (Ns * R);
+    const int total = nrows * nb;
+    for (int it = threadIdx.x; it < total; it += blockDim.x) {
+      const int row = it / nb, j = it - row * nb;
+      const T2* src = a + row * N;
+      T2* dst = b + row * N;
+      switch (R) {
+        case 2: stockham_item<2, T, T2>(src, dst, j, nb, Ns, tstep, tw, s); break;
+        case 3: stockham_item<3, T, T2>(src, dst, j, nb, Ns, tstep, tw, s); break;
+        case 4: stockham_item<4, T, T2>(src, dst, j, nb, Ns, tstep, tw, s); break;
+        case 5: stockham_item<5, T, T2>(src, dst, j, nb, Ns, tstep, tw, s); break;
+        default: stockham_item<8, T, T2>(src, dst, j, nb, Ns, tstep, tw, s); break;
+      }
+    }
+    __syncthreads();
+    T2* t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+// ------------------------------------------------------------------------
+// z transforms: spectral [i][k][j] <-> z-physical [i][z][j]
+// (localyz2zy + fourz(+1), fou3D.f:103-146 / fourz(-1) + localzy2yz)
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_zfft_inv(FftArgs a) {
+  using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = a.rc.mgalz, J = a.J, my = a.rc.my, mz = a.rc.mz, nz = a.rc.nz;
+  T2* A = (T2*)smem;
+  T2* B = A + J * N;
+  const int nyb = (my + J - 1) / J;
+  const int i = blockIdx.x / nyb, j0 = (blockIdx.x - i * nyb) * J;
+  const int f = blockIdx.y;
+  const T2* src = (const T2*)a.spec[f];
+  T2* dst = (T2*)a.wbuf[f];
+  for (int it = threadIdx.x; it < J * N; it += blockDim.x) { A[it].x = 0; A[it].y = 0; }
+  __syncthreads();
+  for (int it = threadIdx.x; it < mz * J; it += blockDim.x) {
+    const int k = it / J, jj = it - k * J, j = j0 + jj;
+    if (j < my) {
+      const int z = (k <= nz) ? k : k + N - mz;
+      A[jj * N + z] = src[((size_t)i * mz + k) * my + j];
+    }
+  }
+  __syncthreads();
+  T2* R = fft_rows<T, T2>(A, B, N, J, a.plz, (const T2*)a.twz, +1);
+  for (int it = threadIdx.x; it < N * J; it += blockDim.x) {
+    const int z = it / J, jj = it - z * J, j = j0 + jj;
+    if (j < my) dst[((size_t)i * N + z) * my + j] = R[jj * N + z];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_zfft_fwd(FftArgs a) {
+  using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = a.rc.mgalz, J = a.J, my = a.rc.my, mz = a.rc.mz, nz = a.rc.nz;
+  T2* A = (T2*)smem;
+  T2* B = A + J * N;
+  const int nyb = (my + J - 1) / J;
+  const int i = blockIdx.x / nyb, j0 = (blockIdx.x - i * nyb) * J;
+  const int f = blockIdx.y;
+  const T2* src = (const T2*)a.wbuf[f];
+  T2* dst = (T2*)a.spec[f];
+  for (int it = threadIdx.x; it < N * J; it += blockDim.x) {
+    const int z = it / J, jj = it - z * J, j = j0 + jj;
+    T2 v;
+    v.x = 0; v.y = 0;
+    if (j < my) v = src[((size_t)i * N + z) * my + j];
+    A[jj * N + z] = v;
+  }
+  __syncthreads();
+  T2* R = fft_rows<T, T2>(A, B, N, J, a.plz, (const T2*)a.twz, -1);
+  const T sc = (T)(1.0 / (double)N);
+  for (int it = threadIdx.x; it < mz * J; it += blockDim.x) {
+    const int k = it / J, jj = it - k * J, j = j0 + jj;
+    if (j < my) {
+      const int z = (k <= nz) ? k : k + N - mz;
+      T2 v = R[jj * N + z];
+      v.x *= sc;
+      v.y *= sc;
+      dst[((size_t)i * mz + k) * my + j] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// x lines: inverse x transform of 8 (12 with Z) fields, phys2 algebra,
+// forward x transform of 11 (12) fields, for L lines (fixed z, L
+// consecutive y) per workgroup.  Two real lines share one complex FFT.
+// ------------------------------------------------------------------------
+constexpr int XL_EPT = 4;  // complex entries per thread (pairs*N <= 1024)
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
+  if (!(v >= 0.0)) v = __longlong_as_double(0x7ff8000000000000LL);  // NaN -> max bits
+  atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+template <typename T, bool WZ>
+__global__ __launch_bounds__(256) void k_xline(FftArgs a) {
+  using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const RunConst& rc = a.rc;
+  const int N = rc.mgalx, np = rc.nplanes, my = rc.my, mgz = rc.mgalz;
+  const int L = a.L, pairs = L / 2;
+  const int nyb = (my + L - 1) / L;
+  const int z = blockIdx.x / nyb, j0 = (blockIdx.x - z * nyb) * L;
+  T2* A = (T2*)smem;
+  T2* B = A + pairs * N;
+  const int tot = pairs * N;
+  constexpr int NIN = WZ ? 12 : 8;
+  const T2* tw = (const T2*)a.twx;
+
+  // physical values: ph[f][m] = (line a, line b) at entry e = tid + 256 m
+  double ph[NIN][XL_EPT][2];
+#pragma unroll
+  for (int f = 0; f < NIN; ++f) {
+    for (int it = threadIdx.x; it < tot; it += blockDim.x) { A[it].x = 0; A[it].y = 0; }
+    __syncthreads();
+    const T2* src = (const T2*)a.wbuf[f];
+    for (int it = threadIdx.x; it < np * pairs; it += blockDim.x) {
+      const int ii = it / pairs, p = it - ii * pairs;
+      const int ja = j0 + 2 * p, jb = ja + 1;
+      T2 ca, cb;
+      ca.x = 0; ca.y = 0; cb.x = 0; cb.y = 0;
+      const size_t base = ((size_t)ii * mgz + z) * my;
+      if (ja < my) ca = src[base + ja];
+      if (jb < my) cb = src[base + jb];
+      T2* row = A + p * N;
+      if (ii == 0) {
+        // the imaginary part of the kx=0 coefficient is ignored (rft, rftsingle.f:87-91)
+        T2 v; v.x = ca.x; v.y = cb.x;
+        row[0] = v;
+      } else {
+        T2 v, w;
+        v.x = ca.x - cb.y; v.y = ca.y + cb.x;   // X_a + i X_b
+        w.x = ca.x + cb.y; w.y = -ca.y + cb.x;  // conj(X_a) + i conj(X_b)
+        row[ii] = v;
+        row[N - ii] = w;
+      }
+    }
+    __syncthreads();
+    T2* R = fft_rows<T, T2>(A, B, N, pairs, a.plx, tw, +1);
+#pragma unroll
+    for (int m = 0; m < XL_EPT; ++m) {
+      const int e = threadIdx.x + 256 * m;
+      if (e < tot) {
+        ph[f][m][0] = (double)R[e].x;
+        ph[f][m][1] = (double)R[e].y;
+      } else {
+        ph[f][m][0] = ph[f][m][1] = 0.0;
+      }
+    }
+    __syncthreads();
+  }
+
+  // pointwise physics (cross.f:1613-1673); rhst history in FP64
+  const double dt = a.sc->dt;
+  const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+  const double xis[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+  const double p0 = 5.0 / 66.0, p1 = -1.0;
+  const double b3 = (p1 * (gam[0] * gam[2] + gam[1] * gam[2] + xis[1] * gam[2] + gam[0] * xis[2]) -
+                     gam[0] * gam[1] * gam[2] - gam[0] * gam[1] * xis[2]) /
+                    (p1 - gam[0] * gam[1]);
+  const double betas[3] = {gam[0] - p0, p1, b3};
+  const double alphas[3] = {p0, gam[1] + xis[1] - p1, gam[2] + xis[2] - b3};
+  const int rk = a.rkstep - 1;
+  const double dtxi = dt * xis[rk], dtgamma = dt * gam[rk], dtbeta = dt * betas[rk];
+  const double ab = alphas[rk] / betas[rk];
+  const double ipe = 1.0 / rc.peclet;
+  double dT[XL_EPT][2], drh[XL_EPT][2], dZ[XL_EPT][2];
+  double cx = 0.0, cy = 0.0, cz = 0.0, cd = 0.0;
+#pragma unroll
+  for (int m = 0; m < XL_EPT; ++m) {
+    const int e = threadIdx.x + 256 * m;
+    const int p = e / N, n = e - p * N;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int j = j0 + 2 * p + h;
+      dT[m][h] = drh[m][h] = dZ[m][h] = 0.0;
+      if (e >= tot || j >= my) continue;
+      const double ru = ph[0][m][h], rv = ph[1][m][h], rw = ph[2][m][h];
+      const double lapT = ph[3][m][h], Tx = ph[4][m][h], Ty = ph[5][m][h];
+      const double Tz = ph[6][m][h], Tt = ph[7][m][h];
+      const double u = ru * Tt, v = rv * Tt, w = rw * Tt;
+      const size_t ri = ((size_t)z * my + j) * N + n;
+      const double rold = a.rhst[ri];
+      double tnext = Tt + dtxi * rold;
+      double d = dtxi * rold;
+      const double r = -u * Tx - v * Ty - w * Tz + Tt * lapT * ipe;
+      a.rhst[ri] = r;
+      tnext = tnext + dtgamma * r;
+      d = d + dtgamma * r;
+      dT[m][h] = d;
+      drh[m][h] = (d / (Tt * tnext)) / dtbeta - ab * r / (Tt * Tt);
+      if constexpr (WZ) {
+        const double zx = ph[8][m][h], zy = ph[9][m][h], zz = ph[10][m][h], lz = ph[11][m][h];
+        const double zold = a.rhsz[ri];
+        const double rz = -u * zx - v * zy - w * zz + Tt * lz * ipe;
+        a.rhsz[ri] = rz;
+        dZ[m][h] = dtxi * zold + dtgamma * rz;
+      }
+      if (a.do_cfl) {
+        const double ihy = 1.0 / a.hy[j];
+        cx = fmax(cx, fabs(u));
+        cy = fmax(cy, fabs(v) * ihy);
+        cz = fmax(cz, fabs(w));
+        const double hx = rc.hxalp * 0.5, hz = rc.hzbet * 0.5;
+        cd = fmax(cd, 2.0 * (ipe * Tt) * (hx * hx + hz * hz + ihy * ihy));
+      }
+    }
+  }
+  if (a.do_cfl) {
+    // cfly uses max|v| / hy per line, cross.f:1705-1707
+    for (int d = 32; d >= 1; d >>= 1) {
+      cx = fmax(cx, __shfl_xor(cx, d));
+      cy = fmax(cy, __shfl_xor(cy, d));
+      cz = fmax(cz, __shfl_xor(cz, d));
+      cd = fmax(cd, __shfl_xor(cd, d));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomic_max_nonneg(&a.sc->cflc_bits[0], cx);
+      atomic_max_nonneg(&a.sc->cflc_bits[1], cy);
+      atomic_max_nonneg(&a.sc->cflc_bits[2], cz);
+      atomic_max_nonneg(&a.sc->cfld_bits, cd);
+    }
+  }
+
+  // outputs: 0 u 1 v 2 w 3 ruu 4 rvv 5 rww 6 ruv 7 ruw 8 rvw 9 drho 10 DT 11 DZ
+  constexpr int nout = WZ ? 12 : 11;
+  const T sc = (T)(1.0 / (double)N);
+  for (int q = 0; q < nout; ++q) {
+#pragma unroll
+    for (int m = 0; m < XL_EPT; ++m) {
+      const int e = threadIdx.x + 256 * m;
+      if (e >= tot) continue;
+      double val[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double ru = ph[0][m][h], rv = ph[1][m][h], rw = ph[2][m][h], Tt = ph[7][m][h];
+        double x;
+        switch (q) {
+          case 0: x = ru * Tt; break;
+          case 1: x = rv * Tt; break;
+          case 2: x = rw * Tt; break;
+          case 3: x = ru * (ru * Tt); break;
+          case 4: x = rv * (rv * Tt); break;
+          case 5: x = rw * (rw * Tt); break;
+          case 6: x = ru * (rv * Tt); break;
+          case 7: x = ru * (rw * Tt); break;
+          case 8: x = rv * (rw * Tt); break;
+          case 9: x = drh[m][h]; break;
+          case 10: x = dT[m][h]; break;
+          default: x = dZ[m][h]; break;
+        }
+        val[h] = x;
+      }
+      T2 v;
+      v.x = (T)val[0];
+      v.y = (T)val[1];
+      A[e] = v;
+    }
+    __syncthreads();
+    T2* R = fft_rows<T, T2>(A, B, N, pairs, a.plx, tw, -1);
+    T2* dst = (T2*)a.wbuf[q];
+    for (int it = threadIdx.x; it < np * pairs; it += blockDim.x) {
+      const int ii = it / pairs, p = it - ii * pairs;
+      const int ja = j0 + 2 * p, jb = ja + 1;
+      const T2* row = R + p * N;
+      T2 zk = row[ii];
+      T2 zc = row[(N - ii) % N];
+      zc.y = -zc.y;
+      T2 ya, yb;
+      ya.x = (zk.x + zc.x) * (T)0.5 * sc;
+      ya.y = (zk.y + zc.y) * (T)0.5 * sc;
+      // (zk - zc) / (2i)
+      yb.x = (zk.y - zc.y) * (T)0.5 * sc;
+      yb.y = -(zk.x - zc.x) * (T)0.5 * sc;
+      if (ii == 0) { ya.y = 0; yb.y = 0; }
+      const size_t base = ((size_t)ii * mgz + z) * my;
+      if (ja < my) dst[base + ja] = ya;
+      if (jb < my) dst[base + jb] = yb;
+    }
+    __syncthreads();
+  }
+}
+
+// fourz on standalone rows (FFTPACK convention), for the per-operator API
+__global__ __launch_bounds__(256) void k_fourz(FftPlan pl, const double2* tw, double2* c,
+                                               int nrows, int iopt) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = pl.n;
+  double2* A = (double2*)smem;
+  double2* B = A + N;
+  const int row = blockIdx.x;
+  if (row >= nrows) return;
+  for (int it = threadIdx.x; it < N; it += blockDim.x) A[it] = c[(size_t)row * N + it];
+  __syncthreads();
+  double2* R = fft_rows<double, double2>(A, B, N, 1, pl, tw, iopt < 0 ? -1 : 1);
+  const double sc = iopt < 0 ? 1.0 / N : 1.0;
+  for (int it = threadIdx.x; it < N; it += blockDim.x) {
+    double2 v = R[it];
+    v.x *= sc;
+    v.y *= sc;
+    c[(size_t)row * N + it] = v;
+  }
+}
+
+// ------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------
+static size_t zfft_lds(bool fp32, const FftArgs& a) {
+  return (size_t)2 * a.J * a.rc.mgalz * (fp32 ? 8 : 16);
+}
+static size_t xline_lds(bool fp32, const FftArgs& a) {
+  return (size_t)2 * (a.L / 2) * a.rc.mgalx * (fp32 ? 8 : 16);
+}
+
+int launch_zfft_inv(bool fp32, const FftArgs& a, hipStream_t st) {
+  const int nyb = (a.rc.my + a.J - 1) / a.J;
+  dim3 g(a.rc.nplanes * nyb, a.nfields), b(256);
+  const size_t lds = zfft_lds(fp32, a);
+  if (fp32)
+    hipLaunchKernelGGL(k_zfft_inv<float>, g, b, lds, st, a);
+  else
+    hipLaunchKernelGGL(k_zfft_inv<double>, g, b, lds, st, a);
+  return (int)hipGetLastError();
+}
+
+int launch_zfft_fwd(bool fp32, const FftArgs& a, hipStream_t st) {
+  const int nyb = (a.rc.my + a.J - 1) / a.J;
+  dim3 g(a.rc.nplanes * nyb, a.nfields), b(256);
+  const size_t lds = zfft_lds(fp32, a);
+  if (fp32)
+    hipLaunchKernelGGL(k_zfft_fwd<float>, g, b, lds, st, a);
+  else
+    hipLaunchKernelGGL(k_zfft_fwd<double>, g, b, lds, st, a);
+  return (int)hipGetLastError();
+}
+
+int launch_xline(bool fp32, const FftArgs& a, hipStream_t st) {
+  const int nyb = (a.rc.my + a.L - 1) / a.L;
+  dim3 g(a.rc.mgalz * nyb), b(256);
+  const size_t lds = xline_lds(fp32, a);
+  if (fp32) {
+    if (a.with_z) hipLaunchKernelGGL((k_xline<float, true>), g, b, lds, st, a);
+    else hipLaunchKernelGGL((k_xline<float, false>), g, b, lds, st, a);
+  } else {
+    if (a.with_z) hipLaunchKernelGGL((k_xline<double, true>), g, b, lds, st, a);
+    else hipLaunchKernelGGL((k_xline<double, false>), g, b, lds, st, a);
+  }
+  return (int)hipGetLastError();
+}
+
+int launch_fourz(const FftPlan& pl, const double2* tw, double2* c, int nrows, int iopt,
+                 hipStream_t st) {
+  if (nrows <= 0) return 0;
+  hipLaunchKernelGGL(k_fourz, dim3(nrows), dim3(256), (size_t)2 * pl.n * 16, st, pl, tw, c,
+                     nrows, iopt);
+  return (int)hipGetLastError();
+}
+
+
+}  // namespace loma

@@ -1,0 +1,103 @@
+// loma_kernels.h -- launch interface between the host orchestration
+// (loma_api.hip) and the kernels (loma_yops.hip, loma_fft.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "loma_dev.h"
+
+namespace loma {
+
+// RK3 coefficients (trunk/lomacte3/src/modulos.f:194-209)
+struct RkCoef {
+  double gama[3], xi[3], alpha[3], beta[3];
+};
+RkCoef rk_coefficients();
+
+// device-resident scalars of one run
+struct DevScalars {
+  double dt;      // current step size (Deltat)
+  double time;
+  double dtr;     // next step size from the CFL estimate
+  unsigned long long cflc_bits[3];  // max |u|, max |v|/hy, max |w| (as bits)
+  unsigned long long cfld_bits;     // max diffusive inverse dt (as bits)
+  int nan;
+  int pad;
+};
+
+// per-run physics constants passed by value to kernels
+struct RunConst {
+  int mgalx, my, mgalz, mx, mz, nplanes, nz;
+  double alp, bet, ire, peclet;
+  double hxalp, hzbet;   // 2/dx and 2/dz as in phys2 (cross.f:1500-1501)
+  double rhobot, rhotop, zbot, ztop;
+  double cfl;
+  int dt_fp32;
+};
+
+// FFT plan for one length
+struct FftPlan {
+  int n;
+  int npass;
+  int rad[10];
+};
+FftPlan make_plan(int n);
+
+// argument blocks of the fused phase kernels (pointers are device memory)
+struct PhaseArgs {
+  RunConst rc;
+  YTab T;
+  const double* helm_lu;     // per (i,|kz|) class LU tables
+  int lu_rows;               // rows per class table (incl. padding)
+  double2 *vor, *phi, *psi, *scal, *ten12, *ten13, *zf;
+  void* spec[12];            // S-sized transform buffers (float2 or double2)
+  double *u00wk, *v00wk, *w00wk, *rf0u, *rf0w;
+  DevScalars* sc;
+  int rkstep;
+  int with_z;
+  int mode_base;             // first mode handled by this launch
+  int nmodes;                // modes handled by this launch
+};
+
+struct FftArgs {
+  RunConst rc;
+  FftPlan plz, plx;
+  const void* twz;           // twiddles exp(-2 pi i t/mgalz), T2
+  const void* twx;
+  void* spec[12];
+  void* wbuf[12];
+  double* rhst;
+  double* rhsz;
+  const double* hy;          // my
+  DevScalars* sc;
+  int nfields;
+  int rkstep;
+  int do_cfl;
+  int with_z;
+  int J;                     // y rows per z-transform block
+  int L;                     // lines per x-line block
+};
+
+// launchers (return hipError_t as int)
+int launch_dy(int M, int order, const YTab& T, const double* u, double* du, int ncols,
+              hipStream_t st);
+int launch_helm_factor(const double* dt21, const double* dt22, int my, int rows_pad,
+                       const double* shifts, int nsys, double* lu, hipStream_t st);
+int launch_lapvdvhom(int M, const YTab& T, const double* lu, int lu_rows,
+                     const double* phi, double* v, double* dv, const double* rK,
+                     int ncols, int mode, hipStream_t st);
+int launch_inty8(int M, const YTab& T, const double* du, double* u, hipStream_t st);
+
+int launch_phaseA(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
+int launch_phaseC(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
+int launch_finish_step(DevScalars* sc, RunConst rc, int icfl, hipStream_t st);
+
+int launch_zfft_inv(bool fp32, const FftArgs& a, hipStream_t st);
+int launch_xline(bool fp32, const FftArgs& a, hipStream_t st);
+int launch_zfft_fwd(bool fp32, const FftArgs& a, hipStream_t st);
+int launch_fourz(const FftPlan& pl, const double2* tw, double2* c, int nrows, int iopt,
+                 hipStream_t st);
+int launch_band5(const double* band, void* rhs, double* x, int n, int nsys, int rhs_fp32,
+                 hipStream_t st);
+
+}  // namespace loma

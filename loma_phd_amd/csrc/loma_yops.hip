@@ -1,0 +1,149 @@
+// loma_yops.hip -- dispatch of the y-direction kernels over the chunk size M
+// (rows of a y-column per lane); the kernels are in loma_yops_impl.h and are
+// instantiated one M per translation unit (loma_yops_m<M>.hip).
+#include <hip/hip_runtime.h>
+
+#include "loma_kernels.h"
+
+namespace loma {
+
+template <int M>
+int launch_dy_m(int, const YTab&, const double*, double*, int, hipStream_t);
+template <int M>
+int launch_lapvdvhom_m(const YTab&, const double*, int, const double*, double*, double*,
+                       const double*, int, int, hipStream_t);
+template <int M>
+int launch_inty8_m(const YTab&, const double*, double*, hipStream_t);
+template <int M>
+int launch_phase_m(int, bool, const PhaseArgs&, hipStream_t);
+
+RkCoef rk_coefficients() {
+  RkCoef r;
+  const double p0 = 5.0 / 66.0, p1 = -1.0;
+  const double g[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+  const double xi[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+  const double b3 = (p1 * (g[0] * g[2] + g[1] * g[2] + xi[1] * g[2] + g[0] * xi[2]) -
+                     g[0] * g[1] * g[2] - g[0] * g[1] * xi[2]) /
+                    (p1 - g[0] * g[1]);
+  for (int q = 0; q < 3; ++q) { r.gama[q] = g[q]; r.xi[q] = xi[q]; }
+  r.beta[0] = g[0] - p0; r.beta[1] = p1; r.beta[2] = b3;
+  r.alpha[0] = p0; r.alpha[1] = g[1] + xi[1] - p1; r.alpha[2] = g[2] + xi[2] - r.beta[2];
+  return r;
+}
+
+
+#define LOMA_DISPATCH_M(M, ...)                 \
+  switch (M) {                                  \
+    case 3: { constexpr int MM = 3; return __VA_ARGS__; } \
+    case 4: { constexpr int MM = 4; return __VA_ARGS__; } \
+    case 5: { constexpr int MM = 5; return __VA_ARGS__; } \
+    case 6: { constexpr int MM = 6; return __VA_ARGS__; } \
+    case 7: { constexpr int MM = 7; return __VA_ARGS__; } \
+    case 8: { constexpr int MM = 8; return __VA_ARGS__; } \
+    case 9: { constexpr int MM = 9; return __VA_ARGS__; } \
+    case 10: { constexpr int MM = 10; return __VA_ARGS__; } \
+    case 12: { constexpr int MM = 12; return __VA_ARGS__; } \
+    case 14: { constexpr int MM = 14; return __VA_ARGS__; } \
+    case 16: { constexpr int MM = 16; return __VA_ARGS__; } \
+    default: return (int)hipErrorInvalidValue;  \
+  }
+
+int launch_dy(int M, int order, const YTab& T, const double* u, double* du, int ncols,
+              hipStream_t st) {
+  if (ncols <= 0) return 0;
+  LOMA_DISPATCH_M(M, launch_dy_m<MM>(order, T, u, du, ncols, st));
+}
+int launch_lapvdvhom(int M, const YTab& T, const double* lu, int lu_rows, const double* phi,
+                     double* v, double* dv, const double* rK, int ncols, int mode,
+                     hipStream_t st) {
+  if (ncols <= 0) return 0;
+  LOMA_DISPATCH_M(M, launch_lapvdvhom_m<MM>(T, lu, lu_rows, phi, v, dv, rK, ncols, mode, st));
+}
+int launch_inty8(int M, const YTab& T, const double* du, double* u, hipStream_t st) {
+  LOMA_DISPATCH_M(M, launch_inty8_m<MM>(T, du, u, st));
+}
+int launch_phaseA(int M, bool fp32, const PhaseArgs& a, hipStream_t st) {
+  if (a.nmodes <= 0) return 0;
+  LOMA_DISPATCH_M(M, launch_phase_m<MM>(0, fp32, a, st));
+}
+int launch_phaseC(int M, bool fp32, const PhaseArgs& a, hipStream_t st) {
+  if (a.nmodes <= 0) return 0;
+  LOMA_DISPATCH_M(M, launch_phase_m<MM>(1, fp32, a, st));
+}
+
+// LU of (dt22 - s dt21) with identity end rows (Lapvdvhom laps.f:349-366),
+// one thread per shift s.  Output rows: inv, up1, up2, lo1, lo2.
+__global__ void k_helm_factor(const double* __restrict__ dt21,
+                              const double* __restrict__ dt22, int my, int rows_pad,
+                              const double* __restrict__ shifts, int nsys,
+                              double* __restrict__ lu) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  const double sh = shifts[s];
+  double* out = lu + (size_t)s * rows_pad * 5;
+  for (int q = 0; q < rows_pad * 5; ++q) out[q] = 0.0;
+  auto row = [&](int j, double* r) {
+    if (j >= my) { for (int d = 0; d < 5; ++d) r[d] = 0.0; return; }
+    if (j == 0 || j == my - 1) {
+      for (int d = 0; d < 5; ++d) r[d] = 0.0;
+      r[2] = 1.0;
+      return;
+    }
+    for (int d = 0; d < 5; ++d) r[d] = dt22[j * 5 + d] - sh * dt21[j * 5 + d];
+  };
+  double w[3][5];
+  row(0, w[0]); row(1, w[1]); row(2, w[2]);
+  for (int k = 0; k < my; ++k) {
+    double* o = out + (size_t)(k + PADR) * 5;
+    const double iv = 1.0 / w[0][2];
+    o[0] = iv;
+    o[1] = (k + 1 < my) ? w[0][3] : 0.0;
+    o[2] = (k + 2 < my) ? w[0][4] : 0.0;
+    for (int i = 1; i <= 2; ++i) {
+      if (k + i >= my) { o[2 + i] = 0.0; continue; }
+      const double l = w[i][2 - i] * iv;
+      o[2 + i] = l;
+      w[i][2 - i] = 0.0;
+      for (int d = 1; d <= 2; ++d) w[i][2 - i + d] -= l * w[0][2 + d];
+    }
+    for (int d = 0; d < 5; ++d) { w[0][d] = w[1][d]; w[1][d] = w[2][d]; }
+    row(k + 3, w[2]);
+  }
+}
+
+int launch_helm_factor(const double* dt21, const double* dt22, int my, int rows_pad,
+                       const double* shifts, int nsys, double* lu, hipStream_t st) {
+  if (nsys <= 0) return 0;
+  hipLaunchKernelGGL(k_helm_factor, dim3((nsys + 63) / 64), dim3(64), 0, st, dt21, dt22,
+                     my, rows_pad, shifts, nsys, lu);
+  return (int)hipGetLastError();
+}
+
+// end of step: time += dt; dt <- CFL estimate (cross.f:1196-1204,
+// 1769-1818).  NaN flag from the CFL maxima.
+__global__ void k_finish_step(DevScalars* sc, RunConst rc, int icfl) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  sc->time = sc->time + sc->dt;
+  if (icfl) {
+    const double cx = __longlong_as_double((long long)sc->cflc_bits[0]);
+    const double cy = __longlong_as_double((long long)sc->cflc_bits[1]);
+    const double cz = __longlong_as_double((long long)sc->cflc_bits[2]);
+    const double cd = __longlong_as_double((long long)sc->cfld_bits);
+    const double conv = cx * rc.hxalp * 0.5 + cy + cz * rc.hzbet * 0.5;
+    double dtr = rc.cfl / (conv + cd);
+    if (rc.dt_fp32) dtr = (double)(float)dtr;
+    if (!(dtr > 0.0) || isinf(dtr)) sc->nan = 1;
+    sc->dtr = dtr;
+    sc->dt = dtr;
+    sc->cflc_bits[0] = sc->cflc_bits[1] = sc->cflc_bits[2] = 0ull;
+    sc->cfld_bits = 0ull;
+  }
+}
+
+
+int launch_finish_step(DevScalars* sc, RunConst rc, int icfl, hipStream_t st) {
+  hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(64), 0, st, sc, rc, icfl);
+  return (int)hipGetLastError();
+}
+
+}  // namespace loma

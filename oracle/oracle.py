@@ -1,0 +1,252 @@
+"""ctypes wrapper of the CPU FP64 oracle (oracle/loma_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(BUILD, "libloma_oracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with gcc (-O2, OpenMP) into oracle/_build."""
+    os.makedirs(BUILD, exist_ok=True)
+    src = os.path.join(HERE, "loma_oracle.c")
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(src):
+        cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-std=gnu11",
+               "-ffp-contract=off", src, "-o", LIB + ".tmp", "-lm"]
+        subprocess.check_call(cmd)
+        os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        vp = ctypes.c_void_p
+        L.orc_grid_create.restype = vp
+        L.orc_grid_create.argtypes = [ctypes.c_int] * 3 + [_dp] + [ctypes.c_double] * 4
+        L.orc_grid_destroy.argtypes = [vp]
+        L.orc_grid_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.orc_grid_matrices.argtypes = [vp] + [_dp] * 10
+        L.orc_pred1m6o.argtypes = [ctypes.c_int, _dp, _dp]
+        L.orc_prederiv2.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_gaussj.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int]
+        L.orc_gaussj.restype = ctypes.c_int
+        for n in ("orc_bandec", "orc_bandec7"):
+            getattr(L, n).argtypes = [_dp, ctypes.c_int]
+        for n in ("orc_banbks", "orc_banbks7"):
+            getattr(L, n).argtypes = [_dp, ctypes.c_int, _dp]
+        for n in ("orc_deryr2", "orc_deryyr2"):
+            getattr(L, n).argtypes = [vp, _dp, _dp, ctypes.c_int]
+        for n in ("orc_deryr", "orc_deryyr", "orc_inty8"):
+            getattr(L, n).argtypes = [vp, _dp, _dp]
+        for n in ("orc_lapvdvhom", "orc_solvepsi"):
+            getattr(L, n).argtypes = [vp, _dp, _dp, _dp, ctypes.c_double]
+        L.orc_helmholtz_dirichlet.argtypes = [vp, _dp, _dp, ctypes.c_double, ctypes.c_double]
+        for n in ("orc_cft", "orc_rft"):
+            getattr(L, n).argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_state_create.restype = vp
+        L.orc_state_create.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_int]
+        L.orc_state_destroy.argtypes = [vp]
+        L.orc_state_set.argtypes = [vp] + [_dp] * 7
+        L.orc_state_get.argtypes = [vp] + [_dp] * 7
+        L.orc_state_history.argtypes = [vp] + [_dp] * 5
+        L.orc_state_set_z.argtypes = [vp, _dp]
+        L.orc_state_get_z.argtypes = [vp, _dp]
+        L.orc_step.argtypes = [vp, ctypes.c_int]
+        L.orc_step.restype = ctypes.c_int
+        L.orc_substep.argtypes = [vp, ctypes.c_int]
+        L.orc_substep.restype = ctypes.c_int
+        L.orc_state_scalars.argtypes = [vp, _dp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Grid:
+    """FP64 grid + compact-FD matrices (initcr/derivadas restatement)."""
+
+    def __init__(self, mgalx, my, mgalz, y, Re, alp, bet, pr=0.7):
+        self.y = _f64(y)
+        assert self.y.size == my
+        self.mgalx, self.my, self.mgalz = mgalx, my, mgalz
+        self._h = lib().orc_grid_create(mgalx, my, mgalz, _p(self.y), Re, alp, bet, pr)
+        d = (ctypes.c_int * 4)()
+        lib().orc_grid_dims(self._h, d)
+        self.mx, self.mz, self.nplanes, self.nz = list(d)
+        self.Re, self.alp, self.bet, self.pr = Re, alp, bet, pr
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_grid_destroy(self._h)
+            self._h = None
+
+    def matrices(self):
+        my = self.my
+        out = {k: np.zeros(n * my) for k, n in
+               (("prem1", 7), ("dt11", 7), ("dt12", 7), ("prem3", 5), ("dt21", 5),
+                ("dt22", 5), ("fmap", 1), ("hy", 1), ("trp", 1), ("endder", 1))}
+        lib().orc_grid_matrices(self._h, *[_p(out[k]) for k in
+                                           ("prem1", "dt11", "dt12", "prem3", "dt21", "dt22",
+                                            "fmap", "hy", "trp", "endder")])
+        for k, n in (("prem1", 7), ("dt11", 7), ("dt12", 7), ("prem3", 5), ("dt21", 5),
+                     ("dt22", 5)):
+            out[k] = out[k].reshape(my, n)
+        return out
+
+    # complex columns: arrays of shape (m, my) complex128
+    def _colop(self, fn, u):
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        shp = u.shape
+        u2 = u.reshape(-1, self.my)
+        out = np.zeros_like(u2)
+        getattr(lib(), fn)(self._h, _p(u2.view(np.float64)), _p(out.view(np.float64)),
+                           u2.shape[0])
+        return out.reshape(shp)
+
+    def deryr2(self, u):
+        return self._colop("orc_deryr2", u)
+
+    def deryyr2(self, u):
+        return self._colop("orc_deryyr2", u)
+
+    def deryr(self, u):
+        u = _f64(u); o = np.zeros_like(u)
+        lib().orc_deryr(self._h, _p(u), _p(o)); return o
+
+    def deryyr(self, u):
+        u = _f64(u); o = np.zeros_like(u)
+        lib().orc_deryyr(self._h, _p(u), _p(o)); return o
+
+    def inty8(self, du):
+        du = _f64(du); o = np.zeros_like(du)
+        lib().orc_inty8(self._h, _p(du), _p(o)); return o
+
+    def _solve(self, fn, phi, rK):
+        phi = np.ascontiguousarray(phi, dtype=np.complex128)
+        v = np.zeros_like(phi); dv = np.zeros_like(phi)
+        getattr(lib(), fn)(self._h, _p(phi.view(np.float64)), _p(v.view(np.float64)),
+                           _p(dv.view(np.float64)), float(rK))
+        return v, dv
+
+    def lapvdvhom(self, phi, rK):
+        return self._solve("orc_lapvdvhom", phi, rK)
+
+    def solvepsi(self, phi, rK):
+        return self._solve("orc_solvepsi", phi, rK)
+
+    def helmholtz(self, f, rK, shift):
+        f = np.ascontiguousarray(f, dtype=np.complex128)
+        x = np.zeros_like(f)
+        lib().orc_helmholtz_dirichlet(self._h, _p(f.view(np.float64)), _p(x.view(np.float64)),
+                                      float(rK), float(shift))
+        return x
+
+    def rK(self):
+        """rK[i,k] = kx^2 + kz^2 in the reference mode order."""
+        kx = self.alp * np.arange(self.nplanes)
+        k = np.arange(self.mz)
+        kz = np.where(k <= self.nz, self.bet * k, -self.bet * (self.mz - k))
+        return kx[:, None] ** 2 + kz[None, :] ** 2
+
+
+def cft(c, iopt):
+    """FFTPACK-convention complex transform of rows of c (complex128 2-D)."""
+    c = np.array(c, dtype=np.complex128, order="C")
+    m, n = c.shape
+    lib().orc_cft(_p(c.view(np.float64)), n, m, iopt)
+    return c
+
+
+def rft(c, n, iopt):
+    """FFTPACK-convention real transform; c has rows of n+2 doubles."""
+    c = np.array(c, dtype=np.float64, order="C")
+    lib().orc_rft(_p(c), n, c.shape[0], iopt)
+    return c
+
+
+class State:
+    """Oracle time-advance state (cross1 restatement)."""
+
+    def __init__(self, grid: Grid, dt0=1e-4, cfl=0.5, ncfl=1, flags=0):
+        self.g = grid
+        self.flags = flags
+        self._h = lib().orc_state_create(grid._h, dt0, cfl, ncfl, flags)
+        self.S = grid.nplanes * grid.mz * grid.my
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_state_destroy(self._h)
+            self._h = None
+
+    def _shape(self):
+        return (self.g.nplanes, self.g.mz, self.g.my)
+
+    def set(self, vor, phi, psi, scal, u00, v00, w00, z=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128).reshape(self._shape())
+                for a in (vor, phi, psi, scal)]
+        o = [_f64(a) for a in (u00, v00, w00)]
+        lib().orc_state_set(self._h, *[_p(a.view(np.float64)) for a in arrs], *[_p(a) for a in o])
+        if z is not None:
+            zz = np.ascontiguousarray(z, dtype=np.complex128).reshape(self._shape())
+            lib().orc_state_set_z(self._h, _p(zz.view(np.float64)))
+
+    def get(self):
+        shp = self._shape()
+        f = [np.zeros(shp, np.complex128) for _ in range(4)]
+        o = [np.zeros(self.g.my) for _ in range(3)]
+        lib().orc_state_get(self._h, *[_p(a.view(np.float64)) for a in f], *[_p(a) for a in o])
+        out = dict(vor=f[0], phi=f[1], psi=f[2], scal=f[3], u00=o[0], v00=o[1], w00=o[2])
+        if self.flags & 2:
+            z = np.zeros(shp, np.complex128)
+            lib().orc_state_get_z(self._h, _p(z.view(np.float64)))
+            out["z"] = z
+        return out
+
+    def history(self):
+        shp = self._shape()
+        t12 = np.zeros(shp, np.complex128); t13 = np.zeros(shp, np.complex128)
+        rhst = np.zeros((self.g.my, self.g.mgalz, self.g.mgalx))
+        a = np.zeros(self.g.my); b = np.zeros(self.g.my)
+        lib().orc_state_history(self._h, _p(t12.view(np.float64)), _p(t13.view(np.float64)),
+                                _p(rhst), _p(a), _p(b))
+        return dict(ten12=t12, ten13=t13, rhst=rhst, rf0u=a, rf0w=b)
+
+    def step(self, n=1):
+        return lib().orc_step(self._h, n)
+
+    def substep(self, rk):
+        return lib().orc_substep(self._h, rk)
+
+    def scalars(self):
+        o = np.zeros(4)
+        lib().orc_state_scalars(self._h, _p(o))
+        return dict(dt=o[0], time=o[1], dtr=o[2], istep=int(o[3]))
+
+
+def set_threads(n: int):
+    lib().orc_set_threads(int(n))
