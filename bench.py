@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark of the RK3 time advance on one MI355X.
+
+Workload: the single production run 512x513x256 (x, y, z), density ratio 4,
+Re = 800, seeded perturbation of amplitude 1e-3 with a Gaussian envelope in y
+(synthetic initial condition, loma_phd_amd.ic).  A "step" is one full RK3
+step (3 substeps) over the whole grid, state resident in HBM.  Both forms
+are timed: plain FP64 and mixed (FP32 transforms, FP64 y-solves and RK
+accumulation).  `value` is the mixed form's grid-point*RK3-steps/s; the FP64
+form, their ratio, p50/p99 step latencies, the whole-step fraction of the HBM
+roofline and the dominant kernel's roofline are reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config B|A]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes(run, with_z: bool):
+    """Compulsory HBM bytes per launch of each kernel class (one substep)
+    and per RK3 step; see DESIGN.md for the derivation."""
+    S, W, P = run.S, run.W, run.P
+    es = 8 if run.precision == 1 else 16
+    z = 1 if with_z else 0
+    nin, nout = (12, 12) if with_z else (8, 11)
+    k = {
+        "phaseA": (6 + z) * 16 * S + (2 + z) * 16 * S + nin * es * S,
+        "zfft_inv": nin * es * S + nin * es * W,
+        "xline": nin * es * W + nout * es * W + (1 + z) * 2 * 8 * P,
+        "zfft_fwd": nout * es * W + nout * es * S,
+        "phaseC": nout * es * S + (3 + z) * 16 * S + (6 + z) * 16 * S,
+    }
+    step = 3 * sum(k.values())
+    return k, step
+
+
+def model_bytes_per_step(S, P, mixed: bool):
+    """Three-phase compulsory-traffic model of the reference dataflow:
+    55 S-sized complex passes + 2 P-sized real passes per substep."""
+    return (1728 * S + 48 * P) if mixed else (2640 * S + 48 * P)
+
+
+def bench_precision(cfg, st, precision, steps, warmup, prof_steps):
+    import torch
+    from loma_phd_amd import capi
+    flags = capi.LOMA_FLAG_SCALAR_Z if cfg.scalar_z else 0
+    run = capi.Run(cfg.mgalx, cfg.my, cfg.mgalz, st["y"], cfg.Re, cfg.alp, cfg.bet, cfg.pr,
+                   cfg.cfl, cfg.ncfl, cfg.dt0, precision, flags)
+    run.upload(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"],
+               z=st["z"] if cfg.scalar_z else None)
+    run.advance(warmup)
+    stream = torch.cuda.ExternalStream(run.stream()) if run.stream() else \
+        torch.cuda.default_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for q in range(steps):
+        run.advance_async(1)
+        ev[q + 1].record(stream)
+    run.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = np.array([ev[q].elapsed_time(ev[q + 1]) for q in range(steps)])
+    total_ms = ev[0].elapsed_time(ev[steps])
+    # per-kernel durations, HIP events on the run's stream
+    run.profile(True)
+    run.advance(prof_steps)
+    prof = run.profile_read()
+    run.profile(False)
+    kb, step_bytes = algorithmic_bytes(run, cfg.scalar_z)
+    out = dict(
+        ms_per_step=total_ms / steps,
+        wall_ms_per_step=wall * 1e3 / steps,
+        p50_ms=float(np.percentile(per, 50)),
+        p99_ms=float(np.percentile(per, 99)),
+        gpts_per_s=cfg.points * steps / (total_ms * 1e-3),
+        kernels_ms={k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()},
+        kernel_launches={k: v[1] for k, v in prof.items()},
+        kernel_bytes=kb,
+        step_bytes=step_bytes,
+        S=run.S, W=run.W, P=run.P, device_bytes=run.bytes,
+    )
+    run.close()
+    return out
+
+
+def roofline_of(res):
+    # dominant kernel = largest total time per step
+    per_step = {k: res["kernels_ms"][k] * (3 if k != "finish" else 1)
+                for k in res["kernels_ms"]}
+    dom = max((k for k in per_step if k in res["kernel_bytes"]), key=lambda k: per_step[k])
+    t = res["kernels_ms"][dom] * 1e-3
+    achieved = res["kernel_bytes"][dom] / t / 1e9
+    return dom, dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                     kernel=dom, bytes_per_launch=int(res["kernel_bytes"][dom]),
+                     avg_launch_ms=round(res["kernels_ms"][dom], 4))
+
+
+def cpu_baseline(cfg, st, max_threads=16):
+    """FP64 oracle (C restatement) on the host cores: one RK3 substep of the
+    same workload, scaled to grid-pt*RK3-steps/s (x 1/3 of a step)."""
+    from oracle import oracle as O
+    O.build()
+    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    O.set_threads(threads)
+    g = O.Grid(cfg.mgalx, cfg.my, cfg.mgalz, st["y"], cfg.Re, cfg.alp, cfg.bet, cfg.pr)
+    S = O.State(g, dt0=cfg.dt0, cfl=cfg.cfl, ncfl=cfg.ncfl, flags=2 if cfg.scalar_z else 0)
+    S.set(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"],
+          z=st["z"] if cfg.scalar_z else None)
+    t0 = time.perf_counter()
+    S.substep(1)
+    dt = time.perf_counter() - t0
+    return dict(value=cfg.points / (3 * dt), unit="grid-pt*RK3-steps/s", cores=threads,
+                kind="port",
+                sample=f"1 RK3 substep (1/3 step) of {cfg.mgalx}x{cfg.my}x{cfg.mgalz}, "
+                       f"FP64 C oracle, {threads} OpenMP threads, {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="B", choices=["A", "B"])
+    ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
+              f"(asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    from loma_phd_amd import grid, ic
+    import dataclasses
+    cfg = dataclasses.replace(grid.CONFIG_B if args.config == "B" else grid.CONFIG_A)
+    if args.config == "B":
+        cfg.scalar_z = False
+    print(f"[bench] building initial condition {cfg.mgalx}x{cfg.my}x{cfg.mgalz}",
+          file=sys.stderr, flush=True)
+    st = ic.initial_condition(cfg)
+    res = {}
+    for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
+        print(f"[bench] timing {prec}", file=sys.stderr, flush=True)
+        res[prec] = bench_precision(cfg, st, 0 if prec == "fp64" else 1, args.steps,
+                                    args.warmup, args.profile_steps)
+        print(f"[bench] {prec}: {res[prec]['ms_per_step']:.3f} ms/step", file=sys.stderr,
+              flush=True)
+    head = res.get("mixed") or res["fp64"]
+    mixed = "mixed" in res
+    dom, roof = roofline_of(head)
+    S, P = head["S"], head["P"]
+    line = {
+        "metric": "grid-pt*RK3-steps/s (mixed vs FP64, 1 GPU)",
+        "value": round(head["gpts_per_s"], 1),
+        "unit": "grid-pt*RK3-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if mixed else "f64",
+        "data": "synthetic (seeded mixing-layer initial condition)",
+        "config": {"workload": f"temporal mixing layer {cfg.mgalx}x{cfg.my}x{cfg.mgalz}, "
+                               f"s={cfg.s:g}, Re={cfg.Re:g}, one RK3 step",
+                   "grid": [cfg.mgalx, cfg.my, cfg.mgalz], "precision": "mixed" if mixed
+                   else "fp64"},
+        "p50_ms": round(head["p50_ms"], 4),
+        "p99_ms": round(head["p99_ms"], 4),
+        "hbm_roof_frac_step": round(model_bytes_per_step(S, P, mixed) /
+                                    (head["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+        "roofline": roof,
+    }
+    for prec, r in res.items():
+        line[prec] = {"gpts_per_s": round(r["gpts_per_s"], 1),
+                      "ms_per_step": round(r["ms_per_step"], 4),
+                      "p50_ms": round(r["p50_ms"], 4), "p99_ms": round(r["p99_ms"], 4),
+                      "hbm_roof_frac_step": round(
+                          model_bytes_per_step(S, P, prec == "mixed") /
+                          (r["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                      "kernel_ms": {k: round(v, 4) for k, v in r["kernels_ms"].items()},
+                      "dominant": roofline_of(r)[1]}
+    if "fp64" in res and "mixed" in res:
+        line["speedup_mixed_vs_fp64"] = round(res["mixed"]["gpts_per_s"] /
+                                              res["fp64"]["gpts_per_s"], 3)
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(cfg, st)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -98,10 +98,22 @@ int loma_scalars(loma_run *run, double *out);
  * modes ordered u00,w00,v00 as in cross1's argument list). */
 int loma_cross1(loma_run *run, float *vor, float *phi, float *psi, float *scal,
                 double *u00, double *w00, double *v00, int nstep);
+/* Enqueue nsteps without waiting (no host synchronisation); loma_sync waits
+ * for the run's stream and reports LOMA_ERR_NAN like loma_advance. */
+int loma_advance_async(loma_run *run, int nsteps);
+int loma_sync(loma_run *run);
 /* HIP stream the run launches on (0 = default), and graph capture of one
  * full step (replayed by loma_advance when enabled). */
 int loma_set_stream(loma_run *run, void *stream);
+int loma_stream(loma_run *run, void **stream);
 int loma_use_graph(loma_run *run, int enable);
+/* Per-kernel timing with HIP events recorded on the run's stream around
+ * every launch (graph replay is bypassed while enabled).  profile_read
+ * returns, per kernel class (0 phase A, 1 z inverse FFT, 2 x lines,
+ * 3 z forward FFT, 4 phase C, 5 end of step), the summed milliseconds and
+ * launch counts since the last read. */
+int loma_profile(loma_run *run, int enable);
+int loma_profile_read(loma_run *run, double *total_ms, int *counts, int nclasses);
 
 /* ---- per-operator entry points (device pointers, double) --------------- */
 /* All act on ncols complex y-columns of length my stored back to back

@@ -27,8 +27,10 @@ EXPORTS = [
     "loma_cross1", "loma_set_stream", "loma_use_graph", "loma_deryr2_dev",
     "loma_deryyr2_dev", "loma_lapvdvhom_dev", "loma_solvepsi_dev", "loma_helmholtz_dev",
     "loma_inty8_dev", "loma_deryr2", "loma_deryyr2", "loma_lapvdvhom", "loma_solvepsi",
-    "loma_fourz_dev", "loma_band5_solve_dev", "loma_advance_many",
+    "loma_fourz_dev", "loma_band5_solve_dev", "loma_advance_many", "loma_advance_async",
+    "loma_sync", "loma_stream", "loma_profile", "loma_profile_read",
 ]
+KERNEL_CLASSES = ["phaseA", "zfft_inv", "xline", "zfft_fwd", "phaseC", "finish"]
 
 _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -82,6 +84,11 @@ def lib():
             "loma_fourz_dev": ([_vp, _vp, i, i], i),
             "loma_band5_solve_dev": ([_vp, _vp, _vp, i, i, i, _vp], i),
             "loma_advance_many": ([ctypes.POINTER(_vp), i, i], i),
+            "loma_advance_async": ([_vp, i], i),
+            "loma_sync": ([_vp], i),
+            "loma_stream": ([_vp, ctypes.POINTER(_vp)], i),
+            "loma_profile": ([_vp, i], i),
+            "loma_profile_read": ([_vp, _dp, ctypes.POINTER(i), i], i),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -180,6 +187,27 @@ class Run:
 
     def advance(self, nsteps=1):
         check(lib().loma_advance(self._h, nsteps))
+
+    def advance_async(self, nsteps=1):
+        check(lib().loma_advance_async(self._h, nsteps))
+
+    def sync(self):
+        check(lib().loma_sync(self._h))
+
+    def stream(self) -> int:
+        p = _vp()
+        check(lib().loma_stream(self._h, ctypes.byref(p)))
+        return p.value or 0
+
+    def profile(self, on=True):
+        check(lib().loma_profile(self._h, 1 if on else 0))
+
+    def profile_read(self):
+        n = len(KERNEL_CLASSES)
+        ms = np.zeros(n)
+        cnt = (ctypes.c_int * n)()
+        check(lib().loma_profile_read(self._h, _d(ms), cnt, n))
+        return {k: (float(ms[q]), int(cnt[q])) for q, k in enumerate(KERNEL_CLASSES)}
 
     def substep(self, rk):
         check(lib().loma_substep(self._h, rk))

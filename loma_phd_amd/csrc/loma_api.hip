@@ -78,7 +78,30 @@ struct loma_run {
   // graph
   bool use_graph = false;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // per-kernel profiling with HIP events on the run's stream
+  bool prof = false;
+  struct ProfEv { int cls; hipEvent_t a, b; };
+  std::vector<ProfEv> prof_ev;
 };
+
+// kernel classes for profiling
+enum { KC_PHASEA = 0, KC_ZINV, KC_XLINE, KC_ZFWD, KC_PHASEC, KC_FINISH, KC_COUNT };
+
+static int prof_begin(loma_run* r, int cls, hipEvent_t* a) {
+  if (!r->prof) return 0;
+  HIPCHK(hipEventCreate(a));
+  HIPCHK(hipEventRecord(*a, r->stream));
+  (void)cls;
+  return 0;
+}
+static int prof_end(loma_run* r, int cls, hipEvent_t a) {
+  if (!r->prof) return 0;
+  hipEvent_t b;
+  HIPCHK(hipEventCreate(&b));
+  HIPCHK(hipEventRecord(b, r->stream));
+  r->prof_ev.push_back({cls, a, b});
+  return 0;
+}
 
 template <typename P>
 static int dalloc(loma_run* r, P** p, size_t bytes) {
@@ -485,17 +508,27 @@ static FftArgs fft_args(loma_run* r, int rkstep, int do_cfl) {
   return a;
 }
 
+#define PROF(cls, call)                      \
+  do {                                       \
+    hipEvent_t ev_ = nullptr;                \
+    int pe_ = prof_begin(r, cls, &ev_);      \
+    if (pe_) return pe_;                     \
+    LCHK(call);                              \
+    pe_ = prof_end(r, cls, ev_);             \
+    if (pe_) return pe_;                     \
+  } while (0)
+
 static int enqueue_substep(loma_run* r, int rkstep, int icfl) {
   hipStream_t st = r->stream;
   PhaseArgs pa = phase_args(r, rkstep);
-  LCHK(launch_phaseA(r->M, r->fp32, pa, st));
+  PROF(KC_PHASEA, launch_phaseA(r->M, r->fp32, pa, st));
   FftArgs fa = fft_args(r, rkstep, (rkstep == 1 && icfl) ? 1 : 0);
   fa.nfields = r->with_z ? 12 : 8;
-  LCHK(launch_zfft_inv(r->fp32, fa, st));
-  LCHK(launch_xline(r->fp32, fa, st));
+  PROF(KC_ZINV, launch_zfft_inv(r->fp32, fa, st));
+  PROF(KC_XLINE, launch_xline(r->fp32, fa, st));
   fa.nfields = r->with_z ? 12 : 11;
-  LCHK(launch_zfft_fwd(r->fp32, fa, st));
-  LCHK(launch_phaseC(r->M, r->fp32, pa, st));
+  PROF(KC_ZFWD, launch_zfft_fwd(r->fp32, fa, st));
+  PROF(KC_PHASEC, launch_phaseC(r->M, r->fp32, pa, st));
   return LOMA_OK;
 }
 
@@ -504,7 +537,7 @@ static int enqueue_step(loma_run* r, int icfl) {
     int e = enqueue_substep(r, rk, icfl);
     if (e) return e;
   }
-  LCHK(launch_finish_step(r->sc, r->rc, icfl, r->stream));
+  PROF(KC_FINISH, launch_finish_step(r->sc, r->rc, icfl, r->stream));
   return LOMA_OK;
 }
 
@@ -528,7 +561,7 @@ static int run_steps(loma_run* r, int nsteps) {
   for (int n = 0; n < nsteps; ++n) {
     // cross.f:238: the step size is re-estimated every ncfl steps
     const int icfl = ((r->istep) % r->ncfl == 0) ? 1 : 0;
-    if (r->use_graph) {
+    if (r->use_graph && !r->prof) {
       hipGraphExec_t& ge = r->gexec[icfl];
       if (!ge) {
         hipGraph_t g;
@@ -563,6 +596,49 @@ int loma_advance(loma_run* r, int nsteps) {
   DevScalars s;
   HIPCHK(hipMemcpy(&s, r->sc, sizeof(s), hipMemcpyDeviceToHost));
   if (s.nan) { set_error("loma_advance: non-finite state"); return LOMA_ERR_NAN; }
+  return LOMA_OK;
+}
+
+int loma_advance_async(loma_run* r, int nsteps) {
+  if (!r || !r->uploaded) { set_error("loma_advance_async: nothing uploaded"); return LOMA_ERR_STATE; }
+  if (nsteps < 0) { set_error("loma_advance_async: nsteps < 0"); return LOMA_ERR_ARG; }
+  if (r->stream == nullptr && r->use_graph) HIPCHK(hipStreamCreate(&r->stream));
+  return run_steps(r, nsteps);
+}
+
+int loma_sync(loma_run* r) {
+  if (!r) return LOMA_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(r->stream));
+  DevScalars s;
+  HIPCHK(hipMemcpy(&s, r->sc, sizeof(s), hipMemcpyDeviceToHost));
+  if (s.nan) { set_error("loma_sync: non-finite state"); return LOMA_ERR_NAN; }
+  return LOMA_OK;
+}
+
+int loma_profile(loma_run* r, int enable) {
+  if (!r) return LOMA_ERR_ARG;
+  r->prof = enable != 0;
+  return LOMA_OK;
+}
+
+int loma_profile_read(loma_run* r, double* total_ms, int* counts, int nclasses) {
+  if (!r || !total_ms || !counts) return LOMA_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(r->stream));
+  for (int c = 0; c < nclasses; ++c) { total_ms[c] = 0.0; counts[c] = 0; }
+  for (auto& e : r->prof_ev) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+    if (e.cls < nclasses) { total_ms[e.cls] += ms; counts[e.cls] += 1; }
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  r->prof_ev.clear();
+  return LOMA_OK;
+}
+
+int loma_stream(loma_run* r, void** stream) {
+  if (!r || !stream) return LOMA_ERR_ARG;
+  *stream = (void*)r->stream;
   return LOMA_OK;
 }
 

@@ -158,6 +158,9 @@ __device__ __forceinline__ void op_inty(const YTab& T, int r0, int lane, double 
     f[0][t] = (r < T.my) ? x[0][t] / bget(BandRef{T.fmap, 1, 0}, r, 0) : 0.0;
   }
   bmatvec<M, 3, 1>(BandRef{T.a1, 7, 0}, r0, lane, f, x);
+#pragma unroll
+  for (int t = 0; t < M; ++t)
+    if (r0 + t == T.my - 1) x[0][t] = 0.0;  // gauge row x(my) = 0
   lsolve<M, 3, 1>(BandRef{T.loi, 3, 0}, r0, lane, x);
   usolve<M, 3, 1>(BandRef{T.upi, 3, 0}, BandRef{T.invi, 1, 0}, r0, lane, x);
   const double v0 = col_value<M>(x[0], 0);

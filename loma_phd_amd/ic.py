@@ -79,6 +79,47 @@ class HostOps:
         r[-1] = 0.0
         return solve_banded((2, 2), self._ab(H), r)
 
+    def helm_batch(self, F, rK):
+        """helm() for many modes at once: F (nmodes, my) complex, rK (nmodes,).
+        Banded elimination without pivoting, vectorised over modes."""
+        nm, my = F.shape
+        rK = np.asarray(rK, dtype=np.float64)
+        R = self._apply(self.a2, F.T.copy()).T  # (nm, my)
+        R[:, 0] = 0.0
+        R[:, -1] = 0.0
+        H = self.b2[None, :, :] - rK[:, None, None] * self.a2[None, :, :]  # (nm, my, 5)
+        H[:, 0, :] = 0.0
+        H[:, 0, 2] = 1.0
+        H[:, -1, :] = 0.0
+        H[:, -1, 2] = 1.0
+        U1 = np.zeros((nm, my))
+        U2 = np.zeros((nm, my))
+        D = np.zeros((nm, my))
+        w = H.copy()
+        for k in range(my):
+            D[:, k] = w[:, k, 2]
+            if k + 1 < my:
+                U1[:, k] = w[:, k, 3]
+            if k + 2 < my:
+                U2[:, k] = w[:, k, 4]
+            for i in (1, 2):
+                if k + i >= my:
+                    continue
+                l = w[:, k + i, 2 - i] / D[:, k]
+                w[:, k + i, 2 - i] = 0.0
+                w[:, k + i, 3 - i] -= l * U1[:, k]
+                w[:, k + i, 4 - i] -= l * U2[:, k]
+                R[:, k + i] -= l * R[:, k]
+        X = np.zeros_like(R)
+        for k in range(my - 1, -1, -1):
+            s = R[:, k].copy()
+            if k + 1 < my:
+                s -= U1[:, k] * X[:, k + 1]
+            if k + 2 < my:
+                s -= U2[:, k] * X[:, k + 2]
+            X[:, k] = s / D[:, k]
+        return X
+
 
 def mean_profiles(cfg: MixingLayerConfig, y):
     """rho00, U, Z mean profiles (initloma3 main.f:360-373 with dm0 = 1/4)."""
@@ -105,14 +146,15 @@ def initial_condition(cfg: MixingLayerConfig, ops: HostOps | None = None):
     k0 = max(cfg.alp, cfg.bet) * 4.0
     ywin = np.exp(-(y / cfg.env_width) ** 2)
     vel = []
+    shape = (kh / k0) ** 2 * np.exp(-(kh / k0) ** 2)
     for _ in range(3):
         amp = 2.0 * (0.5 - rng.random((np_, mz, nymax)))
         phs = rng.random((np_, mz, nymax))
-        ky = 2.0 * kh[:, :, None] / nymax * np.arange(1, nymax + 1)[None, None, :]
-        shape = (kh / k0) ** 2 * np.exp(-(kh / k0) ** 2)
-        f = np.einsum("ikn,ikny->iky", amp,
-                      np.exp(1j * (ky[..., None] * y[None, None, None, :] +
-                                   2 * np.pi * phs[..., None])))
+        f = np.zeros((np_, mz, my), np.complex128)
+        for n in range(nymax):
+            ky = 2.0 * kh / nymax * (n + 1)
+            f += amp[:, :, n, None] * np.exp(1j * (ky[:, :, None] * y[None, None, :] +
+                                                   2 * np.pi * phs[:, :, n, None]))
         vel.append(f * (shape[:, :, None] * ywin[None, None, :]))
     u, v, w = vel
     ikx = 1j * kx[:, None, None]
@@ -141,13 +183,9 @@ def initial_condition(cfg: MixingLayerConfig, ops: HostOps | None = None):
     # psi from lap psi = div(rho u)   (main.f:452-463)
     dv = np.moveaxis(ops.dy1(np.moveaxis(rv, 2, 0)), 0, 2)
     div = ikx * ru + dv + ikz * rw
-    psi = np.zeros_like(div)
     rK = kx[:, None] ** 2 + kz[None, :] ** 2
-    for i in range(np_):
-        for k in range(mz):
-            if rK[i, k] == 0.0:
-                continue
-            psi[i, k] = ops.helm(div[i, k], rK[i, k])
+    psi = ops.helm_batch(div.reshape(-1, my), rK.ravel()).reshape(div.shape)
+    psi[rK == 0.0] = 0.0
     vor = ikz * ru - ikx * rw
     dpsi = np.moveaxis(ops.dy1(np.moveaxis(psi, 2, 0)), 0, 2)
     my_ = rv - dpsi

@@ -631,8 +631,12 @@ void orc_solvepsi(const orc_grid *g, const double *phi, double *v,
   free(a);
 }
 
-/* inty8, laps.f:484-575: u = integral of du, u(1) = 0 */
-void orc_inty8(const orc_grid *g, const double *du, double *u) {
+/* inty8, laps.f:484-575, exactly as the reference computes it: LU of dt12
+ * (bandec7) and u = dt12^{-1} dt11 (du/fmap) - (value at row 1).  dt12 has
+ * the constant vector in its null space (each of its rows sums to zero), so
+ * the last pivot of this LU is rounding noise (|1/pivot| ~ 1e11-1e12) and the
+ * result carries cancellation noise of relative size ~eps/|pivot|. */
+void orc_inty8_ref(const orc_grid *g, const double *du, double *u) {
   int my = g->my;
   double *w = malloc(7 * my * sizeof(double));
   memcpy(w, g->dt12, 7 * my * sizeof(double));
@@ -640,6 +644,28 @@ void orc_inty8(const orc_grid *g, const double *du, double *u) {
   double *f = malloc(my * sizeof(double)), *v = malloc(my * sizeof(double));
   for (int j = 0; j < my; j++) f[j] = du[j] / g->fmap[j];
   band_matvec(g->dt11, my, 3, f, 1, v);
+  orc_banbks7(w, my, v);
+  double v0 = v[0];
+  for (int j = 0; j < my; j++) u[j] = v[j] - v0;
+  free(f); free(v); free(w);
+}
+
+/* inty8 in the well-conditioned form used by the time advance: the same
+ * function in exact arithmetic.  The reference's solve (see orc_inty8_ref)
+ * returns x~ + C*1 where x~ solves rows 1..my-1 of dt12 x = dt11 f with
+ * x(my) = 0, and C (huge) is removed by the final subtraction.  Here the
+ * last row of dt12 is replaced by the gauge x(my) = 0, so no huge C arises. */
+void orc_inty8(const orc_grid *g, const double *du, double *u) {
+  int my = g->my;
+  double *w = malloc(7 * my * sizeof(double));
+  memcpy(w, g->dt12, 7 * my * sizeof(double));
+  for (int d = 0; d < 7; d++) w[(my - 1) * 7 + d] = 0.0;
+  w[(my - 1) * 7 + 3] = 1.0;
+  orc_bandec7(w, my);
+  double *f = malloc(my * sizeof(double)), *v = malloc(my * sizeof(double));
+  for (int j = 0; j < my; j++) f[j] = du[j] / g->fmap[j];
+  band_matvec(g->dt11, my, 3, f, 1, v);
+  v[my - 1] = 0.0;
   orc_banbks7(w, my, v);
   double v0 = v[0];
   for (int j = 0; j < my; j++) u[j] = v[j] - v0;

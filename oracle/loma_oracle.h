@@ -60,6 +60,8 @@ void orc_lapvdvhom(const orc_grid *g, const double *phi, double *v,
 void orc_solvepsi(const orc_grid *g, const double *phi, double *v,
                   double *dvdy, double rK);
 void orc_inty8(const orc_grid *g, const double *du, double *u);
+/* inty8 with the reference's singular dt12 factorisation (pinning only) */
+void orc_inty8_ref(const orc_grid *g, const double *du, double *u);
 /* implicit viscous Helmholtz (trunk/mixing/src/laps.f:535 visc kernel):
  * solves (D2 - (rK + shift)) x = f with homogeneous Dirichlet ends */
 void orc_helmholtz_dirichlet(const orc_grid *g, const double *f, double *x,

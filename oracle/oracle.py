@@ -53,7 +53,7 @@ def lib():
             getattr(L, n).argtypes = [_dp, ctypes.c_int, _dp]
         for n in ("orc_deryr2", "orc_deryyr2"):
             getattr(L, n).argtypes = [vp, _dp, _dp, ctypes.c_int]
-        for n in ("orc_deryr", "orc_deryyr", "orc_inty8"):
+        for n in ("orc_deryr", "orc_deryyr", "orc_inty8", "orc_inty8_ref"):
             getattr(L, n).argtypes = [vp, _dp, _dp]
         for n in ("orc_lapvdvhom", "orc_solvepsi"):
             getattr(L, n).argtypes = [vp, _dp, _dp, _dp, ctypes.c_double]
@@ -145,6 +145,10 @@ class Grid:
     def inty8(self, du):
         du = _f64(du); o = np.zeros_like(du)
         lib().orc_inty8(self._h, _p(du), _p(o)); return o
+
+    def inty8_ref(self, du):
+        du = _f64(du); o = np.zeros_like(du)
+        lib().orc_inty8_ref(self._h, _p(du), _p(o)); return o
 
     def _solve(self, fn, phi, rK):
         phi = np.ascontiguousarray(phi, dtype=np.complex128)

@@ -1,0 +1,259 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each oracle function is checked against something independent of its own
+code: the reference's golden vectors, outputs of the reference's own Fortran
+routines (fixture written by tests/golden/make_fixtures.py from the build in
+oracle/_ref, and the live library when present), textbook routines
+(numpy.fft, dense / scipy banded solves) and closed forms (exactness on
+polynomials, discrete residuals, boundary conditions).
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy.linalg import solve_banded
+
+from conftest import rel_l2
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def fx():
+    return np.load(os.path.join(GOLD, "fftpack_fixtures.npz"))
+
+
+@pytest.fixture(scope="module")
+def ref453():
+    return np.load(os.path.join(GOLD, "ref_operators_my453.npz"))
+
+
+@pytest.fixture(scope="module")
+def g453(oracle_mod, ref453):
+    y = ref453["y"]
+    return oracle_mod.Grid(192, y.size, 96, y, 160.0, 0.5, 0.5)
+
+
+# ---------------------------------------------------------------- FFTs
+@pytest.mark.parametrize("prec", ["single", "double"])
+def test_cft_against_reference_vectors(oracle_mod, fx, prec):
+    """cftsingle.f forward transform of the rectangle test (benchmark_fft)."""
+    x = fx[f"cft_{prec}_in"].astype(np.complex128)[None, :]
+    out = oracle_mod.cft(x, -1)[0]
+    ref = fx[f"cft_{prec}_out"]
+    # printed with 7 significant digits; the single file also carries the
+    # float32 arithmetic of cftsingle.f
+    tol = 2e-7 if prec == "double" else 3e-6
+    assert np.abs(out - ref).max() < tol
+    # a sign flip of the exponent would conjugate the odd modes
+    assert np.abs(np.conj(out) - ref).max() > 1e-3
+
+
+@pytest.mark.parametrize("prec", ["single", "double"])
+def test_rft_against_reference_vectors(oracle_mod, fx, prec):
+    """rftsingle.f forward packing: mean, 0, (Re, Im) of modes 1.."""
+    n = 768
+    x = np.zeros((1, n + 2))
+    x[0, :n] = fx[f"rft_{prec}_in"]
+    out = oracle_mod.rft(x, n, -1)[0, :n]
+    assert np.abs(out - fx[f"rft_{prec}_out"]).max() < 2e-6
+    assert out[1] == 0.0
+
+
+@pytest.mark.parametrize("n", [64, 96, 192, 256, 384, 512, 30])
+def test_fft_against_numpy(oracle_mod, n):
+    rng = np.random.default_rng(n)
+    c = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    assert rel_l2(oracle_mod.cft(c, -1), np.fft.fft(c, axis=1) / n) < 1e-14
+    assert rel_l2(oracle_mod.cft(c, 1), np.fft.ifft(c, axis=1) * n) < 1e-14
+    if n % 2 == 0:
+        r = np.zeros((2, n + 2))
+        r[:, :n] = rng.standard_normal((2, n))
+        f = oracle_mod.rft(r, n, -1)
+        ref = np.fft.rfft(r[:, :n], axis=1) / n
+        assert rel_l2(f[:, 0:n + 1:2], ref.real) < 1e-14
+        assert rel_l2(f[:, 3:n:2], ref.imag[:, 1:n // 2]) < 1e-14
+        back = oracle_mod.rft(f, n, 1)
+        assert rel_l2(back[:, :n], r[:, :n]) < 1e-13
+
+
+# ---------------------------------------------------------------- FD set-up
+def test_fd_matrices_against_reference(g453, ref453):
+    M = g453.matrices()
+    for k in ("prem1", "dt11", "dt12", "fmap", "endder"):
+        assert np.array_equal(M[k], ref453[k]), k  # same operations, same bits
+    for k in ("prem3", "dt21", "dt22"):  # Taylor systems: Gauss-Jordan rounding
+        assert rel_l2(M[k], ref453[k]) < 1e-12, k
+
+
+def test_fd_exact_on_polynomials(oracle_mod):
+    from loma_phd_amd.grid import tanh_mesh
+    my = 129
+    y = tanh_mesh(my, 3.0)
+    g = oracle_mod.Grid(48, my, 48, y, 100.0, 1.0, 1.0)
+    # first derivative: exact for linear functions through fmap
+    assert np.abs(g.deryr(2.0 * y + 1.0) - 2.0).max() < 1e-12
+    # second derivative: the scheme is global (A2^-1 B2), so the degree-4
+    # boundary closures bound its exactness: exact up to degree 4, not 5
+    for p in range(0, 5):
+        d2 = g.deryyr(y ** p)
+        ex = p * (p - 1) * y ** (p - 2) if p >= 2 else np.zeros(my)
+        assert np.abs(d2 - ex).max() < 1e-8 * max(1.0, np.abs(ex).max()), p
+    assert np.abs(g.deryyr(y ** 5) - 20 * y ** 3).max() > 1e-9 * 540
+    # convergence on a smooth function: the 2nd-order closures at the
+    # boundaries (cfdiff.f:83-130) give at least ~h^2 globally
+    errs = []
+    for my_ in (65, 129, 257):
+        y_ = tanh_mesh(my_, 3.0)
+        g_ = oracle_mod.Grid(48, my_, 48, y_, 100.0, 1.0, 1.0)
+        f = np.sin(2 * y_)
+        errs.append(np.abs(g_.deryr(f) - 2 * np.cos(2 * y_))[5:-5].max())
+    assert errs[1] < errs[0] / 3.5 and errs[2] < errs[1] / 3.5
+
+
+def test_band_lu_against_scipy(oracle_mod):
+    rng = np.random.default_rng(1)
+    for p, (dec, bks) in ((2, ("orc_bandec", "orc_banbks")), (3, ("orc_bandec7", "orc_banbks7"))):
+        n, nb = 40, 2 * p + 1
+        a = rng.standard_normal((n, nb))
+        a[:, p] += 3.0 * np.abs(a).sum(axis=1)  # diagonally dominant
+        for r in range(n):
+            for d in range(nb):
+                if not 0 <= r - p + d < n:
+                    a[r, d] = 0.0
+        b = rng.standard_normal(n)
+        ab = np.zeros((nb, n))
+        for r in range(n):
+            for d in range(nb):
+                c = r - p + d
+                if 0 <= c < n:
+                    ab[2 * p - d, c] = a[r, d]
+        ref = solve_banded((p, p), ab, b)
+        w = a.copy().ravel()
+        getattr(oracle_mod.lib(), dec)(oracle_mod._p(w), n)
+        x = b.copy()
+        getattr(oracle_mod.lib(), bks)(oracle_mod._p(w), n, oracle_mod._p(x))
+        assert rel_l2(x, ref) < 1e-13
+
+
+# ---------------------------------------------------------------- operators
+def test_operators_against_reference(g453, ref453):
+    u = ref453["u"].astype(np.complex128)
+    # the reference's I/O is real(4): agreement to float32 rounding
+    assert rel_l2(g453.deryr2(u), ref453["deryr2"]) < 2e-7
+    assert rel_l2(g453.deryyr2(u), ref453["deryyr2"]) < 2e-7
+    for name in ("lapvdvhom", "solvepsi"):
+        for q, rk in enumerate(ref453["rK"]):
+            v, dv = getattr(g453, name)(u[0], rk)
+            if rk == 0.0:
+                assert np.all(v == 0) and np.all(dv == 0)
+                continue
+            assert rel_l2(v, ref453[name + "_v"][q]) < 2e-7, (name, rk)
+            assert rel_l2(dv, ref453[name + "_dv"][q]) < 2e-7, (name, rk)
+    f = ref453["f"]
+    assert np.array_equal(g453.deryr(f), ref453["deryr"])
+    assert rel_l2(g453.deryyr(f), ref453["deryyr"]) < 1e-12
+    assert np.array_equal(g453.inty8_ref(f), ref453["inty8"])
+    # the well-conditioned form agrees within the reference's own
+    # cancellation noise
+    assert rel_l2(g453.inty8(f), ref453["inty8"]) < 1e-5
+
+
+def test_operators_against_live_reference(oracle_mod):
+    from oracle import refwrap as R
+    if not R.available():
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    y = np.loadtxt(R.MESH453)
+    R.setup(y)
+    g = oracle_mod.Grid(192, y.size, 96, y, 160.0, 0.5, 0.5)
+    rng = np.random.default_rng(99)
+    u = (rng.standard_normal((3, y.size)) + 1j * rng.standard_normal((3, y.size))).astype(np.complex64)
+    assert rel_l2(g.deryr2(u.astype(np.complex128)), R.colop("ref_deryr2", u)) < 2e-7
+    v, dv = g.solvepsi(u[1].astype(np.complex128), 0.3)
+    vr, dvr = R.solve("ref_solvepsi", u[1], 0.3)
+    assert rel_l2(v, vr) < 2e-7 and rel_l2(dv, dvr) < 2e-7
+
+
+def test_poisson_solves_satisfy_discrete_system(oracle_mod):
+    from loma_phd_amd.grid import tanh_mesh
+    my = 97
+    y = tanh_mesh(my, 10.0)
+    g = oracle_mod.Grid(48, my, 48, y, 100.0, 0.5, 0.5)
+    M = g.matrices()
+    rng = np.random.default_rng(5)
+    phi = rng.standard_normal(my) + 1j * rng.standard_normal(my)
+    rK = 1.3
+
+    def band_apply(B, x):
+        p = B.shape[1] // 2
+        out = np.zeros_like(x)
+        for j in range(my):
+            for d in range(B.shape[1]):
+                c = j - p + d
+                if 0 <= c < my:
+                    out[j] += B[j, d] * x[c]
+        return out
+
+    v, dv = g.lapvdvhom(phi, rK)
+    res = band_apply(M["dt22"], v) - rK * band_apply(M["dt21"], v) - band_apply(M["dt21"], phi)
+    assert np.abs(res[1:-1]).max() < 1e-11 * np.abs(band_apply(M["dt21"], phi)).max()
+    assert v[0] == 0 and v[-1] == 0
+    assert rel_l2(dv, g.deryr2(v)) < 1e-13
+    psi, dpsi = g.solvepsi(phi, rK)
+    res = band_apply(M["dt22"], psi) - rK * band_apply(M["dt21"], psi) - band_apply(M["dt21"], phi)
+    assert np.abs(res[1:-1]).max() < 1e-10 * np.abs(band_apply(M["dt21"], phi)).max()
+    d = g.deryr2(psi)
+    assert abs(d[0]) < 1e-10 * np.abs(d).max() and abs(d[-1]) < 1e-10 * np.abs(d).max()
+    assert rel_l2(dpsi, d) < 1e-12
+
+
+def test_inty8_inverts_derivative(oracle_mod):
+    from loma_phd_amd.grid import tanh_mesh
+    my = 257
+    y = tanh_mesh(my, 20.0)
+    g = oracle_mod.Grid(48, my, 48, y, 100.0, 0.5, 0.5)
+    f = np.exp(-y ** 2)
+    u = g.inty8(f)
+    assert u[0] == 0.0
+    assert abs(u[-1] - np.sqrt(np.pi)) < 1e-6           # closed-form integral
+    assert np.abs(g.deryr(u) - f).max() < 1e-5
+    assert rel_l2(u, g.inty8_ref(f)) < 1e-6
+
+
+def test_rk_coefficients_closed_form():
+    """Low-storage RK3 consistency: gamma + xi sums to 1; alpha + beta =
+    gamma + xi per substep (modulos.f:194-209)."""
+    g = np.array([8 / 15, 5 / 12, 3 / 4])
+    xi = np.array([0, -17 / 60, -5 / 12])
+    assert abs((g + xi).sum() - 1.0) < 1e-15
+    p0, p1 = 5 / 66, -1.0
+    b3 = (p1 * (g[0] * g[2] + g[1] * g[2] + xi[1] * g[2] + g[0] * xi[2]) - g[0] * g[1] * g[2]
+          - g[0] * g[1] * xi[2]) / (p1 - g[0] * g[1])
+    beta = np.array([g[0] - p0, p1, b3])
+    alpha = np.array([p0, g[1] + xi[1] - p1, g[2] + xi[2] - b3])
+    assert np.allclose(alpha + beta, g + xi, atol=1e-15)
+
+
+def test_substep_conserves_and_is_stable(oracle_mod):
+    """Mass of the 00 mode of T is advanced by its flux; a zero-perturbation
+    state stays a pure 00-mode state (no spurious modes); kx=0 plane keeps
+    Hermitian symmetry in kz."""
+    from loma_phd_amd import grid, ic
+    cfg = grid.MixingLayerConfig(mgalx=32, my=65, mgalz=32, Re=100.0, s=2.0, amp=0.0,
+                                 Ly=10.0)
+    st = ic.initial_condition(cfg)
+    g = oracle_mod.Grid(cfg.mgalx, cfg.my, cfg.mgalz, st["y"], cfg.Re, cfg.alp, cfg.bet)
+    S = oracle_mod.State(g, flags=2)
+    S.set(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"],
+          z=st["z"])
+    S.step(3)
+    o = S.get()
+    for f in ("vor", "phi", "psi"):
+        assert np.abs(o[f]).max() < 1e-12, f
+    t = o["scal"].copy()
+    t[0, 0] = 0.0
+    assert np.abs(t).max() < 1e-12
+    # T and Z stay monotone profiles between their boundary values
+    T00 = o["scal"][0, 0].real
+    assert T00[0] == pytest.approx(1.0 / st["scal"][0, 0, 0].real ** -1)
+    assert np.all(np.diff(T00) > -1e-10)
