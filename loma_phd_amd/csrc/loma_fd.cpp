@@ -192,9 +192,18 @@ int fd_build(int my, const double* y, FdOps& op) {
     band_lu_solve(op.inv1.data(), op.up1.data(), op.lo1.data(), my, 3, w.data());
     for (int j = 0; j < my; ++j) op.fmap[j] = 1.0 / w[j];
   }
-  // ---- integration operator (inty8): LU of B1 itself ----
-  op.invi.resize(my); op.upi.resize((size_t)my * 3); op.loi.resize((size_t)my * 3);
-  band_lu(op.b1.data(), my, 3, op.invi.data(), op.upi.data(), op.loi.data());
+  // ---- integration operator (inty8, laps.f:484): B1 x = A1 f.  B1 has the
+  // constant vector in its null space (its rows sum to zero); the reference
+  // factors it anyway and removes the resulting huge constant by subtracting
+  // x(1).  The same function is obtained, without that cancellation, by
+  // replacing the last row of B1 with the gauge x(my) = 0.
+  {
+    std::vector<double> bi(op.b1);
+    for (int d = 0; d < 7; ++d) bi[(size_t)(my - 1) * 7 + d] = 0.0;
+    bi[(size_t)(my - 1) * 7 + 3] = 1.0;
+    op.invi.resize(my); op.upi.resize((size_t)my * 3); op.loi.resize((size_t)my * 3);
+    band_lu(bi.data(), my, 3, op.invi.data(), op.upi.data(), op.loi.data());
+  }
   return LOMA_OK;
 }
 

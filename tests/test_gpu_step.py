@@ -72,13 +72,25 @@ def test_steps_fp64(gpu, oracle_mod, shape):
 
 
 def test_steps_mixed(gpu, oracle_mod):
-    from loma_phd_amd import capi
+    """Mixed form (FP32 transforms) vs the FP64 oracle after 10 steps, per
+    physical field u, v, w (as rho*u_i), T and Z, means included: 1e-5."""
+    from loma_phd_amd import capi, diagnostics, ic
     cfg = _cfg(64, 129, 64, Re=160.0, s=2.0, seed=5)
     print("10 steps mixed 64x129x64", flush=True)
     run, S, g = _setup(cfg, capi.LOMA_MIXED)
     run.advance(10)
     S.step(10)
-    _compare(run, S, 1e-5, "10 steps mixed")
+    a, b = run.download(), S.get()
+    ops = ic.HostOps(cfg.y())
+    errs = diagnostics.field_errors(a, b, ops, cfg.alp, cfg.bet)
+    spec = {f: rel_l2(a[f], b[f]) for f in ("vor", "phi", "psi")}
+    print("10 steps mixed, physical fields", errs, "spectral perturbations", spec, flush=True)
+    for f, e in errs.items():
+        assert e < 1e-5, (f, e)
+    # the perturbation spectra carry FP32 transform rounding relative to the
+    # mean flow: bounded, not FP64-small
+    for f, e in spec.items():
+        assert e < 1e-3, (f, e)
 
 
 def test_graph_replay_matches_eager(gpu):
