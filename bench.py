@@ -36,11 +36,17 @@ def algorithmic_bytes(run, with_z: bool):
     z = 1 if with_z else 0
     nin, nout = (12, 12) if with_z else (8, 11)
     k = {
-        "phaseA": (6 + z) * 16 * S + (2 + z) * 16 * S + nin * es * S,
+        # reads phi, ten13, vor, ten12, psi, T (+Z); writes phi, vor; writes
+        # the transform inputs
+        "phaseA": (6 + z) * 16 * S + 2 * 16 * S + nin * es * S,
         "zfft_inv": nin * es * S + nin * es * W,
         "xline": nin * es * W + nout * es * W + (1 + z) * 2 * 8 * P,
         "zfft_fwd": nout * es * W + nout * es * S,
-        "phaseC": nout * es * S + (3 + z) * 16 * S + (6 + z) * 16 * S,
+        # reads the transform outputs but drho; reads T (+Z), vor, phi;
+        # writes T (+Z), vor, phi, RHS(vor), RHS(phi)
+        "phaseC": (nout - 1) * es * S + (3 + z) * 16 * S + (5 + z) * 16 * S,
+        # reads drho, writes psi
+        "phaseC2": es * S + 16 * S,
     }
     step = 3 * sum(k.values())
     return k, step

@@ -110,7 +110,8 @@ int loma_use_graph(loma_run *run, int enable);
 /* Per-kernel timing with HIP events recorded on the run's stream around
  * every launch (graph replay is bypassed while enabled).  profile_read
  * returns, per kernel class (0 phase A, 1 z inverse FFT, 2 x lines,
- * 3 z forward FFT, 4 phase C, 5 end of step), the summed milliseconds and
+ * 3 z forward FFT, 4 phase C, 5 end of step, 6 phase C psi solve), the
+ * summed milliseconds and
  * launch counts since the last read. */
 int loma_profile(loma_run *run, int enable);
 int loma_profile_read(loma_run *run, double *total_ms, int *counts, int nclasses);

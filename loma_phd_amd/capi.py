@@ -30,7 +30,7 @@ EXPORTS = [
     "loma_fourz_dev", "loma_band5_solve_dev", "loma_advance_many", "loma_advance_async",
     "loma_sync", "loma_stream", "loma_profile", "loma_profile_read",
 ]
-KERNEL_CLASSES = ["phaseA", "zfft_inv", "xline", "zfft_fwd", "phaseC", "finish"]
+KERNEL_CLASSES = ["phaseA", "zfft_inv", "xline", "zfft_fwd", "phaseC", "finish", "phaseC2"]
 
 _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)

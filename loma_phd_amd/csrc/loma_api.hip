@@ -6,6 +6,7 @@
 // calls; host transfers happen only at upload/download (restart boundary).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -44,7 +45,7 @@ void set_error(const std::string& msg) { g_err = msg; }
 struct loma_run {
   int mgalx = 0, my = 0, mgalz = 0, mx = 0, mz = 0, nplanes = 0, nz = 0;
   int M = 0;            // rows per lane of the y-column solver
-  int rows_pad = 0;     // rows of every per-row table incl. padding
+  int lu_stride = 0;    // doubles per Helmholtz class table (5 * 64 * M)
   int precision = 0, flags = 0;
   bool fp32 = false, with_z = false;
   double Re = 0, alp = 0, bet = 0, pr = 0.7, cfl = 0.5, dt0 = 1e-4;
@@ -60,11 +61,13 @@ struct loma_run {
   double* dt21 = nullptr;  // plain (my,5) row-major copies for the factor kernel
   double* dt22 = nullptr;
   double* helm_lu = nullptr;
+  double* helm_bc = nullptr;  // per class Neumann constants
   int nclass = 0;
   double2 *vor = nullptr, *phi = nullptr, *psi = nullptr, *scal = nullptr;
   double2 *ten12 = nullptr, *ten13 = nullptr, *zf = nullptr;
   void* spec[12] = {};
   void* wbuf[12] = {};
+  void* wout[12] = {};
   double *rhst = nullptr, *rhsz = nullptr, *hy = nullptr;
   double *u00wk = nullptr, *v00wk = nullptr, *w00wk = nullptr, *rf0u = nullptr,
          *rf0w = nullptr;
@@ -85,7 +88,7 @@ struct loma_run {
 };
 
 // kernel classes for profiling
-enum { KC_PHASEA = 0, KC_ZINV, KC_XLINE, KC_ZFWD, KC_PHASEC, KC_FINISH, KC_COUNT };
+enum { KC_PHASEA = 0, KC_ZINV, KC_XLINE, KC_ZFWD, KC_PHASEC, KC_FINISH, KC_PHASEC2, KC_COUNT };
 
 static int prof_begin(loma_run* r, int cls, hipEvent_t* a) {
   if (!r->prof) return 0;
@@ -202,7 +205,6 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   }
   r->M = pick_M(my);
   if (r->M < 0) { set_error("loma_init: my too large"); delete r; return LOMA_ERR_ARG; }
-  r->rows_pad = 64 * r->M + 2 * PADR;
   int rc = fd_build(my, y, r->fd);
   if (rc) { set_error("loma_init: FD setup failed"); delete r; return rc; }
   // run constants
@@ -215,21 +217,48 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   c.cfl = cfl;
   c.dt_fp32 = (flags & LOMA_FLAG_DT_FP32) ? 1 : 0;
 
-  // ---- per-row tables (rows shifted by PADR, zero padded) ----
-  const int R = r->rows_pad;
-  // offsets (in doubles) of each table inside one allocation
-  const int nb[14] = {7, 7, 1, 3, 3, 1, 5, 5, 1, 2, 2, 1, 3, 3};
-  size_t off[14], tot = 0;
-  for (int q = 0; q < 14; ++q) { off[q] = tot; tot += (size_t)nb[q] * R; }
+  // ---- per-row tables, lane-major over 64*M rows, zero padded ----
+  const int R = 64 * r->M;
+  const int M = r->M;
+  const FdOps& f = r->fd;
+  // boundary-derivative functionals: w0[j] = d/dy(e_j) at row 0, w1[j] at
+  // row my-1 (fmap included), used by the Neumann solve
+  std::vector<double> wbd((size_t)2 * my, 0.0);
+  {
+    std::vector<double> e(my), d(my);
+    for (int j = 0; j < my; ++j) {
+      std::fill(e.begin(), e.end(), 0.0);
+      e[j] = 1.0;
+      band_apply(f.b1.data(), my, 3, e.data(), d.data());
+      band_lu_solve(f.inv1.data(), f.up1.data(), f.lo1.data(), my, 3, d.data());
+      wbd[(size_t)j * 2 + 0] = d[0] * f.fmap[0];
+      wbd[(size_t)j * 2 + 1] = d[my - 1] * f.fmap[my - 1];
+    }
+  }
+  const int NT = 15;
+  const int nb[NT] = {7, 7, 1, 3, 3, 1, 5, 5, 1, 2, 2, 1, 3, 3, 2};
+  size_t off[NT], tot = 0;
+  for (int q = 0; q < NT; ++q) { off[q] = tot; tot += (size_t)nb[q] * R; }
   std::vector<double> h(tot, 0.0);
+  auto lm = [&](int row, int w, int idx) -> size_t {
+    const int l = row / M, t = row - l * M;
+    return (size_t)(t * w + idx) * 64 + l;
+  };
+  // row-indexed table -> lane-major
   auto put = [&](int q, const std::vector<double>& src, int w) {
     for (int j = 0; j < my; ++j)
-      for (int d = 0; d < w; ++d) h[off[q] + (size_t)(j + PADR) * w + d] = src[(size_t)j * w + d];
+      for (int d = 0; d < w; ++d) h[off[q] + lm(j, w, d)] = src[(size_t)j * w + d];
   };
-  const FdOps& f = r->fd;
-  put(0, f.b1, 7); put(1, f.a1, 7); put(2, f.inv1, 1); put(3, f.up1, 3); put(4, f.lo1, 3);
+  // lower multipliers: lo[k][i-1] (row k into row k+i) -> destination row k+i
+  auto put_lo = [&](int q, const std::vector<double>& src, int w) {
+    for (int j = 0; j < my; ++j)
+      for (int i = 1; i <= w; ++i)
+        if (j - i >= 0) h[off[q] + lm(j, w, i - 1)] = src[(size_t)(j - i) * w + i - 1];
+  };
+  put(0, f.b1, 7); put(1, f.a1, 7); put(2, f.inv1, 1); put(3, f.up1, 3); put_lo(4, f.lo1, 3);
   put(5, f.fmap, 1); put(6, f.b2, 5); put(7, f.a2, 5); put(8, f.inv2, 1); put(9, f.up2, 2);
-  put(10, f.lo2, 2); put(11, f.invi, 1); put(12, f.upi, 3); put(13, f.loi, 3);
+  put_lo(10, f.lo2, 2); put(11, f.invi, 1); put(12, f.upi, 3); put_lo(13, f.loi, 3);
+  put(14, wbd, 2);
   if ((rc = dalloc(r, &r->tab, tot * sizeof(double)))) { loma_free(r); return rc; }
   if (hipMemcpy(r->tab, h.data(), tot * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
     set_error("loma_init: table upload failed"); loma_free(r); return LOMA_ERR_DEVICE;
@@ -239,7 +268,7 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   T.up1 = r->tab + off[3]; T.lo1 = r->tab + off[4]; T.fmap = r->tab + off[5];
   T.b2 = r->tab + off[6]; T.a2 = r->tab + off[7]; T.inv2 = r->tab + off[8];
   T.up2 = r->tab + off[9]; T.lo2 = r->tab + off[10]; T.invi = r->tab + off[11];
-  T.upi = r->tab + off[12]; T.loi = r->tab + off[13];
+  T.upi = r->tab + off[12]; T.loi = r->tab + off[13]; T.wbd = r->tab + off[14];
   T.my = my;
   // plain band copies for the Helmholtz factor kernel
   if ((rc = dalloc(r, &r->dt21, sizeof(double) * 5 * my))) { loma_free(r); return rc; }
@@ -248,6 +277,7 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   hipMemcpy(r->dt22, f.b2.data(), sizeof(double) * 5 * my, hipMemcpyHostToDevice);
   // ---- Helmholtz LU per (kx, |kz|) class: rK = kx^2 + kz^2 ----
   r->nclass = r->nplanes * (r->nz + 1);
+  r->lu_stride = 5 * R;
   {
     std::vector<double> shifts(r->nclass);
     for (int i = 0; i < r->nplanes; ++i)
@@ -258,10 +288,15 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
     double* dsh = nullptr;
     if ((rc = dalloc(r, &dsh, sizeof(double) * r->nclass))) { loma_free(r); return rc; }
     hipMemcpy(dsh, shifts.data(), sizeof(double) * r->nclass, hipMemcpyHostToDevice);
-    if ((rc = dalloc(r, &r->helm_lu, sizeof(double) * 5 * (size_t)R * r->nclass))) {
+    if ((rc = dalloc(r, &r->helm_lu, sizeof(double) * (size_t)r->lu_stride * r->nclass))) {
       loma_free(r); return rc;
     }
-    if (launch_helm_factor(r->dt21, r->dt22, my, R, dsh, r->nclass, r->helm_lu, 0)) {
+    if ((rc = dalloc(r, &r->helm_bc, sizeof(double) * 4 * (size_t)r->nclass))) {
+      loma_free(r); return rc;
+    }
+    if (launch_helm_factor(r->dt21, r->dt22, my, M, dsh, r->nclass, r->helm_lu, 0) ||
+        launch_helm_bc(M, r->T, r->helm_lu, r->helm_bc, r->nclass, 0) ||
+        hipDeviceSynchronize() != hipSuccess) {
       set_error("loma_init: factor launch failed"); loma_free(r); return LOMA_ERR_DEVICE;
     }
   }
@@ -278,6 +313,7 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   for (int q = 0; q < nsp; ++q) {
     if ((rc = dalloc(r, &r->spec[q], S * es))) { loma_free(r); return rc; }
     if ((rc = dalloc(r, &r->wbuf[q], W * es))) { loma_free(r); return rc; }
+    if ((rc = dalloc(r, &r->wout[q], W * es))) { loma_free(r); return rc; }
   }
   if ((rc = dalloc(r, &r->rhst, P * 8))) { loma_free(r); return rc; }
   if (r->with_z && (rc = dalloc(r, &r->rhsz, P * 8))) { loma_free(r); return rc; }
@@ -325,19 +361,29 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   // tile sizes: keep LDS per workgroup <= 64 KiB
   {
     const int e2 = r->fp32 ? 8 : 16;
-    int J = 16;
-    while (J > 1 && (size_t)2 * J * mgalz * e2 > 65536) J /= 2;
-    r->J = J;
-    int pairs = 1024 / mgalx;
-    if (pairs < 1) pairs = 1;
-    if (pairs > 8) pairs = 8;
-    while (pairs > 1 && (size_t)2 * pairs * mgalx * e2 > 65536) pairs /= 2;
-    r->L = 2 * pairs;
-    if (mgalx * pairs > 1024) {
-      set_error("loma_init: mgalx too large for the x-line kernel (max 1024)");
+    // z transforms: J y entries per workgroup, J * mgalz <= 256 * E
+    // (E = 16 in FP32, 8 in FP64): one LDS buffer of at most 32 KiB
+    const int capz = r->fp32 ? 4096 : 2048;
+    int J = 32;
+    while (J > 1 && J * mgalz > capz) J /= 2;
+    if (J * mgalz > capz) {
+      set_error("loma_init: mgalz too large for the z-transform kernel");
       loma_free(r);
       return LOMA_ERR_ARG;
     }
+    r->J = J;
+    // x lines: 2*pairs lines per 256-thread workgroup, pairs*mgalx entries
+    // at most 256*EPT (EPT = 4 in FP64, 8 in FP32): two LDS buffers of
+    // 32 KiB together
+    const int cap = r->fp32 ? 2048 : 1024;
+    int pairs = cap / mgalx;
+    if (pairs > 16) pairs = 16;
+    if (pairs < 1) {
+      set_error("loma_init: mgalx too large for the x-line kernel");
+      loma_free(r);
+      return LOMA_ERR_ARG;
+    }
+    r->L = 2 * pairs;
   }
   if (hipDeviceSynchronize() != hipSuccess) {
     set_error("loma_init: device error during set-up"); loma_free(r); return LOMA_ERR_DEVICE;
@@ -474,7 +520,8 @@ static PhaseArgs phase_args(loma_run* r, int rkstep) {
   a.rc = r->rc;
   a.T = r->T;
   a.helm_lu = r->helm_lu;
-  a.lu_rows = r->rows_pad;
+  a.helm_bc = r->helm_bc;
+  a.lu_stride = r->lu_stride;
   a.vor = r->vor; a.phi = r->phi; a.psi = r->psi; a.scal = r->scal;
   a.ten12 = r->ten12; a.ten13 = r->ten13; a.zf = r->zf;
   for (int q = 0; q < 12; ++q) a.spec[q] = r->spec[q];
@@ -495,7 +542,11 @@ static FftArgs fft_args(loma_run* r, int rkstep, int do_cfl) {
   a.plx = r->plx;
   a.twz = r->twz;
   a.twx = r->twx;
-  for (int q = 0; q < 12; ++q) { a.spec[q] = r->spec[q]; a.wbuf[q] = r->wbuf[q]; }
+  for (int q = 0; q < 12; ++q) {
+    a.spec[q] = r->spec[q];
+    a.wbuf[q] = r->wbuf[q];
+    a.wout[q] = r->wout[q];
+  }
   a.rhst = r->rhst;
   a.rhsz = r->rhsz;
   a.hy = r->hy;
@@ -529,6 +580,7 @@ static int enqueue_substep(loma_run* r, int rkstep, int icfl) {
   fa.nfields = r->with_z ? 12 : 11;
   PROF(KC_ZFWD, launch_zfft_fwd(r->fp32, fa, st));
   PROF(KC_PHASEC, launch_phaseC(r->M, r->fp32, pa, st));
+  PROF(KC_PHASEC2, launch_phaseC2(r->M, r->fp32, pa, st));
   return LOMA_OK;
 }
 
@@ -695,20 +747,22 @@ static int solve_cols(loma_run* r, const double* phi, double* v, double* dvdy,
   std::vector<double> h(ncols);
   HIPCHK(hipMemcpy(h.data(), rK, sizeof(double) * ncols, hipMemcpyDeviceToHost));
   for (double& x : h) x += shift;
-  double *sh = nullptr, *lu = nullptr;
+  double *sh = nullptr, *lu = nullptr, *bc = nullptr;
   HIPCHK(hipMalloc(&sh, sizeof(double) * ncols));
-  HIPCHK(hipMalloc(&lu, sizeof(double) * 5 * (size_t)r->rows_pad * ncols));
+  HIPCHK(hipMalloc(&lu, sizeof(double) * (size_t)r->lu_stride * ncols));
+  HIPCHK(hipMalloc(&bc, sizeof(double) * 4 * (size_t)ncols));
   int rc = LOMA_OK;
   if (hipMemcpy(sh, h.data(), sizeof(double) * ncols, hipMemcpyHostToDevice) != hipSuccess ||
-      launch_helm_factor(r->dt21, r->dt22, r->my, r->rows_pad, sh, ncols, lu, r->stream) ||
-      launch_lapvdvhom(r->M, r->T, lu, r->rows_pad, phi, v, dvdy, rK, ncols, mode,
-                       r->stream) ||
+      launch_helm_factor(r->dt21, r->dt22, r->my, r->M, sh, ncols, lu, r->stream) ||
+      launch_helm_bc(r->M, r->T, lu, bc, ncols, r->stream) ||
+      launch_lapvdvhom(r->M, r->T, lu, bc, phi, v, dvdy, rK, ncols, mode, r->stream) ||
       hipStreamSynchronize(r->stream) != hipSuccess) {
     set_error("solve_cols: device failure");
     rc = LOMA_ERR_DEVICE;
   }
   hipFree(sh);
   hipFree(lu);
+  hipFree(bc);
   return rc;
 }
 

@@ -3,28 +3,83 @@
 // One 64-lane wavefront owns one complex y-column (one (kx,kz) mode).  Lane l
 // holds the contiguous rows [l*M, l*M+M) in registers (rows >= my are zero
 // padding whose coefficients are zero).  Banded triangular solves are split
-// into a local sweep per lane, a log-step scan of the affine carry maps
-// across the wavefront (Hillis-Steele over 64 lanes with __shfl_up/down), and
-// a second local sweep from the exact incoming carry.  This gives the result
-// of the reference's sequential banbks/banbks7 sweeps (cfdiff.f:954-1051)
-// with all 64 lanes busy.
+// into an in-place local sweep per lane, a log-step scan of the affine carry
+// maps across the wavefront (Hillis-Steele over 64 lanes), and a correction
+// sweep driven by the exact incoming carry.  This gives the result of the
+// reference's sequential banbks/banbks7 sweeps (cfdiff.f:954-1051) with all
+// 64 lanes busy.
+//
+// Per-row coefficient tables are stored lane-major: entry `idx` of row
+// (lane*M + t) sits at base[(t*ncol + idx)*64 + lane], so every table load of
+// a wavefront is one contiguous 512-byte segment.  Lower-triangle
+// multipliers are stored by destination row (lo(r, i-1) multiplies row r-i
+// into row r), so every lane reads only its own rows.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace loma {
 
-constexpr int PADR = 4;  // leading zero rows in every per-row table
+constexpr int WAVE = 64;
 
-// element (row k, entry idx) of a row-major per-row table with `rs` doubles
-// per row starting at column `off`; rows are shifted by PADR.
-struct BandRef {
-  const double* __restrict__ p;
-  int rs;
+// Scheduling fence between the stages of an operator (and between
+// operators): without it the scheduler hoists the table loads of later
+// stages over the scans, holding every coefficient of the operator in
+// registers at once.
+__device__ __forceinline__ void op_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// ---- buffer addressing ----------------------------------------------------
+// Column and table accesses go through buffer resources held in scalar
+// registers: the per-row and per-entry offsets are scalar (or instruction
+// immediates), the only vector offset is the lane's, and the hardware range
+// check replaces per-row bounds tests (out-of-range loads return 0,
+// out-of-range stores are dropped).  The range check covers the vector
+// offset and the immediate, not the scalar offset, so anything that may run
+// past the end is folded into the resource base instead.
+constexpr int RSRC_WORD3 = 0x00020000;  // raw 32-bit data format (CDNA)
+
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0,
+                                           bytes < 0 ? 0 : bytes, RSRC_WORD3);
+}
+
+// lane-major table view: resource over the table, entries per row, first
+// entry used, this lane's byte offset and a scalar bias (see relaunder)
+struct TabRef {
+  __amdgpu_buffer_rsrc_t r;
+  int ncol;
   int off;
+  int lane8;
+  int sbias;
 };
-__device__ __forceinline__ double bget(const BandRef& b, int k, int idx) {
-  return b.p[(k + PADR) * b.rs + b.off + idx];
+__device__ __forceinline__ TabRef tab(const double* base, int ncol, int off, int lane) {
+  return TabRef{make_rsrc(base, 0x7fffffff), ncol, off, lane * 8, 0};
+}
+__device__ __forceinline__ double tget(const TabRef& b, int t, int idx) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(
+      b.r, b.lane8, b.sbias + (t * b.ncol + b.off + idx) * (WAVE * 8), 0);
+  return __builtin_bit_cast(double, v);
+}
+// The same view, opaque to the optimiser: loads through it are issued again
+// instead of keeping coefficients loaded earlier live across a scan (table
+// reloads hit the cache; held coefficients cost registers and occupancy).
+__device__ __forceinline__ TabRef relaunder(const TabRef& b) {
+  int z = b.sbias;
+  asm volatile("" : "+s"(z));
+  return TabRef{b.r, b.ncol, b.off, b.lane8, z};
+}
+// index of entry `idx` of grid row `row` in a lane-major table
+template <int M>
+__host__ __device__ inline int lane_major_index(int row, int ncol, int idx) {
+  const int l = row / M, t = row - l * M;
+  return (t * ncol + idx) * WAVE + l;
+}
+// entry `idx` of grid row `row` (any lane's) of a lane-major table
+template <int M>
+__device__ __forceinline__ double trow(const double* base, int ncol, int idx, int row) {
+  return base[lane_major_index<M>(row, ncol, idx)];
 }
 
 // ---- wavefront scans of affine maps s_out = A s_in + c -----------------
@@ -32,7 +87,7 @@ template <int P, int NC>
 __device__ __forceinline__ void scan_affine_up(double (&A)[P][P], double (&c)[NC][P],
                                                int lane) {
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
+  for (int d = 1; d < WAVE; d <<= 1) {
     double Ap[P][P], cp[NC][P];
 #pragma unroll
     for (int q = 0; q < P; ++q)
@@ -78,7 +133,7 @@ template <int P, int NC>
 __device__ __forceinline__ void scan_affine_down(double (&A)[P][P], double (&c)[NC][P],
                                                  int lane) {
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
+  for (int d = 1; d < WAVE; d <<= 1) {
     double Ap[P][P], cp[NC][P];
 #pragma unroll
     for (int q = 0; q < P; ++q)
@@ -88,7 +143,7 @@ __device__ __forceinline__ void scan_affine_down(double (&A)[P][P], double (&c)[
     for (int n = 0; n < NC; ++n)
 #pragma unroll
       for (int q = 0; q < P; ++q) cp[n][q] = __shfl_down(c[n][q], d);
-    if (lane + d < 64) {
+    if (lane + d < WAVE) {
       double cn[NC][P], An[P][P];
 #pragma unroll
       for (int n = 0; n < NC; ++n)
@@ -120,157 +175,167 @@ __device__ __forceinline__ void scan_affine_down(double (&A)[P][P], double (&c)[
   }
 }
 
-// ---- unit lower solve: w_r = x_r - sum_{i=P..1} L(r-i,i) w_{r-i} ---------
-// lo(k, i-1) is the multiplier row k contributes to row k+i.
+// ---- unit lower solve: w_r = x_r - sum_{i=1..P} L(r,i) w_{r-i} ------------
+// lo(t, i-1) = L(r0+t, i): multiplier of row r0+t-i into row r0+t.
 template <int M, int P, int NC>
-__device__ __forceinline__ void lsolve(const BandRef lo, int r0, int lane,
-                                       double (&x)[NC][M]) {
+__device__ __forceinline__ void lsolve(const TabRef lo, int lane, double (&x)[NC][M]) {
   static_assert(P <= M, "chunk shorter than bandwidth");
-  // carry of this chunk with zero incoming state
-  double c[NC][P];
-  {
-    double w[NC][M];
+  // (1) in place, zero incoming state
 #pragma unroll
-    for (int t = 0; t < M; ++t) {
+  for (int t = 1; t < M; ++t)
 #pragma unroll
-      for (int n = 0; n < NC; ++n) w[n][t] = x[n][t];
+    for (int i = 1; i <= P; ++i) {
+      if (t - i < 0) continue;
+      const double l = tget(lo, t, i - 1);
 #pragma unroll
-      for (int i = P; i >= 1; --i) {
-        if (t - i < 0) continue;
-        double l = bget(lo, r0 + t - i, i - 1);
-#pragma unroll
-        for (int n = 0; n < NC; ++n) w[n][t] -= l * w[n][t - i];
-      }
+      for (int n = 0; n < NC; ++n) x[n][t] -= l * x[n][t - i];
     }
+  op_fence();
+  // carry: last P rows, c[q] = w_{r0+M-1-q}
+  double c[NC][P];
 #pragma unroll
-    for (int n = 0; n < NC; ++n)
+  for (int n = 0; n < NC; ++n)
 #pragma unroll
-      for (int q = 1; q <= P; ++q) c[n][q - 1] = w[n][M - q];
-  }
-  // response of the carry to a unit incoming state s_b = w_{r0-b}
+    for (int q = 0; q < P; ++q) c[n][q] = x[n][M - 1 - q];
+  // (2) response of the carry to a unit incoming state s_b = w_{r0-1-b}
   double A[P][P];
 #pragma unroll
-  for (int b = 1; b <= P; ++b) {
+  for (int b = 0; b < P; ++b) {
     double g[M];
 #pragma unroll
     for (int t = 0; t < M; ++t) {
       double acc = 0.0;
 #pragma unroll
-      for (int i = P; i >= 1; --i) {
-        double pv = (t - i >= 0) ? g[t - i] : ((i - t) == b ? 1.0 : 0.0);
-        acc -= bget(lo, r0 + t - i, i - 1) * pv;
+      for (int i = 1; i <= P; ++i) {
+        if (t - i >= 0) acc -= tget(lo, t, i - 1) * g[t - i];
+        else if (i - t - 1 == b) acc -= tget(lo, t, i - 1);
       }
       g[t] = acc;
     }
 #pragma unroll
-    for (int q = 1; q <= P; ++q) A[q - 1][b - 1] = g[M - q];
+    for (int q = 0; q < P; ++q) A[q][b] = g[M - 1 - q];
   }
+  op_fence();
   scan_affine_up<P, NC>(A, c, lane);
+  op_fence();
+  // (3) exact incoming state and the correction it causes
   double s[NC][P];
 #pragma unroll
   for (int n = 0; n < NC; ++n)
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-      double v = __shfl_up(c[n][q], 1);
+      const double v = __shfl_up(c[n][q], 1);
       s[n][q] = lane == 0 ? 0.0 : v;
     }
+  const TabRef lo2 = relaunder(lo);
+  double dl[NC][M];
 #pragma unroll
   for (int t = 0; t < M; ++t) {
 #pragma unroll
-    for (int i = P; i >= 1; --i) {
-      double l = bget(lo, r0 + t - i, i - 1);
+    for (int n = 0; n < NC; ++n) dl[n][t] = 0.0;
+#pragma unroll
+    for (int i = 1; i <= P; ++i) {
+      const double l = tget(lo2, t, i - 1);
 #pragma unroll
       for (int n = 0; n < NC; ++n) {
-        double pv = (t - i >= 0) ? x[n][t - i] : s[n][i - t - 1];
-        x[n][t] -= l * pv;
+        const double pv = (t - i >= 0) ? dl[n][t - i] : s[n][i - t - 1];
+        dl[n][t] -= l * pv;
       }
     }
+#pragma unroll
+    for (int n = 0; n < NC; ++n) x[n][t] += dl[n][t];
   }
 }
 
 // ---- upper solve: x_r = (w_r - sum_{d=1..P} U(r,d) x_{r+d}) * inv_r ------
 template <int M, int P, int NC>
-__device__ __forceinline__ void usolve(const BandRef up, const BandRef inv, int r0,
-                                       int lane, double (&x)[NC][M]) {
-  double c[NC][P];
-  {
-    double w[NC][M];
+__device__ __forceinline__ void usolve(const TabRef up, const TabRef inv, int lane,
+                                       double (&x)[NC][M]) {
 #pragma unroll
-    for (int t = M - 1; t >= 0; --t) {
+  for (int t = M - 1; t >= 0; --t) {
 #pragma unroll
-      for (int n = 0; n < NC; ++n) w[n][t] = x[n][t];
+    for (int d = 1; d <= P; ++d) {
+      if (t + d >= M) continue;
+      const double u = tget(up, t, d - 1);
 #pragma unroll
-      for (int d = 1; d <= P; ++d) {
-        if (t + d >= M) continue;
-        double u = bget(up, r0 + t, d - 1);
-#pragma unroll
-        for (int n = 0; n < NC; ++n) w[n][t] -= u * w[n][t + d];
-      }
-      double iv = bget(inv, r0 + t, 0);
-#pragma unroll
-      for (int n = 0; n < NC; ++n) w[n][t] *= iv;
+      for (int n = 0; n < NC; ++n) x[n][t] -= u * x[n][t + d];
     }
+    const double iv = tget(inv, t, 0);
 #pragma unroll
-    for (int n = 0; n < NC; ++n)
-#pragma unroll
-      for (int q = 0; q < P; ++q) c[n][q] = w[n][q];
+    for (int n = 0; n < NC; ++n) x[n][t] *= iv;
   }
+  op_fence();
+  // carry: first P rows, c[q] = x_{r0+q}
+  double c[NC][P];
+#pragma unroll
+  for (int n = 0; n < NC; ++n)
+#pragma unroll
+    for (int q = 0; q < P; ++q) c[n][q] = x[n][q];
   double A[P][P];
 #pragma unroll
-  for (int b = 1; b <= P; ++b) {
+  for (int b = 0; b < P; ++b) {
     double g[M];
 #pragma unroll
     for (int t = M - 1; t >= 0; --t) {
       double acc = 0.0;
 #pragma unroll
       for (int d = 1; d <= P; ++d) {
-        double pv = (t + d < M) ? g[t + d] : ((t + d - M + 1) == b ? 1.0 : 0.0);
-        acc -= bget(up, r0 + t, d - 1) * pv;
+        if (t + d < M) acc -= tget(up, t, d - 1) * g[t + d];
+        else if (t + d - M == b) acc -= tget(up, t, d - 1);
       }
-      g[t] = acc * bget(inv, r0 + t, 0);
+      g[t] = acc * tget(inv, t, 0);
     }
 #pragma unroll
-    for (int q = 0; q < P; ++q) A[q][b - 1] = g[q];
+    for (int q = 0; q < P; ++q) A[q][b] = g[q];
   }
+  op_fence();
   scan_affine_down<P, NC>(A, c, lane);
+  op_fence();
   double s[NC][P];
 #pragma unroll
   for (int n = 0; n < NC; ++n)
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-      double v = __shfl_down(c[n][q], 1);
-      s[n][q] = lane == 63 ? 0.0 : v;
+      const double v = __shfl_down(c[n][q], 1);
+      s[n][q] = lane == WAVE - 1 ? 0.0 : v;
     }
+  const TabRef up2 = relaunder(up), inv2 = relaunder(inv);
+  double dl[NC][M];
 #pragma unroll
   for (int t = M - 1; t >= 0; --t) {
 #pragma unroll
+    for (int n = 0; n < NC; ++n) dl[n][t] = 0.0;
+#pragma unroll
     for (int d = 1; d <= P; ++d) {
-      double u = bget(up, r0 + t, d - 1);
+      const double u = tget(up2, t, d - 1);
 #pragma unroll
       for (int n = 0; n < NC; ++n) {
-        double pv = (t + d < M) ? x[n][t + d] : s[n][t + d - M];
-        x[n][t] -= u * pv;
+        const double pv = (t + d < M) ? dl[n][t + d] : s[n][t + d - M];
+        dl[n][t] -= u * pv;
       }
     }
-    double iv = bget(inv, r0 + t, 0);
+    const double iv = tget(inv2, t, 0);
 #pragma unroll
-    for (int n = 0; n < NC; ++n) x[n][t] *= iv;
+    for (int n = 0; n < NC; ++n) {
+      dl[n][t] *= iv;
+      x[n][t] += dl[n][t];
+    }
   }
 }
 
 // ---- banded matvec y_r = sum_d B(r,d) u_{r-P+d} ---------------------------
 template <int M, int P, int NC>
-__device__ __forceinline__ void bmatvec(const BandRef B, int r0, int lane,
-                                        const double (&u)[NC][M], double (&y)[NC][M]) {
+__device__ __forceinline__ void bmatvec(const TabRef B, int lane, const double (&u)[NC][M],
+                                        double (&y)[NC][M]) {
   double hl[NC][P], hr[NC][P];
 #pragma unroll
   for (int n = 0; n < NC; ++n)
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-      double a = __shfl_up(u[n][M - P + q], 1);
-      double b = __shfl_down(u[n][q], 1);
+      const double a = __shfl_up(u[n][M - P + q], 1);
+      const double b = __shfl_down(u[n][q], 1);
       hl[n][q] = lane == 0 ? 0.0 : a;
-      hr[n][q] = lane == 63 ? 0.0 : b;
+      hr[n][q] = lane == WAVE - 1 ? 0.0 : b;
     }
 #pragma unroll
   for (int t = 0; t < M; ++t) {
@@ -280,10 +345,10 @@ __device__ __forceinline__ void bmatvec(const BandRef B, int r0, int lane,
 #pragma unroll
     for (int d = 0; d <= 2 * P; ++d) {
       const int col = t - P + d;
-      double bv = bget(B, r0 + t, d);
+      const double bv = tget(B, t, d);
 #pragma unroll
       for (int n = 0; n < NC; ++n) {
-        double v = col < 0 ? hl[n][col + P] : (col >= M ? hr[n][col - M] : u[n][col]);
+        const double v = col < 0 ? hl[n][col + P] : (col >= M ? hr[n][col - M] : u[n][col]);
         acc[n] += bv * v;
       }
     }
@@ -303,60 +368,70 @@ __device__ __forceinline__ double col_value(const double (&x)[M], int row) {
   return __shfl(mine, l);
 }
 
-// Device tables of one y grid, rows shifted by PADR, zero padded.
+// Device tables of one y grid, lane-major, zero padded to 64*M rows.
 struct YTab {
-  const double* b1;    // [rows][7]  RHS of d/deta (dt12)
-  const double* a1;    // [rows][7]  LHS of d/deta (dt11, unfactored)
-  const double* inv1;  // [rows]     LU of a1
-  const double* up1;   // [rows][3]
-  const double* lo1;   // [rows][3]
-  const double* fmap;  // [rows]
-  const double* b2;    // [rows][5]  RHS of d2/dy2 (dt22)
-  const double* a2;    // [rows][5]  LHS of d2/dy2 (dt21, unfactored)
+  const double* b1;    // [7]  RHS of d/deta (dt12)
+  const double* a1;    // [7]  LHS of d/deta (dt11, unfactored)
+  const double* inv1;  // [1]  LU of a1
+  const double* up1;   // [3]
+  const double* lo1;   // [3]  (by destination row)
+  const double* fmap;  // [1]
+  const double* b2;    // [5]  RHS of d2/dy2 (dt22)
+  const double* a2;    // [5]  LHS of d2/dy2 (dt21, unfactored)
   const double* inv2;  // LU of a2
-  const double* up2;   // [rows][2]
-  const double* lo2;   // [rows][2]
+  const double* up2;   // [2]
+  const double* lo2;   // [2]
   const double* invi;  // LU of b1 (integration)
-  const double* upi;   // [rows][3]
-  const double* loi;   // [rows][3]
+  const double* upi;   // [3]
+  const double* loi;   // [3]
+  const double* wbd;   // [2]  d/dy at row 0 and at row my-1 as row functionals
   int my;
 };
 
 // first derivative du = fmap * A1^{-1} B1 u   (deryr2, cfdiff.f:518-601)
 template <int M, int NC>
-__device__ __forceinline__ void op_dy1(const YTab& T, int r0, int lane,
-                                       const double (&u)[NC][M], double (&du)[NC][M],
-                                       bool apply_fmap = true) {
-  bmatvec<M, 3, NC>(BandRef{T.b1, 7, 0}, r0, lane, u, du);
-  lsolve<M, 3, NC>(BandRef{T.lo1, 3, 0}, r0, lane, du);
-  usolve<M, 3, NC>(BandRef{T.up1, 3, 0}, BandRef{T.inv1, 1, 0}, r0, lane, du);
+__device__ __forceinline__ void op_dy1(const YTab& T, int lane, const double (&u)[NC][M],
+                                       double (&du)[NC][M], bool apply_fmap = true) {
+  bmatvec<M, 3, NC>(tab(T.b1, 7, 0, lane), lane, u, du);
+  op_fence();
+  lsolve<M, 3, NC>(tab(T.lo1, 3, 0, lane), lane, du);
+  op_fence();
+  usolve<M, 3, NC>(tab(T.up1, 3, 0, lane), tab(T.inv1, 1, 0, lane), lane, du);
+  op_fence();
   if (apply_fmap) {
+    const TabRef fm = tab(T.fmap, 1, 0, lane);
 #pragma unroll
     for (int t = 0; t < M; ++t) {
-      double f = bget(BandRef{T.fmap, 1, 0}, r0 + t, 0);
+      const double f = tget(fm, t, 0);
 #pragma unroll
       for (int n = 0; n < NC; ++n) du[n][t] *= f;
     }
   }
+  op_fence();
 }
 
 // second derivative d2u = A2^{-1} B2 u   (deryyr2, cfdiff.f:735-797)
 template <int M, int NC>
-__device__ __forceinline__ void op_dy2(const YTab& T, int r0, int lane,
-                                       const double (&u)[NC][M], double (&d2u)[NC][M]) {
-  bmatvec<M, 2, NC>(BandRef{T.b2, 5, 0}, r0, lane, u, d2u);
-  lsolve<M, 2, NC>(BandRef{T.lo2, 2, 0}, r0, lane, d2u);
-  usolve<M, 2, NC>(BandRef{T.up2, 2, 0}, BandRef{T.inv2, 1, 0}, r0, lane, d2u);
+__device__ __forceinline__ void op_dy2(const YTab& T, int lane, const double (&u)[NC][M],
+                                       double (&d2u)[NC][M]) {
+  bmatvec<M, 2, NC>(tab(T.b2, 5, 0, lane), lane, u, d2u);
+  op_fence();
+  lsolve<M, 2, NC>(tab(T.lo2, 2, 0, lane), lane, d2u);
+  op_fence();
+  usolve<M, 2, NC>(tab(T.up2, 2, 0, lane), tab(T.inv2, 1, 0, lane), lane, d2u);
+  op_fence();
 }
 
 // Helmholtz/Poisson solve (dt22 - rK dt21) v = rhs with identity end rows,
-// LU per mode class in `lu` ([rows][5] = inv, up1, up2, lo1, lo2), the RHS
-// being dt21*f with zero end rows (calcrhslap, laps.f:17-46).
+// LU per mode class in `lu` (lane-major, 5 entries per row: inv, up1, up2,
+// lo1, lo2), the RHS being dt21*f with zero end rows (calcrhslap,
+// laps.f:17-46).
 template <int M, int NC>
 __device__ __forceinline__ void op_helm(const YTab& T, const double* __restrict__ lu,
-                                        int r0, int lane, const double (&f)[NC][M],
+                                        int lane, const double (&f)[NC][M],
                                         double (&v)[NC][M]) {
-  bmatvec<M, 2, NC>(BandRef{T.a2, 5, 0}, r0, lane, f, v);
+  bmatvec<M, 2, NC>(tab(T.a2, 5, 0, lane), lane, f, v);
+  const int r0 = lane * M;
 #pragma unroll
   for (int t = 0; t < M; ++t) {
     const int r = r0 + t;
@@ -365,8 +440,117 @@ __device__ __forceinline__ void op_helm(const YTab& T, const double* __restrict_
       for (int n = 0; n < NC; ++n) v[n][t] = 0.0;
     }
   }
-  lsolve<M, 2, NC>(BandRef{lu, 5, 3}, r0, lane, v);
-  usolve<M, 2, NC>(BandRef{lu, 5, 1}, BandRef{lu, 5, 0}, r0, lane, v);
+  op_fence();
+  lsolve<M, 2, NC>(tab(lu, 5, 3, lane), lane, v);
+  op_fence();
+  usolve<M, 2, NC>(tab(lu, 5, 1, lane), tab(lu, 5, 0, lane), lane, v);
+  op_fence();
+}
+
+// ---- column staging through LDS ------------------------------------------
+// A wavefront moves a complex column between HBM and its registers through
+// a private LDS buffer of 64*M double2: the HBM side is contiguous across
+// lanes (row p = t*64 + lane), the register side is the lane's own chunk.
+// Only the owning wavefront touches its buffer, so a wavefront-scope fence
+// orders the cross-lane accesses.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// one element of a complex column: b64 (float2) or b128 (double2) buffer op
+// (the loaded vector is bit-cast as a whole: bit-casting its elements one by
+// one miscompiles to two copies of element 0 with this toolchain)
+__device__ __forceinline__ double2 buf_ld(__amdgpu_buffer_rsrc_t r, int voff, float2*) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  return make_double2((double)__builtin_bit_cast(float, (unsigned)u),
+                      (double)__builtin_bit_cast(float, (unsigned)(u >> 32)));
+}
+__device__ __forceinline__ double2 buf_ld(__amdgpu_buffer_rsrc_t r, int voff, double2*) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
+  const unsigned long long a = (unsigned long long)v[0] | ((unsigned long long)v[1] << 32);
+  const unsigned long long b = (unsigned long long)v[2] | ((unsigned long long)v[3] << 32);
+  return make_double2(__builtin_bit_cast(double, a), __builtin_bit_cast(double, b));
+}
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, int voff, double2 x, float2*) {
+  __attribute__((ext_vector_type(2))) unsigned v;
+  v[0] = __builtin_bit_cast(unsigned, (float)x.x);
+  v[1] = __builtin_bit_cast(unsigned, (float)x.y);
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, 0, 0);
+}
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, int voff, double2 x, double2*) {
+  const unsigned long long a = __builtin_bit_cast(unsigned long long, x.x);
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x.y);
+  __attribute__((ext_vector_type(4))) unsigned v;
+  v[0] = (unsigned)a; v[1] = (unsigned)(a >> 32);
+  v[2] = (unsigned)b; v[3] = (unsigned)(b >> 32);
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, 0, 0);
+}
+
+// HBM column (T2 = float2 or double2; `col` wavefront-uniform) -> LDS stage.
+// Row p = t*64 + lane; the resource of block t starts at row t*64 and ends at
+// row my, so rows >= my read as 0 without a test.  All M loads are issued
+// before the first is used.
+template <int M, typename T2>
+__device__ __forceinline__ void stage_in(const T2* __restrict__ col, int my, int lane,
+                                         double2* st) {
+  double2 q[M];
+#pragma unroll
+  for (int t = 0; t < M; ++t)
+    q[t] = buf_ld(make_rsrc(col + t * WAVE, (my - t * WAVE) * (int)sizeof(T2)),
+                  lane * (int)sizeof(T2), (T2*)nullptr);
+  wave_sync();
+#pragma unroll
+  for (int t = 0; t < M; ++t) st[t * WAVE + lane] = q[t];
+  wave_sync();
+}
+// LDS stage -> HBM column (rows >= my are dropped by the range check)
+template <int M, typename T2>
+__device__ __forceinline__ void stage_out(const double2* st, int my, int lane,
+                                          T2* __restrict__ col) {
+  wave_sync();
+  double2 v[M];
+#pragma unroll
+  for (int t = 0; t < M; ++t) v[t] = st[t * WAVE + lane];
+  wave_sync();
+#pragma unroll
+  for (int t = 0; t < M; ++t)
+    buf_st(make_rsrc(col + t * WAVE, (my - t * WAVE) * (int)sizeof(T2)),
+           lane * (int)sizeof(T2), v[t], (T2*)nullptr);
+}
+// stage -> registers (own chunk)
+template <int M>
+__device__ __forceinline__ void stage_get(const double2* st, int lane, double (&x)[2][M]) {
+  const int r0 = lane * M;
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    const double2 v = st[r0 + t];
+    x[0][t] = v.x;
+    x[1][t] = v.y;
+  }
+}
+// registers -> stage (own chunk)
+template <int M>
+__device__ __forceinline__ void stage_put(double2* st, int lane, const double (&x)[2][M]) {
+  const int r0 = lane * M;
+#pragma unroll
+  for (int t = 0; t < M; ++t) st[r0 + t] = make_double2(x[0][t], x[1][t]);
+}
+
+template <int M, typename T2>
+__device__ __forceinline__ void load_col(const T2* __restrict__ col, int my, int lane,
+                                         double2* st, double (&x)[2][M]) {
+  stage_in<M>(col, my, lane, st);
+  stage_get<M>(st, lane, x);
+}
+template <int M, typename T2>
+__device__ __forceinline__ void store_col(T2* __restrict__ col, int my, int lane,
+                                          double2* st, const double (&x)[2][M]) {
+  wave_sync();
+  stage_put<M>(st, lane, x);
+  stage_out<M>(st, my, lane, col);
 }
 
 }  // namespace loma

@@ -179,66 +179,135 @@ __device__ T2* fft_rows(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
   return a;
 }
 
+// In-place variant for one LDS buffer: each pass loads its items into
+// registers, synchronises, writes the butterflies back, synchronises.
+// nrows * N <= blockDim.x * E (E = complex entries per thread).
+template <int R, typename T, typename T2>
+__device__ __forceinline__ void radix_pass_ip(T2* a, int N, int nrows, int Ns, int tstep,
+                                              const T2* __restrict__ tw, int s, T2* v,
+                                              int nitems) {
+  const int nb = N / R;
+  const int total = nrows * nb;
+#pragma unroll
+  for (int k = 0; k < nitems; ++k) {
+    const int it = threadIdx.x + k * blockDim.x;
+    if (it < total) {
+      const int row = it / nb, j = it - row * nb;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[k * R + r] = a[row * N + j + r * nb];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < nitems; ++k) {
+    const int it = threadIdx.x + k * blockDim.x;
+    if (it < total) {
+      const int row = it / nb, j = it - row * nb;
+      const int jm = j % Ns;
+      T2* w = v + k * R;
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          T2 t = tw[jm * r * tstep];
+          if (s > 0) t.y = -t.y;
+          w[r] = cmul(w[r], t);
+        }
+      }
+      if (R == 2) dft2<T, T2>(w);
+      if (R == 3) dft3<T, T2>(w, s);
+      if (R == 4) dft4<T, T2>(w, s);
+      if (R == 5) dft5<T, T2>(w, s);
+      if (R == 8) dft8<T, T2>(w, s);
+      const int idxD = row * N + (j / Ns) * Ns * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[idxD + r * Ns] = w[r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, typename T2, int E>
+__device__ void fft_rows_ip(T2* a, int N, int nrows, const FftPlan& pl,
+                            const T2* __restrict__ tw, int s) {
+  T2 v[E + 8];
+  int Ns = 1;
+  for (int p = 0; p < pl.npass; ++p) {
+    const int R = pl.rad[p];
+    const int tstep = N / (Ns * R);
+    switch (R) {
+      case 2: radix_pass_ip<2, T, T2>(a, N, nrows, Ns, tstep, tw, s, v, (E + 1) / 2); break;
+      case 3: radix_pass_ip<3, T, T2>(a, N, nrows, Ns, tstep, tw, s, v, (E + 2) / 3); break;
+      case 4: radix_pass_ip<4, T, T2>(a, N, nrows, Ns, tstep, tw, s, v, (E + 3) / 4); break;
+      case 5: radix_pass_ip<5, T, T2>(a, N, nrows, Ns, tstep, tw, s, v, (E + 4) / 5); break;
+      default: radix_pass_ip<8, T, T2>(a, N, nrows, Ns, tstep, tw, s, v, (E + 7) / 8); break;
+    }
+    Ns *= R;
+  }
+}
+
 // ------------------------------------------------------------------------
 // z transforms: spectral [i][k][j] <-> z-physical [i][z][j]
 // (localyz2zy + fourz(+1), fou3D.f:103-146 / fourz(-1) + localzy2yz)
 // ------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_zfft_inv(FftArgs a) {
+// One workgroup: J consecutive y entries of one kx plane of one field, all
+// z; one LDS buffer, in-place FFT (J * mgalz <= 256 * E).
+template <typename T, int E>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_zfft_inv(FftArgs a) {
   using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = a.rc.mgalz, J = a.J, my = a.rc.my, mz = a.rc.mz, nz = a.rc.nz;
   T2* A = (T2*)smem;
-  T2* B = A + J * N;
   const int nyb = (my + J - 1) / J;
   const int i = blockIdx.x / nyb, j0 = (blockIdx.x - i * nyb) * J;
   const int f = blockIdx.y;
   const T2* src = (const T2*)a.spec[f];
   T2* dst = (T2*)a.wbuf[f];
-  for (int it = threadIdx.x; it < J * N; it += blockDim.x) { A[it].x = 0; A[it].y = 0; }
-  __syncthreads();
+  // the dealiased band nz+1 .. N-nz-1 is zero
+  const int nzero = N - mz;
+  for (int it = threadIdx.x; it < nzero * J; it += blockDim.x) {
+    const int q = it / J, jj = it - q * J;
+    A[jj * N + nz + 1 + q] = T2{0, 0};
+  }
   for (int it = threadIdx.x; it < mz * J; it += blockDim.x) {
     const int k = it / J, jj = it - k * J, j = j0 + jj;
-    if (j < my) {
-      const int z = (k <= nz) ? k : k + N - mz;
-      A[jj * N + z] = src[((size_t)i * mz + k) * my + j];
-    }
+    const int z = (k <= nz) ? k : k + N - mz;
+    T2 v = T2{0, 0};
+    if (j < my) v = src[((size_t)i * mz + k) * my + j];
+    A[jj * N + z] = v;
   }
   __syncthreads();
-  T2* R = fft_rows<T, T2>(A, B, N, J, a.plz, (const T2*)a.twz, +1);
+  fft_rows_ip<T, T2, E>(A, N, J, a.plz, (const T2*)a.twz, +1);
   for (int it = threadIdx.x; it < N * J; it += blockDim.x) {
     const int z = it / J, jj = it - z * J, j = j0 + jj;
-    if (j < my) dst[((size_t)i * N + z) * my + j] = R[jj * N + z];
+    if (j < my) dst[((size_t)i * N + z) * my + j] = A[jj * N + z];
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_zfft_fwd(FftArgs a) {
+template <typename T, int E>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_zfft_fwd(FftArgs a) {
   using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = a.rc.mgalz, J = a.J, my = a.rc.my, mz = a.rc.mz, nz = a.rc.nz;
   T2* A = (T2*)smem;
-  T2* B = A + J * N;
   const int nyb = (my + J - 1) / J;
   const int i = blockIdx.x / nyb, j0 = (blockIdx.x - i * nyb) * J;
   const int f = blockIdx.y;
-  const T2* src = (const T2*)a.wbuf[f];
+  const T2* src = (const T2*)a.wout[f];
   T2* dst = (T2*)a.spec[f];
   for (int it = threadIdx.x; it < N * J; it += blockDim.x) {
     const int z = it / J, jj = it - z * J, j = j0 + jj;
-    T2 v;
-    v.x = 0; v.y = 0;
+    T2 v = T2{0, 0};
     if (j < my) v = src[((size_t)i * N + z) * my + j];
     A[jj * N + z] = v;
   }
   __syncthreads();
-  T2* R = fft_rows<T, T2>(A, B, N, J, a.plz, (const T2*)a.twz, -1);
+  fft_rows_ip<T, T2, E>(A, N, J, a.plz, (const T2*)a.twz, -1);
   const T sc = (T)(1.0 / (double)N);
   for (int it = threadIdx.x; it < mz * J; it += blockDim.x) {
     const int k = it / J, jj = it - k * J, j = j0 + jj;
     if (j < my) {
       const int z = (k <= nz) ? k : k + N - mz;
-      T2 v = R[jj * N + z];
+      T2 v = A[jj * N + z];
       v.x *= sc;
       v.y *= sc;
       dst[((size_t)i * mz + k) * my + j] = v;
@@ -247,133 +316,155 @@ __global__ __launch_bounds__(256) void k_zfft_fwd(FftArgs a) {
 }
 
 // ------------------------------------------------------------------------
-// x lines: inverse x transform of 8 (12 with Z) fields, phys2 algebra,
-// forward x transform of 11 (12) fields, for L lines (fixed z, L
-// consecutive y) per workgroup.  Two real lines share one complex FFT.
+// x lines: inverse x transform of the 8 (12 with Z) transform inputs, phys2
+// algebra, forward x transform of the 11 (12) outputs, for 2*PAIRS lines
+// (fixed z, consecutive y) per 512-thread workgroup.  Two real lines share
+// one complex FFT.  The algebra runs in two passes so that only rho*u_i, T
+// and one accumulator per point stay in registers:
+//   pass 1: T, rho u, rho v, rho w -> u_i = rho u_i T and the six rho u_i u_j
+//   pass 2: lap T, T_x, T_y, T_z (and Z's) accumulated into
+//           r = T (lap T / Pe - rho u . grad T), then DT, drho (and DZ).
+// Every global access of a field is a run of 2*PAIRS consecutive y entries.
 // ------------------------------------------------------------------------
-constexpr int XL_EPT = 4;  // complex entries per thread (pairs*N <= 1024)
-
 __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
   if (!(v >= 0.0)) v = __longlong_as_double(0x7ff8000000000000LL);  // NaN -> max bits
   atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
-template <typename T, bool WZ>
-__global__ __launch_bounds__(256) void k_xline(FftArgs a) {
+constexpr int XL_THREADS = 256;
+
+// Workgroups are dealt to the 8 XCDs round-robin; this maps the launch so
+// that tiles t, t+1, ... (neighbouring y runs that share cache lines) run on
+// one XCD and share its L2.  Launch 8*ceil(n/8) groups; tiles >= n idle.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int per = (n + 7) / 8;
+  return (b & 7) * per + (b >> 3);
+}
+__host__ inline int xcd_groups(int n) { return 8 * ((n + 7) / 8); }
+
+// spectral lines of field `src` (W layout [i][z][j]) -> Hermitian-packed
+// complex rows in A (row p = lines j0+2p, j0+2p+1), then inverse FFT.
+// Returns the buffer holding the physical pairs.
+template <typename T, typename T2, int EPT>
+__device__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src, T2* A, T2* B, int z,
+                          int j0, int pairs) {
+  const int N = a.rc.mgalx, np = a.rc.nplanes, my = a.rc.my, mgz = a.rc.mgalz;
+  // the dealiased band np..N-np is zero
+  const int nzero = N - 2 * np + 1;
+  for (int it = threadIdx.x; it < pairs * nzero; it += blockDim.x) {
+    const int p = it / nzero, q = it - p * nzero;
+    A[p * N + np + q] = T2{0, 0};
+  }
+  for (int it = threadIdx.x; it < np * pairs; it += blockDim.x) {
+    const int ii = it / pairs, p = it - ii * pairs;
+    const int ja = j0 + 2 * p, jb = ja + 1;
+    const size_t base = ((size_t)ii * mgz + z) * my;
+    T2 ca = T2{0, 0}, cb = T2{0, 0};
+    if (ja < my) ca = src[base + ja];
+    if (jb < my) cb = src[base + jb];
+    T2* row = A + p * N;
+    if (ii == 0) {
+      // the imaginary part of the kx=0 coefficient is ignored (rft, rftsingle.f:87-91)
+      row[0] = T2{ca.x, cb.x};
+    } else {
+      row[ii] = T2{ca.x - cb.y, ca.y + cb.x};       // X_a + i X_b
+      row[N - ii] = T2{ca.x + cb.y, -ca.y + cb.x};  // conj(X_a) + i conj(X_b)
+    }
+  }
+  __syncthreads();
+  return fft_rows<T, T2>(A, B, N, pairs, a.plx, (const T2*)a.twx, +1);
+}
+
+// forward FFT of A (filled by the caller, then synchronised here) and
+// Hermitian unpacking of the pairs into field `dst` (scaled by 1/N)
+template <typename T, typename T2, int EPT>
+__device__ void xl_forward(const FftArgs& a, T2* __restrict__ dst, T2* A, T2* B, int z,
+                           int j0, int pairs) {
+  const int N = a.rc.mgalx, np = a.rc.nplanes, my = a.rc.my, mgz = a.rc.mgalz;
+  __syncthreads();
+  const T2* R = fft_rows<T, T2>(A, B, N, pairs, a.plx, (const T2*)a.twx, -1);
+  const T hs = (T)(0.5 / (double)N);
+  for (int it = threadIdx.x; it < np * pairs; it += blockDim.x) {
+    const int ii = it / pairs, p = it - ii * pairs;
+    const int ja = j0 + 2 * p, jb = ja + 1;
+    const T2* row = R + p * N;
+    const T2 zk = row[ii];
+    T2 zc = row[(N - ii) % N];
+    zc.y = -zc.y;
+    T2 ya{(zk.x + zc.x) * hs, (zk.y + zc.y) * hs};
+    T2 yb{(zk.y - zc.y) * hs, -(zk.x - zc.x) * hs};  // (zk - zc) / (2i)
+    if (ii == 0) { ya.y = 0; yb.y = 0; }
+    const size_t base = ((size_t)ii * mgz + z) * my;
+    if (ja < my) dst[base + ja] = ya;
+    if (jb < my) dst[base + jb] = yb;
+  }
+  __syncthreads();
+}
+
+template <typename T, bool WZ, int EPT>
+__global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_xline(FftArgs a) {
   using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   const RunConst& rc = a.rc;
-  const int N = rc.mgalx, np = rc.nplanes, my = rc.my, mgz = rc.mgalz;
-  const int L = a.L, pairs = L / 2;
-  const int nyb = (my + L - 1) / L;
-  const int z = blockIdx.x / nyb, j0 = (blockIdx.x - z * nyb) * L;
+  const int N = rc.mgalx, my = rc.my;
+  const int pairs = a.L / 2;
+  const int nyb = (my + a.L - 1) / a.L;
+  const int tile = xcd_tile(blockIdx.x, rc.mgalz * nyb);
+  if (tile >= rc.mgalz * nyb) return;
+  const int z = tile / nyb, j0 = (tile - z * nyb) * a.L;
   T2* A = (T2*)smem;
   T2* B = A + pairs * N;
   const int tot = pairs * N;
-  constexpr int NIN = WZ ? 12 : 8;
-  const T2* tw = (const T2*)a.twx;
-
-  // physical values: ph[f][m] = (line a, line b) at entry e = tid + 256 m
-  double ph[NIN][XL_EPT][2];
+  // entry e = threadIdx.x + XL_THREADS*m of the packed rows: pair e / N, x e % N
+  auto grab = [&](const T2* R, T (&v)[EPT][2]) {
 #pragma unroll
-  for (int f = 0; f < NIN; ++f) {
-    for (int it = threadIdx.x; it < tot; it += blockDim.x) { A[it].x = 0; A[it].y = 0; }
-    __syncthreads();
-    const T2* src = (const T2*)a.wbuf[f];
-    for (int it = threadIdx.x; it < np * pairs; it += blockDim.x) {
-      const int ii = it / pairs, p = it - ii * pairs;
-      const int ja = j0 + 2 * p, jb = ja + 1;
-      T2 ca, cb;
-      ca.x = 0; ca.y = 0; cb.x = 0; cb.y = 0;
-      const size_t base = ((size_t)ii * mgz + z) * my;
-      if (ja < my) ca = src[base + ja];
-      if (jb < my) cb = src[base + jb];
-      T2* row = A + p * N;
-      if (ii == 0) {
-        // the imaginary part of the kx=0 coefficient is ignored (rft, rftsingle.f:87-91)
-        T2 v; v.x = ca.x; v.y = cb.x;
-        row[0] = v;
-      } else {
-        T2 v, w;
-        v.x = ca.x - cb.y; v.y = ca.y + cb.x;   // X_a + i X_b
-        w.x = ca.x + cb.y; w.y = -ca.y + cb.x;  // conj(X_a) + i conj(X_b)
-        row[ii] = v;
-        row[N - ii] = w;
-      }
+    for (int m = 0; m < EPT; ++m) {
+      const int e = threadIdx.x + XL_THREADS * m;
+      T2 q = T2{0, 0};
+      if (e < tot) q = R[e];
+      v[m][0] = q.x;
+      v[m][1] = q.y;
     }
     __syncthreads();
-    T2* R = fft_rows<T, T2>(A, B, N, pairs, a.plx, tw, +1);
+  };
+  auto put = [&](const T (&v)[EPT][2]) {
 #pragma unroll
-    for (int m = 0; m < XL_EPT; ++m) {
-      const int e = threadIdx.x + 256 * m;
-      if (e < tot) {
-        ph[f][m][0] = (double)R[e].x;
-        ph[f][m][1] = (double)R[e].y;
-      } else {
-        ph[f][m][0] = ph[f][m][1] = 0.0;
-      }
+    for (int m = 0; m < EPT; ++m) {
+      const int e = threadIdx.x + XL_THREADS * m;
+      if (e < tot) A[e] = T2{v[m][0], v[m][1]};
     }
-    __syncthreads();
-  }
+  };
+  // inputs come from wbuf, outputs go to wout: pass 2 still reads inputs
+  // after pass 1 has written its outputs
+  T2* const* wb = (T2* const*)a.wbuf;
+  T2* const* wo = (T2* const*)a.wout;
 
-  // pointwise physics (cross.f:1613-1673); rhst history in FP64
-  const double dt = a.sc->dt;
-  const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
-  const double xis[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
-  const double p0 = 5.0 / 66.0, p1 = -1.0;
-  const double b3 = (p1 * (gam[0] * gam[2] + gam[1] * gam[2] + xis[1] * gam[2] + gam[0] * xis[2]) -
-                     gam[0] * gam[1] * gam[2] - gam[0] * gam[1] * xis[2]) /
-                    (p1 - gam[0] * gam[1]);
-  const double betas[3] = {gam[0] - p0, p1, b3};
-  const double alphas[3] = {p0, gam[1] + xis[1] - p1, gam[2] + xis[2] - b3};
-  const int rk = a.rkstep - 1;
-  const double dtxi = dt * xis[rk], dtgamma = dt * gam[rk], dtbeta = dt * betas[rk];
-  const double ab = alphas[rk] / betas[rk];
+  // ---- pass 1: T, rho u_i -> u_i, rho u_i u_j; CFL maxima ----
+  T tt[EPT][2], ru[EPT][2], rv[EPT][2], rw[EPT][2];
+  grab(xl_inverse<T, T2, EPT>(a, wb[7], A, B, z, j0, pairs), tt);
+  grab(xl_inverse<T, T2, EPT>(a, wb[0], A, B, z, j0, pairs), ru);
+  grab(xl_inverse<T, T2, EPT>(a, wb[1], A, B, z, j0, pairs), rv);
+  grab(xl_inverse<T, T2, EPT>(a, wb[2], A, B, z, j0, pairs), rw);
   const double ipe = 1.0 / rc.peclet;
-  double dT[XL_EPT][2], drh[XL_EPT][2], dZ[XL_EPT][2];
-  double cx = 0.0, cy = 0.0, cz = 0.0, cd = 0.0;
+  if (a.do_cfl) {
+    double cx = 0.0, cy = 0.0, cz = 0.0, cd = 0.0;
+    const double hx = rc.hxalp * 0.5, hz = rc.hzbet * 0.5;
 #pragma unroll
-  for (int m = 0; m < XL_EPT; ++m) {
-    const int e = threadIdx.x + 256 * m;
-    const int p = e / N, n = e - p * N;
+    for (int m = 0; m < EPT; ++m) {
+      const int e = threadIdx.x + XL_THREADS * m;
+      const int p = e / N;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int j = j0 + 2 * p + h;
-      dT[m][h] = drh[m][h] = dZ[m][h] = 0.0;
-      if (e >= tot || j >= my) continue;
-      const double ru = ph[0][m][h], rv = ph[1][m][h], rw = ph[2][m][h];
-      const double lapT = ph[3][m][h], Tx = ph[4][m][h], Ty = ph[5][m][h];
-      const double Tz = ph[6][m][h], Tt = ph[7][m][h];
-      const double u = ru * Tt, v = rv * Tt, w = rw * Tt;
-      const size_t ri = ((size_t)z * my + j) * N + n;
-      const double rold = a.rhst[ri];
-      double tnext = Tt + dtxi * rold;
-      double d = dtxi * rold;
-      const double r = -u * Tx - v * Ty - w * Tz + Tt * lapT * ipe;
-      a.rhst[ri] = r;
-      tnext = tnext + dtgamma * r;
-      d = d + dtgamma * r;
-      dT[m][h] = d;
-      drh[m][h] = (d / (Tt * tnext)) / dtbeta - ab * r / (Tt * Tt);
-      if constexpr (WZ) {
-        const double zx = ph[8][m][h], zy = ph[9][m][h], zz = ph[10][m][h], lz = ph[11][m][h];
-        const double zold = a.rhsz[ri];
-        const double rz = -u * zx - v * zy - w * zz + Tt * lz * ipe;
-        a.rhsz[ri] = rz;
-        dZ[m][h] = dtxi * zold + dtgamma * rz;
-      }
-      if (a.do_cfl) {
+      for (int h = 0; h < 2; ++h) {
+        const int j = j0 + 2 * p + h;
+        if (e >= tot || j >= my) continue;
+        const double Tt = tt[m][h];
         const double ihy = 1.0 / a.hy[j];
-        cx = fmax(cx, fabs(u));
-        cy = fmax(cy, fabs(v) * ihy);
-        cz = fmax(cz, fabs(w));
-        const double hx = rc.hxalp * 0.5, hz = rc.hzbet * 0.5;
+        cx = fmax(cx, fabs((double)ru[m][h] * Tt));
+        cy = fmax(cy, fabs((double)rv[m][h] * Tt) * ihy);  // cross.f:1705-1707
+        cz = fmax(cz, fabs((double)rw[m][h] * Tt));
         cd = fmax(cd, 2.0 * (ipe * Tt) * (hx * hx + hz * hz + ihy * ihy));
       }
     }
-  }
-  if (a.do_cfl) {
-    // cfly uses max|v| / hy per line, cross.f:1705-1707
     for (int d = 32; d >= 1; d >>= 1) {
       cx = fmax(cx, __shfl_xor(cx, d));
       cy = fmax(cy, __shfl_xor(cy, d));
@@ -387,63 +478,123 @@ __global__ __launch_bounds__(256) void k_xline(FftArgs a) {
       atomic_max_nonneg(&a.sc->cfld_bits, cd);
     }
   }
-
-  // outputs: 0 u 1 v 2 w 3 ruu 4 rvv 5 rww 6 ruv 7 ruw 8 rvw 9 drho 10 DT 11 DZ
-  constexpr int nout = WZ ? 12 : 11;
-  const T sc = (T)(1.0 / (double)N);
-  for (int q = 0; q < nout; ++q) {
+  // outputs 0 u 1 v 2 w 3 ruu 4 rvv 5 rww 6 ruv 7 ruw 8 rvw  (cross.f:1613-1651)
+#pragma unroll 1
+  for (int q = 0; q < 9; ++q) {
 #pragma unroll
-    for (int m = 0; m < XL_EPT; ++m) {
-      const int e = threadIdx.x + 256 * m;
+    for (int m = 0; m < EPT; ++m) {
+      const int e = threadIdx.x + XL_THREADS * m;
       if (e >= tot) continue;
-      double val[2];
+      T val[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const double ru = ph[0][m][h], rv = ph[1][m][h], rw = ph[2][m][h], Tt = ph[7][m][h];
-        double x;
+        const T Tt = tt[m][h], a_ = ru[m][h], b_ = rv[m][h], c_ = rw[m][h];
+        T x;
         switch (q) {
-          case 0: x = ru * Tt; break;
-          case 1: x = rv * Tt; break;
-          case 2: x = rw * Tt; break;
-          case 3: x = ru * (ru * Tt); break;
-          case 4: x = rv * (rv * Tt); break;
-          case 5: x = rw * (rw * Tt); break;
-          case 6: x = ru * (rv * Tt); break;
-          case 7: x = ru * (rw * Tt); break;
-          case 8: x = rv * (rw * Tt); break;
-          case 9: x = drh[m][h]; break;
-          case 10: x = dT[m][h]; break;
-          default: x = dZ[m][h]; break;
+          case 0: x = a_ * Tt; break;
+          case 1: x = b_ * Tt; break;
+          case 2: x = c_ * Tt; break;
+          case 3: x = a_ * (a_ * Tt); break;
+          case 4: x = b_ * (b_ * Tt); break;
+          case 5: x = c_ * (c_ * Tt); break;
+          case 6: x = a_ * (b_ * Tt); break;
+          case 7: x = a_ * (c_ * Tt); break;
+          default: x = b_ * (c_ * Tt); break;
         }
         val[h] = x;
       }
-      T2 v;
-      v.x = (T)val[0];
-      v.y = (T)val[1];
-      A[e] = v;
+      A[e] = T2{val[0], val[1]};
     }
-    __syncthreads();
-    T2* R = fft_rows<T, T2>(A, B, N, pairs, a.plx, tw, -1);
-    T2* dst = (T2*)a.wbuf[q];
-    for (int it = threadIdx.x; it < np * pairs; it += blockDim.x) {
-      const int ii = it / pairs, p = it - ii * pairs;
-      const int ja = j0 + 2 * p, jb = ja + 1;
-      const T2* row = R + p * N;
-      T2 zk = row[ii];
-      T2 zc = row[(N - ii) % N];
-      zc.y = -zc.y;
-      T2 ya, yb;
-      ya.x = (zk.x + zc.x) * (T)0.5 * sc;
-      ya.y = (zk.y + zc.y) * (T)0.5 * sc;
-      // (zk - zc) / (2i)
-      yb.x = (zk.y - zc.y) * (T)0.5 * sc;
-      yb.y = -(zk.x - zc.x) * (T)0.5 * sc;
-      if (ii == 0) { ya.y = 0; yb.y = 0; }
-      const size_t base = ((size_t)ii * mgz + z) * my;
-      if (ja < my) dst[base + ja] = ya;
-      if (jb < my) dst[base + jb] = yb;
+    xl_forward<T, T2, EPT>(a, wo[q], A, B, z, j0, pairs);
+  }
+
+  // ---- pass 2: r = T (lap T / Pe - rho u . grad T)  (cross.f:1652-1673) ----
+  T acc[EPT][2], accz[EPT][2];
+  {
+    T f[EPT][2];
+    grab(xl_inverse<T, T2, EPT>(a, wb[3], A, B, z, j0, pairs), f);  // lap T
+#pragma unroll
+    for (int m = 0; m < EPT; ++m)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) acc[m][h] = f[m][h] * (T)ipe;
+    if constexpr (WZ) {
+      grab(xl_inverse<T, T2, EPT>(a, wb[11], A, B, z, j0, pairs), f);  // lap Z
+#pragma unroll
+      for (int m = 0; m < EPT; ++m)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) accz[m][h] = f[m][h] * (T)ipe;
     }
-    __syncthreads();
+    // gradient terms: T_x (4), T_y (5), T_z (6); Z_x (8), Z_y (9), Z_z (10)
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+      grab(xl_inverse<T, T2, EPT>(a, wb[4 + c], A, B, z, j0, pairs), f);
+#pragma unroll
+      for (int m = 0; m < EPT; ++m)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const T rc_ = c == 0 ? ru[m][h] : (c == 1 ? rv[m][h] : rw[m][h]);
+          acc[m][h] -= rc_ * f[m][h];
+        }
+      if constexpr (WZ) {
+        grab(xl_inverse<T, T2, EPT>(a, wb[8 + c], A, B, z, j0, pairs), f);
+#pragma unroll
+        for (int m = 0; m < EPT; ++m)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const T rc_ = c == 0 ? ru[m][h] : (c == 1 ? rv[m][h] : rw[m][h]);
+            accz[m][h] -= rc_ * f[m][h];
+          }
+      }
+    }
+  }
+  // time advance of T in physical space, rhst history in FP64
+  const double dt = a.sc->dt;
+  const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+  const double xis[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+  const double p0 = 5.0 / 66.0, p1 = -1.0;
+  const double b3 = (p1 * (gam[0] * gam[2] + gam[1] * gam[2] + xis[1] * gam[2] + gam[0] * xis[2]) -
+                     gam[0] * gam[1] * gam[2] - gam[0] * gam[1] * xis[2]) /
+                    (p1 - gam[0] * gam[1]);
+  const double betas[3] = {gam[0] - p0, p1, b3};
+  const double alphas[3] = {p0, gam[1] + xis[1] - p1, gam[2] + xis[2] - b3};
+  const int rk = a.rkstep - 1;
+  const double dtxi = dt * xis[rk], dtgamma = dt * gam[rk], dtbeta = dt * betas[rk];
+  const double ab = alphas[rk] / betas[rk];
+  // ru <- DT, rv <- drho, rw <- DZ (rho u_i are no longer needed)
+#pragma unroll
+  for (int m = 0; m < EPT; ++m) {
+    const int e = threadIdx.x + XL_THREADS * m;
+    const int p = e / N, n = e - p * N;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int j = j0 + 2 * p + h;
+      ru[m][h] = rv[m][h] = rw[m][h] = (T)0;
+      if (e >= tot || j >= my) continue;
+      const double Tt = tt[m][h];
+      const double r = Tt * (double)acc[m][h];
+      const size_t ri = ((size_t)z * my + j) * N + n;
+      const double rold = a.rhst[ri];
+      a.rhst[ri] = r;
+      const double d = dtxi * rold + dtgamma * r;
+      const double tnext = Tt + d;
+      ru[m][h] = (T)d;
+      rv[m][h] = (T)((d / (Tt * tnext)) / dtbeta - ab * r / (Tt * Tt));
+      if constexpr (WZ) {
+        const double rz = Tt * (double)accz[m][h];
+        const double zold = a.rhsz[ri];
+        a.rhsz[ri] = rz;
+        rw[m][h] = (T)(dtxi * zold + dtgamma * rz);
+      }
+    }
+  }
+  // outputs 9 drho, 10 DT, 11 DZ
+  put(rv);
+  xl_forward<T, T2, EPT>(a, wo[9], A, B, z, j0, pairs);
+  put(ru);
+  xl_forward<T, T2, EPT>(a, wo[10], A, B, z, j0, pairs);
+  if constexpr (WZ) {
+    put(rw);
+    xl_forward<T, T2, EPT>(a, wo[11], A, B, z, j0, pairs);
   }
 }
 
@@ -472,7 +623,7 @@ __global__ __launch_bounds__(256) void k_fourz(FftPlan pl, const double2* tw, do
 // launchers
 // ------------------------------------------------------------------------
 static size_t zfft_lds(bool fp32, const FftArgs& a) {
-  return (size_t)2 * a.J * a.rc.mgalz * (fp32 ? 8 : 16);
+  return (size_t)a.J * a.rc.mgalz * (fp32 ? 8 : 16);
 }
 static size_t xline_lds(bool fp32, const FftArgs& a) {
   return (size_t)2 * (a.L / 2) * a.rc.mgalx * (fp32 ? 8 : 16);
@@ -482,10 +633,11 @@ int launch_zfft_inv(bool fp32, const FftArgs& a, hipStream_t st) {
   const int nyb = (a.rc.my + a.J - 1) / a.J;
   dim3 g(a.rc.nplanes * nyb, a.nfields), b(256);
   const size_t lds = zfft_lds(fp32, a);
+  // J * mgalz <= 256 * E: E = 16 in FP32, 8 in FP64 (checked at init)
   if (fp32)
-    hipLaunchKernelGGL(k_zfft_inv<float>, g, b, lds, st, a);
+    hipLaunchKernelGGL((k_zfft_inv<float, 16>), g, b, lds, st, a);
   else
-    hipLaunchKernelGGL(k_zfft_inv<double>, g, b, lds, st, a);
+    hipLaunchKernelGGL((k_zfft_inv<double, 8>), g, b, lds, st, a);
   return (int)hipGetLastError();
 }
 
@@ -494,22 +646,23 @@ int launch_zfft_fwd(bool fp32, const FftArgs& a, hipStream_t st) {
   dim3 g(a.rc.nplanes * nyb, a.nfields), b(256);
   const size_t lds = zfft_lds(fp32, a);
   if (fp32)
-    hipLaunchKernelGGL(k_zfft_fwd<float>, g, b, lds, st, a);
+    hipLaunchKernelGGL((k_zfft_fwd<float, 16>), g, b, lds, st, a);
   else
-    hipLaunchKernelGGL(k_zfft_fwd<double>, g, b, lds, st, a);
+    hipLaunchKernelGGL((k_zfft_fwd<double, 8>), g, b, lds, st, a);
   return (int)hipGetLastError();
 }
 
 int launch_xline(bool fp32, const FftArgs& a, hipStream_t st) {
   const int nyb = (a.rc.my + a.L - 1) / a.L;
-  dim3 g(a.rc.mgalz * nyb), b(256);
+  dim3 g(xcd_groups(a.rc.mgalz * nyb)), b(XL_THREADS);
   const size_t lds = xline_lds(fp32, a);
+  // entries per thread: pairs * mgalx <= XL_THREADS * EPT (checked at init)
   if (fp32) {
-    if (a.with_z) hipLaunchKernelGGL((k_xline<float, true>), g, b, lds, st, a);
-    else hipLaunchKernelGGL((k_xline<float, false>), g, b, lds, st, a);
+    if (a.with_z) hipLaunchKernelGGL((k_xline<float, true, 8>), g, b, lds, st, a);
+    else hipLaunchKernelGGL((k_xline<float, false, 8>), g, b, lds, st, a);
   } else {
-    if (a.with_z) hipLaunchKernelGGL((k_xline<double, true>), g, b, lds, st, a);
-    else hipLaunchKernelGGL((k_xline<double, false>), g, b, lds, st, a);
+    if (a.with_z) hipLaunchKernelGGL((k_xline<double, true, 4>), g, b, lds, st, a);
+    else hipLaunchKernelGGL((k_xline<double, false, 4>), g, b, lds, st, a);
   }
   return (int)hipGetLastError();
 }

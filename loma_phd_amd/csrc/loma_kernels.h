@@ -47,8 +47,9 @@ FftPlan make_plan(int n);
 struct PhaseArgs {
   RunConst rc;
   YTab T;
-  const double* helm_lu;     // per (i,|kz|) class LU tables
-  int lu_rows;               // rows per class table (incl. padding)
+  const double* helm_lu;     // per (i,|kz|) class LU tables (lane-major)
+  const double* helm_bc;     // per class Neumann constants (4 doubles)
+  int lu_stride;             // doubles per class LU table (5 * 64 * M)
   double2 *vor, *phi, *psi, *scal, *ten12, *ten13, *zf;
   void* spec[12];            // S-sized transform buffers (float2 or double2)
   double *u00wk, *v00wk, *w00wk, *rf0u, *rf0w;
@@ -65,7 +66,8 @@ struct FftArgs {
   const void* twz;           // twiddles exp(-2 pi i t/mgalz), T2
   const void* twx;
   void* spec[12];
-  void* wbuf[12];
+  void* wbuf[12];            // z-physical transform inputs  [i][z][j]
+  void* wout[12];            // z-physical transform outputs [i][z][j]
   double* rhst;
   double* rhsz;
   const double* hy;          // my
@@ -81,15 +83,18 @@ struct FftArgs {
 // launchers (return hipError_t as int)
 int launch_dy(int M, int order, const YTab& T, const double* u, double* du, int ncols,
               hipStream_t st);
-int launch_helm_factor(const double* dt21, const double* dt22, int my, int rows_pad,
+int launch_helm_factor(const double* dt21, const double* dt22, int my, int M,
                        const double* shifts, int nsys, double* lu, hipStream_t st);
-int launch_lapvdvhom(int M, const YTab& T, const double* lu, int lu_rows,
+int launch_helm_bc(int M, const YTab& T, const double* lu, double* bc, int nclass,
+                   hipStream_t st);
+int launch_lapvdvhom(int M, const YTab& T, const double* lu, const double* bc,
                      const double* phi, double* v, double* dv, const double* rK,
                      int ncols, int mode, hipStream_t st);
 int launch_inty8(int M, const YTab& T, const double* du, double* u, hipStream_t st);
 
 int launch_phaseA(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
 int launch_phaseC(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
+int launch_phaseC2(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
 int launch_finish_step(DevScalars* sc, RunConst rc, int icfl, hipStream_t st);
 
 int launch_zfft_inv(bool fp32, const FftArgs& a, hipStream_t st);

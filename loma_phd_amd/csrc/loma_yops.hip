@@ -10,8 +10,10 @@ namespace loma {
 template <int M>
 int launch_dy_m(int, const YTab&, const double*, double*, int, hipStream_t);
 template <int M>
-int launch_lapvdvhom_m(const YTab&, const double*, int, const double*, double*, double*,
-                       const double*, int, int, hipStream_t);
+int launch_lapvdvhom_m(const YTab&, const double*, int, const double*, const double*,
+                       double*, double*, const double*, int, int, hipStream_t);
+template <int M>
+int launch_helm_bc_m(const YTab&, const double*, int, double*, int, hipStream_t);
 template <int M>
 int launch_inty8_m(const YTab&, const double*, double*, hipStream_t);
 template <int M>
@@ -53,11 +55,17 @@ int launch_dy(int M, int order, const YTab& T, const double* u, double* du, int 
   if (ncols <= 0) return 0;
   LOMA_DISPATCH_M(M, launch_dy_m<MM>(order, T, u, du, ncols, st));
 }
-int launch_lapvdvhom(int M, const YTab& T, const double* lu, int lu_rows, const double* phi,
-                     double* v, double* dv, const double* rK, int ncols, int mode,
-                     hipStream_t st) {
+int launch_lapvdvhom(int M, const YTab& T, const double* lu, const double* bc,
+                     const double* phi, double* v, double* dv, const double* rK, int ncols,
+                     int mode, hipStream_t st) {
   if (ncols <= 0) return 0;
-  LOMA_DISPATCH_M(M, launch_lapvdvhom_m<MM>(T, lu, lu_rows, phi, v, dv, rK, ncols, mode, st));
+  LOMA_DISPATCH_M(M, launch_lapvdvhom_m<MM>(T, lu, 5 * WAVE * MM, bc, phi, v, dv, rK, ncols,
+                                            mode, st));
+}
+int launch_helm_bc(int M, const YTab& T, const double* lu, double* bc, int nclass,
+                   hipStream_t st) {
+  if (nclass <= 0) return 0;
+  LOMA_DISPATCH_M(M, launch_helm_bc_m<MM>(T, lu, 5 * WAVE * MM, bc, nclass, st));
 }
 int launch_inty8(int M, const YTab& T, const double* du, double* u, hipStream_t st) {
   LOMA_DISPATCH_M(M, launch_inty8_m<MM>(T, du, u, st));
@@ -70,18 +78,29 @@ int launch_phaseC(int M, bool fp32, const PhaseArgs& a, hipStream_t st) {
   if (a.nmodes <= 0) return 0;
   LOMA_DISPATCH_M(M, launch_phase_m<MM>(1, fp32, a, st));
 }
+int launch_phaseC2(int M, bool fp32, const PhaseArgs& a, hipStream_t st) {
+  if (a.nmodes <= 0) return 0;
+  LOMA_DISPATCH_M(M, launch_phase_m<MM>(2, fp32, a, st));
+}
 
 // LU of (dt22 - s dt21) with identity end rows (Lapvdvhom laps.f:349-366),
-// one thread per shift s.  Output rows: inv, up1, up2, lo1, lo2.
+// one thread per shift s.  Output per class: a lane-major table of 64*M rows
+// with entries inv, up1, up2, lo1, lo2 (lower multipliers by destination
+// row), zero beyond row my-1.
 __global__ void k_helm_factor(const double* __restrict__ dt21,
-                              const double* __restrict__ dt22, int my, int rows_pad,
+                              const double* __restrict__ dt22, int my, int M,
                               const double* __restrict__ shifts, int nsys,
                               double* __restrict__ lu) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsys) return;
   const double sh = shifts[s];
-  double* out = lu + (size_t)s * rows_pad * 5;
-  for (int q = 0; q < rows_pad * 5; ++q) out[q] = 0.0;
+  const int stride = 5 * WAVE * M;
+  double* out = lu + (size_t)s * stride;
+  for (int q = 0; q < stride; ++q) out[q] = 0.0;
+  auto at = [&](int row, int idx) -> double& {
+    const int l = row / M, t = row - l * M;
+    return out[(t * 5 + idx) * WAVE + l];
+  };
   auto row = [&](int j, double* r) {
     if (j >= my) { for (int d = 0; d < 5; ++d) r[d] = 0.0; return; }
     if (j == 0 || j == my - 1) {
@@ -94,15 +113,14 @@ __global__ void k_helm_factor(const double* __restrict__ dt21,
   double w[3][5];
   row(0, w[0]); row(1, w[1]); row(2, w[2]);
   for (int k = 0; k < my; ++k) {
-    double* o = out + (size_t)(k + PADR) * 5;
     const double iv = 1.0 / w[0][2];
-    o[0] = iv;
-    o[1] = (k + 1 < my) ? w[0][3] : 0.0;
-    o[2] = (k + 2 < my) ? w[0][4] : 0.0;
+    at(k, 0) = iv;
+    at(k, 1) = (k + 1 < my) ? w[0][3] : 0.0;
+    at(k, 2) = (k + 2 < my) ? w[0][4] : 0.0;
     for (int i = 1; i <= 2; ++i) {
-      if (k + i >= my) { o[2 + i] = 0.0; continue; }
+      if (k + i >= my) continue;
       const double l = w[i][2 - i] * iv;
-      o[2 + i] = l;
+      at(k + i, 2 + i) = l;  // multiplier of row k into row k+i
       w[i][2 - i] = 0.0;
       for (int d = 1; d <= 2; ++d) w[i][2 - i + d] -= l * w[0][2 + d];
     }
@@ -111,11 +129,11 @@ __global__ void k_helm_factor(const double* __restrict__ dt21,
   }
 }
 
-int launch_helm_factor(const double* dt21, const double* dt22, int my, int rows_pad,
+int launch_helm_factor(const double* dt21, const double* dt22, int my, int M,
                        const double* shifts, int nsys, double* lu, hipStream_t st) {
   if (nsys <= 0) return 0;
   hipLaunchKernelGGL(k_helm_factor, dim3((nsys + 63) / 64), dim3(64), 0, st, dt21, dt22,
-                     my, rows_pad, shifts, nsys, lu);
+                     my, M, shifts, nsys, lu);
   return (int)hipGetLastError();
 }
 
