@@ -147,11 +147,14 @@ int loma_solvepsi(loma_run *run, const float *phi, float *v, float *dvdy,
 int loma_fourz_dev(loma_run *run, double *c, int nrows, int iopt);
 
 /* ---- batched banded solves in isolation ------------------------------- */
-/* nsys independent pentadiagonal systems of size n (row-major band
- * [sys][n][5], entry d is column j-2+d) with complex right-hand sides
- * rhs[sys][n] (interleaved doubles, float when rhs_fp32).  Factor (bandec,
- * cfdiff.f:977) and solve (banbks, cfdiff.f:954) in FP64; x overwrites rhs
- * (double) or goes to x (double) when rhs_fp32.  Device pointers. */
+/* nsys independent pentadiagonal systems of size n >= 3 (row-major band
+ * [sys][n][5], entry d of row j is column j-2+d) with complex right-hand
+ * sides rhs[sys][n] (interleaved; float when rhs_fp32, else double).  LU
+ * without pivoting (bandec, cfdiff.f:977) and the two triangular sweeps
+ * (banbks, cfdiff.f:954), FP64.  The solution goes to x (double,
+ * interleaved); x may be NULL for a double rhs, which is then overwritten.
+ * Device pointers; stream may be NULL.  Scratch of n*5*nsys doubles is kept
+ * per device and grown on demand. */
 int loma_band5_solve_dev(const double *band, void *rhs, double *x, int n,
                          int nsys, int rhs_fp32, void *stream);
 

@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(PKG, "_lib", "libloma_hip.so")
+LIB = os.environ.get("LOMA_HIP_LIB") or os.path.join(PKG, "_lib", "libloma_hip.so")
 
 LOMA_OK, LOMA_ERR_ARG, LOMA_ERR_NAN, LOMA_ERR_OOM = 0, 1, 2, 3
 LOMA_ERR_DEVICE, LOMA_ERR_STATE, LOMA_ERR_NODEV = 4, 5, 6
@@ -117,6 +117,14 @@ def fd_matrices(y):
            np.zeros(my)]
     check(lib().loma_fd_matrices(my, _d(y), *[_d(a) for a in out]))
     return tuple(out)
+
+
+def band5_solve_dev(band_ptr, rhs_ptr, x_ptr, n, nsys, rhs_fp32=False, stream=None):
+    """Batched pentadiagonal solves on device pointers (bandec/banbks,
+    cfdiff.f:954-1022): band [nsys][n][5] float64, rhs [nsys][n] complex
+    (complex64 when rhs_fp32), x [nsys][n] complex128 or None for in place."""
+    check(lib().loma_band5_solve_dev(band_ptr, rhs_ptr, x_ptr, n, nsys, 1 if rhs_fp32 else 0,
+                                     stream))
 
 
 def device_count() -> int:

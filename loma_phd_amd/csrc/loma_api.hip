@@ -46,6 +46,7 @@ struct loma_run {
   int mgalx = 0, my = 0, mgalz = 0, mx = 0, mz = 0, nplanes = 0, nz = 0;
   int M = 0;            // rows per lane of the y-column solver
   int lu_stride = 0;    // doubles per Helmholtz class table (5 * 64 * M)
+  int scan_levels[6] = {6, 6, 6, 6, 6, 6};  // L1 U1 L2 U2 Li Ui
   int precision = 0, flags = 0;
   bool fp32 = false, with_z = false;
   double Re = 0, alp = 0, bet = 0, pr = 0.7, cfl = 0.5, dt0 = 1e-4;
@@ -71,6 +72,7 @@ struct loma_run {
   double *rhst = nullptr, *rhsz = nullptr, *hy = nullptr;
   double *u00wk = nullptr, *v00wk = nullptr, *w00wk = nullptr, *rf0u = nullptr,
          *rf0w = nullptr;
+  double *mean_in = nullptr, *mean_out = nullptr, *rbar = nullptr, *rzbar = nullptr;
   DevScalars* sc = nullptr;
   FftPlan plz{}, plx{};
   void *twz = nullptr, *twx = nullptr;
@@ -239,6 +241,10 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   const int nb[NT] = {7, 7, 1, 3, 3, 1, 5, 5, 1, 2, 2, 1, 3, 3, 2};
   size_t off[NT], tot = 0;
   for (int q = 0; q < NT; ++q) { off[q] = tot; tot += (size_t)nb[q] * R; }
+  // scan tables of the fixed factors: L1, U1 (P=3), L2, U2 (P=2), Li, Ui (P=3)
+  const int hp[6] = {3, 3, 2, 2, 3, 3};
+  size_t hoff[6];
+  for (int q = 0; q < 6; ++q) { hoff[q] = tot; tot += (size_t)6 * hp[q] * hp[q] * 64; }
   std::vector<double> h(tot, 0.0);
   auto lm = [&](int row, int w, int idx) -> size_t {
     const int l = row / M, t = row - l * M;
@@ -259,6 +265,13 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   put(5, f.fmap, 1); put(6, f.b2, 5); put(7, f.a2, 5); put(8, f.inv2, 1); put(9, f.up2, 2);
   put_lo(10, f.lo2, 2); put(11, f.invi, 1); put(12, f.upi, 3); put_lo(13, f.loi, 3);
   put(14, wbd, 2);
+  int lev[6];
+  lev[0] = scan_table(my, M, 3, false, f.lo1.data(), nullptr, nullptr, h.data() + hoff[0]);
+  lev[1] = scan_table(my, M, 3, true, nullptr, f.up1.data(), f.inv1.data(), h.data() + hoff[1]);
+  lev[2] = scan_table(my, M, 2, false, f.lo2.data(), nullptr, nullptr, h.data() + hoff[2]);
+  lev[3] = scan_table(my, M, 2, true, nullptr, f.up2.data(), f.inv2.data(), h.data() + hoff[3]);
+  lev[4] = scan_table(my, M, 3, false, f.loi.data(), nullptr, nullptr, h.data() + hoff[4]);
+  lev[5] = scan_table(my, M, 3, true, nullptr, f.upi.data(), f.invi.data(), h.data() + hoff[5]);
   if ((rc = dalloc(r, &r->tab, tot * sizeof(double)))) { loma_free(r); return rc; }
   if (hipMemcpy(r->tab, h.data(), tot * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
     set_error("loma_init: table upload failed"); loma_free(r); return LOMA_ERR_DEVICE;
@@ -269,6 +282,11 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   T.b2 = r->tab + off[6]; T.a2 = r->tab + off[7]; T.inv2 = r->tab + off[8];
   T.up2 = r->tab + off[9]; T.lo2 = r->tab + off[10]; T.invi = r->tab + off[11];
   T.upi = r->tab + off[12]; T.loi = r->tab + off[13]; T.wbd = r->tab + off[14];
+  T.hL1 = r->tab + hoff[0]; T.hU1 = r->tab + hoff[1]; T.hL2 = r->tab + hoff[2];
+  T.hU2 = r->tab + hoff[3]; T.hLi = r->tab + hoff[4]; T.hUi = r->tab + hoff[5];
+  T.kL1 = lev[0]; T.kU1 = lev[1]; T.kL2 = lev[2]; T.kU2 = lev[3]; T.kLi = lev[4];
+  T.kUi = lev[5];
+  std::memcpy(r->scan_levels, lev, sizeof(lev));
   T.my = my;
   // plain band copies for the Helmholtz factor kernel
   if ((rc = dalloc(r, &r->dt21, sizeof(double) * 5 * my))) { loma_free(r); return rc; }
@@ -320,6 +338,16 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   double** v00[6] = {&r->u00wk, &r->v00wk, &r->w00wk, &r->rf0u, &r->rf0w, &r->hy};
   for (auto p : v00)
     if ((rc = dalloc(r, p, sizeof(double) * my))) { loma_free(r); return rc; }
+  {
+    double** mb[4] = {&r->mean_in, &r->mean_out, &r->rbar, &r->rzbar};
+    const size_t nmb[4] = {(size_t)12 * my, (size_t)12 * my, (size_t)2 * my, (size_t)2 * my};
+    for (int q = 0; q < 4; ++q) {
+      if ((rc = dalloc(r, mb[q], sizeof(double) * nmb[q]))) { loma_free(r); return rc; }
+      if (hipMemset(*mb[q], 0, sizeof(double) * nmb[q]) != hipSuccess) {
+        set_error("loma_init: memset failed"); loma_free(r); return LOMA_ERR_DEVICE;
+      }
+    }
+  }
   {
     std::vector<double> hyv(my);
     for (int j = 1; j < my - 1; ++j) hyv[j] = (y[j + 1] - y[j - 1]) / 2.0;  // main.f:689-696
@@ -376,9 +404,9 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
     // at most 256*EPT (EPT = 4 in FP64, 8 in FP32): two LDS buffers of
     // 32 KiB together
     const int cap = r->fp32 ? 2048 : 1024;
-    int pairs = cap / mgalx;
-    if (pairs > 16) pairs = 16;
-    if (pairs < 1) {
+    int pairs = 16;  // a power of two: each thread serves one line pair
+    while (pairs > 1 && pairs * mgalx > cap) pairs /= 2;
+    if (pairs * mgalx > cap) {
       set_error("loma_init: mgalx too large for the x-line kernel");
       loma_free(r);
       return LOMA_ERR_ARG;
@@ -455,6 +483,8 @@ int loma_upload(loma_run* r, const void* vor, const void* phi, const void* psi,
   HIPCHK(hipMemcpy(r->w00wk, w00, sizeof(double) * r->my, hipMemcpyHostToDevice));
   HIPCHK(hipMemset(r->rf0u, 0, sizeof(double) * r->my));
   HIPCHK(hipMemset(r->rf0w, 0, sizeof(double) * r->my));
+  HIPCHK(hipMemset(r->rbar, 0, sizeof(double) * 2 * r->my));
+  HIPCHK(hipMemset(r->rzbar, 0, sizeof(double) * 2 * r->my));
   // boundary densities from the 00 mode of T (cross.f:208-213)
   if (!(T0 > 0.0) || !(T1 > 0.0)) {
     set_error("loma_upload: 00 mode of T must be positive at both ends");
@@ -527,6 +557,7 @@ static PhaseArgs phase_args(loma_run* r, int rkstep) {
   for (int q = 0; q < 12; ++q) a.spec[q] = r->spec[q];
   a.u00wk = r->u00wk; a.v00wk = r->v00wk; a.w00wk = r->w00wk;
   a.rf0u = r->rf0u; a.rf0w = r->rf0w;
+  a.mean_in = r->mean_in; a.mean_out = r->mean_out;
   a.sc = r->sc;
   a.rkstep = rkstep;
   a.with_z = r->with_z ? 1 : 0;
@@ -549,6 +580,10 @@ static FftArgs fft_args(loma_run* r, int rkstep, int do_cfl) {
   }
   a.rhst = r->rhst;
   a.rhsz = r->rhsz;
+  a.mean_in = r->mean_in;
+  a.mean_out = r->mean_out;
+  a.rbar = r->rbar;
+  a.rzbar = r->rzbar;
   a.hy = r->hy;
   a.sc = r->sc;
   a.rkstep = rkstep;
@@ -841,11 +876,42 @@ int loma_fourz_dev(loma_run* r, double* c, int nrows, int iopt) {
   return LOMA_OK;
 }
 
+// scratch of the batched banded solve, kept per device and grown on demand
+static std::mutex g_band_mu;
+static std::vector<std::pair<void*, size_t>> g_band_scr;
+
 int loma_band5_solve_dev(const double* band, void* rhs, double* x, int n, int nsys,
                          int rhs_fp32, void* stream) {
-  (void)band; (void)rhs; (void)x; (void)n; (void)nsys; (void)rhs_fp32; (void)stream;
-  set_error("loma_band5_solve_dev: not built yet");
-  return LOMA_ERR_STATE;
+  if (!band || !rhs || n < 3 || nsys < 0 || (rhs_fp32 && !x)) {
+    set_error("loma_band5_solve_dev: bad arguments (n >= 3; x required for float rhs)");
+    return LOMA_ERR_ARG;
+  }
+  if (nsys == 0) return LOMA_OK;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  const size_t need = (size_t)n * 5 * nsys * sizeof(double);
+  double* scr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_band_mu);
+    if ((int)g_band_scr.size() <= dev) g_band_scr.resize(dev + 1, {nullptr, 0});
+    auto& e = g_band_scr[dev];
+    if (e.second < need) {
+      if (e.first) hipFree(e.first);
+      e.first = nullptr;
+      e.second = 0;
+      if (hipMalloc(&e.first, need) != hipSuccess) {
+        set_error("loma_band5_solve_dev: scratch allocation failed");
+        return LOMA_ERR_OOM;
+      }
+      e.second = need;
+    }
+    scr = (double*)e.first;
+  }
+  double* out = x ? x : (double*)rhs;
+  hipStream_t st = (hipStream_t)stream;
+  LCHK(launch_band5(band, rhs, out, scr, n, nsys, rhs_fp32, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return LOMA_OK;
 }
 
 }  // extern "C"

@@ -323,6 +323,158 @@ __device__ __forceinline__ void usolve(const TabRef up, const TabRef inv, int la
   }
 }
 
+// ---- triangular solves with precomputed scan matrices -------------------
+// For a fixed factor the carry maps of the lanes do not depend on the data,
+// so the Hillis-Steele compositions are tabulated once per grid:
+// H[k][q][b][lane] (lane-major) is the map from the carry of lane l-2^k
+// (l+2^k for the upper solve) to that of lane l after the first k levels.
+// Only the first `levels` levels are applied: the compositions beyond them
+// are below 1e-22 in max norm (the factors' inverses decay geometrically),
+// checked when the tables are built.
+template <int M, int P, int NC>
+__device__ __forceinline__ void lsolve_tab(const TabRef lo, const TabRef H, int levels,
+                                           int lane, double (&x)[NC][M]) {
+#pragma unroll
+  for (int t = 1; t < M; ++t)
+#pragma unroll
+    for (int i = 1; i <= P; ++i) {
+      if (t - i < 0) continue;
+      const double l = tget(lo, t, i - 1);
+#pragma unroll
+      for (int n = 0; n < NC; ++n) x[n][t] -= l * x[n][t - i];
+    }
+  op_fence();
+  double c[NC][P];
+#pragma unroll
+  for (int n = 0; n < NC; ++n)
+#pragma unroll
+    for (int q = 0; q < P; ++q) c[n][q] = x[n][M - 1 - q];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (k < levels) {
+      const int d = 1 << k;
+      double cp[NC][P];
+#pragma unroll
+      for (int n = 0; n < NC; ++n)
+#pragma unroll
+        for (int q = 0; q < P; ++q) cp[n][q] = __shfl_up(c[n][q], d);
+      if (lane >= d) {
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+#pragma unroll
+          for (int b = 0; b < P; ++b) {
+            const double h = tget(H, k * P + q, b);
+#pragma unroll
+            for (int n = 0; n < NC; ++n) c[n][q] += h * cp[n][b];
+          }
+      }
+    }
+  }
+  op_fence();
+  double s[NC][P];
+#pragma unroll
+  for (int n = 0; n < NC; ++n)
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const double v = __shfl_up(c[n][q], 1);
+      s[n][q] = lane == 0 ? 0.0 : v;
+    }
+  const TabRef lo2 = relaunder(lo);
+  double dl[NC][M];
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+#pragma unroll
+    for (int n = 0; n < NC; ++n) dl[n][t] = 0.0;
+#pragma unroll
+    for (int i = 1; i <= P; ++i) {
+      const double l = tget(lo2, t, i - 1);
+#pragma unroll
+      for (int n = 0; n < NC; ++n) {
+        const double pv = (t - i >= 0) ? dl[n][t - i] : s[n][i - t - 1];
+        dl[n][t] -= l * pv;
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < NC; ++n) x[n][t] += dl[n][t];
+  }
+}
+
+template <int M, int P, int NC>
+__device__ __forceinline__ void usolve_tab(const TabRef up, const TabRef inv, const TabRef H,
+                                           int levels, int lane, double (&x)[NC][M]) {
+#pragma unroll
+  for (int t = M - 1; t >= 0; --t) {
+#pragma unroll
+    for (int d = 1; d <= P; ++d) {
+      if (t + d >= M) continue;
+      const double u = tget(up, t, d - 1);
+#pragma unroll
+      for (int n = 0; n < NC; ++n) x[n][t] -= u * x[n][t + d];
+    }
+    const double iv = tget(inv, t, 0);
+#pragma unroll
+    for (int n = 0; n < NC; ++n) x[n][t] *= iv;
+  }
+  op_fence();
+  double c[NC][P];
+#pragma unroll
+  for (int n = 0; n < NC; ++n)
+#pragma unroll
+    for (int q = 0; q < P; ++q) c[n][q] = x[n][q];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (k < levels) {
+      const int d = 1 << k;
+      double cp[NC][P];
+#pragma unroll
+      for (int n = 0; n < NC; ++n)
+#pragma unroll
+        for (int q = 0; q < P; ++q) cp[n][q] = __shfl_down(c[n][q], d);
+      if (lane + d < WAVE) {
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+#pragma unroll
+          for (int b = 0; b < P; ++b) {
+            const double h = tget(H, k * P + q, b);
+#pragma unroll
+            for (int n = 0; n < NC; ++n) c[n][q] += h * cp[n][b];
+          }
+      }
+    }
+  }
+  op_fence();
+  double s[NC][P];
+#pragma unroll
+  for (int n = 0; n < NC; ++n)
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const double v = __shfl_down(c[n][q], 1);
+      s[n][q] = lane == WAVE - 1 ? 0.0 : v;
+    }
+  const TabRef up2 = relaunder(up), inv2 = relaunder(inv);
+  double dl[NC][M];
+#pragma unroll
+  for (int t = M - 1; t >= 0; --t) {
+#pragma unroll
+    for (int n = 0; n < NC; ++n) dl[n][t] = 0.0;
+#pragma unroll
+    for (int d = 1; d <= P; ++d) {
+      const double u = tget(up2, t, d - 1);
+#pragma unroll
+      for (int n = 0; n < NC; ++n) {
+        const double pv = (t + d < M) ? dl[n][t + d] : s[n][t + d - M];
+        dl[n][t] -= u * pv;
+      }
+    }
+    const double iv = tget(inv2, t, 0);
+#pragma unroll
+    for (int n = 0; n < NC; ++n) {
+      dl[n][t] *= iv;
+      x[n][t] += dl[n][t];
+    }
+  }
+}
+
 // ---- banded matvec y_r = sum_d B(r,d) u_{r-P+d} ---------------------------
 template <int M, int P, int NC>
 __device__ __forceinline__ void bmatvec(const TabRef B, int lane, const double (&u)[NC][M],
@@ -385,6 +537,9 @@ struct YTab {
   const double* upi;   // [3]
   const double* loi;   // [3]
   const double* wbd;   // [2]  d/dy at row 0 and at row my-1 as row functionals
+  // scan compositions [6*P][P] per lane (see lsolve_tab) and level counts
+  const double *hL1, *hU1, *hL2, *hU2, *hLi, *hUi;
+  int kL1, kU1, kL2, kU2, kLi, kUi;
   int my;
 };
 
@@ -394,9 +549,10 @@ __device__ __forceinline__ void op_dy1(const YTab& T, int lane, const double (&u
                                        double (&du)[NC][M], bool apply_fmap = true) {
   bmatvec<M, 3, NC>(tab(T.b1, 7, 0, lane), lane, u, du);
   op_fence();
-  lsolve<M, 3, NC>(tab(T.lo1, 3, 0, lane), lane, du);
+  lsolve_tab<M, 3, NC>(tab(T.lo1, 3, 0, lane), tab(T.hL1, 3, 0, lane), T.kL1, lane, du);
   op_fence();
-  usolve<M, 3, NC>(tab(T.up1, 3, 0, lane), tab(T.inv1, 1, 0, lane), lane, du);
+  usolve_tab<M, 3, NC>(tab(T.up1, 3, 0, lane), tab(T.inv1, 1, 0, lane),
+                       tab(T.hU1, 3, 0, lane), T.kU1, lane, du);
   op_fence();
   if (apply_fmap) {
     const TabRef fm = tab(T.fmap, 1, 0, lane);
@@ -416,9 +572,10 @@ __device__ __forceinline__ void op_dy2(const YTab& T, int lane, const double (&u
                                        double (&d2u)[NC][M]) {
   bmatvec<M, 2, NC>(tab(T.b2, 5, 0, lane), lane, u, d2u);
   op_fence();
-  lsolve<M, 2, NC>(tab(T.lo2, 2, 0, lane), lane, d2u);
+  lsolve_tab<M, 2, NC>(tab(T.lo2, 2, 0, lane), tab(T.hL2, 2, 0, lane), T.kL2, lane, d2u);
   op_fence();
-  usolve<M, 2, NC>(tab(T.up2, 2, 0, lane), tab(T.inv2, 1, 0, lane), lane, d2u);
+  usolve_tab<M, 2, NC>(tab(T.up2, 2, 0, lane), tab(T.inv2, 1, 0, lane),
+                       tab(T.hU2, 2, 0, lane), T.kU2, lane, d2u);
   op_fence();
 }
 

@@ -120,14 +120,29 @@ __device__ __forceinline__ void dft5(T2* v, int s) {
   v[3] = csub(m2, r2);
 }
 
+// LDS layout of FFT work buffers: one pad entry after every 8, so that the
+// stride-R butterfly writes of the first passes spread over the banks.
+#ifndef LOMA_LDS_PAD
+#define LOMA_LDS_PAD 1
+#endif
+// waves per SIMD the x-line and z-transform kernels are compiled for
+#ifndef LOMA_XL_WPE
+#define LOMA_XL_WPE 3
+#endif
+#ifndef LOMA_ZF_WPE
+#define LOMA_ZF_WPE 4
+#endif
+__device__ __forceinline__ int lds_i(int i) { return LOMA_LDS_PAD ? i + (i >> 3) : i; }
+__host__ __device__ inline int lds_entries(int n) { return LOMA_LDS_PAD ? n + (n >> 3) + 1 : n; }
+
 template <int R, typename T, typename T2>
 __device__ __forceinline__ void stockham_item(const T2* __restrict__ src, T2* __restrict__ dst,
-                                              int j, int nb, int Ns, int tstep,
+                                              int rowbase, int j, int nb, int Ns, int tstep,
                                               const T2* __restrict__ tw, int s) {
   T2 v[R];
   const int jm = j % Ns;
 #pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = src[j + r * nb];
+  for (int r = 0; r < R; ++r) v[r] = src[lds_i(rowbase + j + r * nb)];
   if (Ns > 1) {
 #pragma unroll
     for (int r = 1; r < R; ++r) {
@@ -141,14 +156,15 @@ __device__ __forceinline__ void stockham_item(const T2* __restrict__ src, T2* __
   if (R == 4) dft4<T, T2>(v, s);
   if (R == 5) dft5<T, T2>(v, s);
   if (R == 8) dft8<T, T2>(v, s);
-  const int idxD = (j / Ns) * Ns * R + jm;
+  const int idxD = rowbase + (j / Ns) * Ns * R + jm;
 #pragma unroll
-  for (int r = 0; r < R; ++r) dst[idxD + r * Ns] = v[r];
+  for (int r = 0; r < R; ++r) dst[lds_i(idxD + r * Ns)] = v[r];
 }
 
-// nrows independent length-N transforms of rows stored with stride N in a;
-// b is scratch of the same size.  Returns the buffer holding the result.
-// s = -1 forward (e^{-i}), +1 inverse; unscaled.  Ends with a barrier.
+// nrows independent length-N transforms of rows stored with stride N in a
+// (LDS, padded layout lds_i); b is scratch of the same size.  Returns the
+// buffer holding the result.  s = -1 forward (e^{-i}), +1 inverse; unscaled.
+// Ends with a barrier.
 template <typename T, typename T2>
 __device__ T2* fft_rows(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
                         const T2* __restrict__ tw, int s) {
@@ -160,14 +176,13 @@ __device__ T2* fft_rows(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
     const int total = nrows * nb;
     for (int it = threadIdx.x; it < total; it += blockDim.x) {
       const int row = it / nb, j = it - row * nb;
-      const T2* src = a + row * N;
-      T2* dst = b + row * N;
+      const int rb = row * N;
       switch (R) {
-        case 2: stockham_item<2, T, T2>(src, dst, j, nb, Ns, tstep, tw, s); break;
-        case 3: stockham_item<3, T, T2>(src, dst, j, nb, Ns, tstep, tw, s); break;
-        case 4: stockham_item<4, T, T2>(src, dst, j, nb, Ns, tstep, tw, s); break;
-        case 5: stockham_item<5, T, T2>(src, dst, j, nb, Ns, tstep, tw, s); break;
-        default: stockham_item<8, T, T2>(src, dst, j, nb, Ns, tstep, tw, s); break;
+        case 2: stockham_item<2, T, T2>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
+        case 3: stockham_item<3, T, T2>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
+        case 4: stockham_item<4, T, T2>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
+        case 5: stockham_item<5, T, T2>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
+        default: stockham_item<8, T, T2>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
       }
     }
     __syncthreads();
@@ -194,7 +209,7 @@ __device__ __forceinline__ void radix_pass_ip(T2* a, int N, int nrows, int Ns, i
     if (it < total) {
       const int row = it / nb, j = it - row * nb;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[k * R + r] = a[row * N + j + r * nb];
+      for (int r = 0; r < R; ++r) v[k * R + r] = a[lds_i(row * N + j + r * nb)];
     }
   }
   __syncthreads();
@@ -220,7 +235,7 @@ __device__ __forceinline__ void radix_pass_ip(T2* a, int N, int nrows, int Ns, i
       if (R == 8) dft8<T, T2>(w, s);
       const int idxD = row * N + (j / Ns) * Ns * R + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) a[idxD + r * Ns] = w[r];
+      for (int r = 0; r < R; ++r) a[lds_i(idxD + r * Ns)] = w[r];
     }
   }
   __syncthreads();
@@ -252,7 +267,7 @@ __device__ void fft_rows_ip(T2* a, int N, int nrows, const FftPlan& pl,
 // One workgroup: J consecutive y entries of one kx plane of one field, all
 // z; one LDS buffer, in-place FFT (J * mgalz <= 256 * E).
 template <typename T, int E>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_zfft_inv(FftArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE))) void k_zfft_inv(FftArgs a) {
   using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = a.rc.mgalz, J = a.J, my = a.rc.my, mz = a.rc.mz, nz = a.rc.nz;
@@ -266,25 +281,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int nzero = N - mz;
   for (int it = threadIdx.x; it < nzero * J; it += blockDim.x) {
     const int q = it / J, jj = it - q * J;
-    A[jj * N + nz + 1 + q] = T2{0, 0};
+    A[lds_i(jj * N + nz + 1 + q)] = T2{0, 0};
   }
   for (int it = threadIdx.x; it < mz * J; it += blockDim.x) {
     const int k = it / J, jj = it - k * J, j = j0 + jj;
     const int z = (k <= nz) ? k : k + N - mz;
     T2 v = T2{0, 0};
     if (j < my) v = src[((size_t)i * mz + k) * my + j];
-    A[jj * N + z] = v;
+    A[lds_i(jj * N + z)] = v;
   }
   __syncthreads();
   fft_rows_ip<T, T2, E>(A, N, J, a.plz, (const T2*)a.twz, +1);
   for (int it = threadIdx.x; it < N * J; it += blockDim.x) {
     const int z = it / J, jj = it - z * J, j = j0 + jj;
-    if (j < my) dst[((size_t)i * N + z) * my + j] = A[jj * N + z];
+    if (j < my) dst[((size_t)i * N + z) * my + j] = A[lds_i(jj * N + z)];
   }
 }
 
 template <typename T, int E>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_zfft_fwd(FftArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE))) void k_zfft_fwd(FftArgs a) {
   using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = a.rc.mgalz, J = a.J, my = a.rc.my, mz = a.rc.mz, nz = a.rc.nz;
@@ -298,7 +313,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int z = it / J, jj = it - z * J, j = j0 + jj;
     T2 v = T2{0, 0};
     if (j < my) v = src[((size_t)i * N + z) * my + j];
-    A[jj * N + z] = v;
+    A[lds_i(jj * N + z)] = v;
   }
   __syncthreads();
   fft_rows_ip<T, T2, E>(A, N, J, a.plz, (const T2*)a.twz, -1);
@@ -307,7 +322,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int k = it / J, jj = it - k * J, j = j0 + jj;
     if (j < my) {
       const int z = (k <= nz) ? k : k + N - mz;
-      T2 v = A[jj * N + z];
+      T2 v = A[lds_i(jj * N + z)];
       v.x *= sc;
       v.y *= sc;
       dst[((size_t)i * mz + k) * my + j] = v;
@@ -353,7 +368,7 @@ __device__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src, T2* A, T
   const int nzero = N - 2 * np + 1;
   for (int it = threadIdx.x; it < pairs * nzero; it += blockDim.x) {
     const int p = it / nzero, q = it - p * nzero;
-    A[p * N + np + q] = T2{0, 0};
+    A[lds_i(p * N + np + q)] = T2{0, 0};
   }
   for (int it = threadIdx.x; it < np * pairs; it += blockDim.x) {
     const int ii = it / pairs, p = it - ii * pairs;
@@ -362,13 +377,13 @@ __device__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src, T2* A, T
     T2 ca = T2{0, 0}, cb = T2{0, 0};
     if (ja < my) ca = src[base + ja];
     if (jb < my) cb = src[base + jb];
-    T2* row = A + p * N;
+    const int rb = p * N;
     if (ii == 0) {
       // the imaginary part of the kx=0 coefficient is ignored (rft, rftsingle.f:87-91)
-      row[0] = T2{ca.x, cb.x};
+      A[lds_i(rb)] = T2{ca.x, cb.x};
     } else {
-      row[ii] = T2{ca.x - cb.y, ca.y + cb.x};       // X_a + i X_b
-      row[N - ii] = T2{ca.x + cb.y, -ca.y + cb.x};  // conj(X_a) + i conj(X_b)
+      A[lds_i(rb + ii)] = T2{ca.x - cb.y, ca.y + cb.x};           // X_a + i X_b
+      A[lds_i(rb + N - ii)] = T2{ca.x + cb.y, -ca.y + cb.x};      // conj(X_a) + i conj(X_b)
     }
   }
   __syncthreads();
@@ -387,9 +402,9 @@ __device__ void xl_forward(const FftArgs& a, T2* __restrict__ dst, T2* A, T2* B,
   for (int it = threadIdx.x; it < np * pairs; it += blockDim.x) {
     const int ii = it / pairs, p = it - ii * pairs;
     const int ja = j0 + 2 * p, jb = ja + 1;
-    const T2* row = R + p * N;
-    const T2 zk = row[ii];
-    T2 zc = row[(N - ii) % N];
+    const int rb = p * N;
+    const T2 zk = R[lds_i(rb + ii)];
+    T2 zc = R[lds_i(rb + (N - ii) % N)];
     zc.y = -zc.y;
     T2 ya{(zk.x + zc.x) * hs, (zk.y + zc.y) * hs};
     T2 yb{(zk.y - zc.y) * hs, -(zk.x - zc.x) * hs};  // (zk - zc) / (2i)
@@ -401,27 +416,107 @@ __device__ void xl_forward(const FftArgs& a, T2* __restrict__ dst, T2* A, T2* B,
   __syncthreads();
 }
 
+// Mean split: the transform inputs carry no 00 mode; its y-profile comes in
+// FP64 (mean_in) and is added back here, per line.  Each output has an FP64
+// per-y estimate g (the same algebra on the profiles) subtracted before the
+// forward transform and stored in mean_out for phase C to add back, so the
+// transforms (FP32 in mixed mode) only carry perturbation-sized values.
+constexpr int XL_MAXL = 32;  // lines per workgroup (2 * pairs, pairs <= 16)
+
+// Makes register arrays opaque to the optimiser at a loop head, so that
+// per-output expressions are not hoisted out of the output loop and kept
+// live for all outputs at once.
+template <typename T, int E>
+__device__ __forceinline__ void opaque_regs(T (&v)[E][2]) {
+#pragma unroll
+  for (int m = 0; m < E; ++m)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) asm volatile("" : "+v"(v[m][h]));
+}
+
+struct XlRk {
+  double dtxi, dtgamma, dtbeta, ab;
+};
+__device__ __forceinline__ XlRk xl_rk(double dt, int rkstep) {
+  const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+  const double xis[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+  const double p0 = 5.0 / 66.0, p1 = -1.0;
+  const double b3 = (p1 * (gam[0] * gam[2] + gam[1] * gam[2] + xis[1] * gam[2] + gam[0] * xis[2]) -
+                     gam[0] * gam[1] * gam[2] - gam[0] * gam[1] * xis[2]) /
+                    (p1 - gam[0] * gam[1]);
+  const double betas[3] = {gam[0] - p0, p1, b3};
+  const double alphas[3] = {p0, gam[1] + xis[1] - p1, gam[2] + xis[2] - b3};
+  const int rk = rkstep - 1;
+  return XlRk{dt * xis[rk], dt * gam[rk], dt * betas[rk], alphas[rk] / betas[rk]};
+}
+
 template <typename T, bool WZ, int EPT>
-__global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_xline(FftArgs a) {
+__global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(LOMA_XL_WPE))) void k_xline(FftArgs a) {
   using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double gln[12][XL_MAXL];   // output estimates per line
   const RunConst& rc = a.rc;
   const int N = rc.mgalx, my = rc.my;
-  const int pairs = a.L / 2;
+  const int pairs = a.L / 2;            // a power of two dividing XL_THREADS
   const int nyb = (my + a.L - 1) / a.L;
   const int tile = xcd_tile(blockIdx.x, rc.mgalz * nyb);
   if (tile >= rc.mgalz * nyb) return;
   const int z = tile / nyb, j0 = (tile - z * nyb) * a.L;
   T2* A = (T2*)smem;
-  T2* B = A + pairs * N;
+  T2* B = A + lds_entries(pairs * N);
   const int tot = pairs * N;
-  // entry e = threadIdx.x + XL_THREADS*m of the packed rows: pair e / N, x e % N
+  constexpr int NIN = WZ ? 12 : 8;
+  const double ipe = 1.0 / rc.peclet;
+  const XlRk k = xl_rk(a.sc->dt, a.rkstep);
+  const int rnew = a.rkstep & 1, rold = (a.rkstep - 1) & 1;
+  // thread -> one line pair p (lines 2p, 2p+1) and entries x = x0 + TP*m
+  const int TP = XL_THREADS / pairs;
+  const int p = threadIdx.x / TP, x0 = threadIdx.x - p * TP;
+  const int jl[2] = {j0 + 2 * p, j0 + 2 * p + 1};
+  auto prof = [&](int h, int q) -> double {  // y-profile q of line h (0 past my)
+    return (jl[h] < my && q < NIN) ? a.mean_in[(size_t)q * my + jl[h]] : 0.0;
+  };
+  // output estimates per line (one thread per line)
+  if (threadIdx.x < a.L) {
+    const int l = threadIdx.x, j = j0 + l;
+    const bool in = j < my;
+    double m[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) m[q] = (in && q < NIN) ? a.mean_in[(size_t)q * my + j] : 0.0;
+    const double mu = m[0], mv = m[1], mw = m[2], mT = in ? m[7] : 1.0;
+    double g[12];
+    g[0] = mu * mT; g[1] = mv * mT; g[2] = mw * mT;
+    g[3] = mu * (mu * mT); g[4] = mv * (mv * mT); g[5] = mw * (mw * mT);
+    g[6] = mu * (mv * mT); g[7] = mu * (mw * mT); g[8] = mv * (mw * mT);
+    // RHS of T on the profiles (cross.f:1631-1637) and what it makes of
+    // dT and drho (cross.f:1669-1673)
+    const double rb = mT * (m[3] * ipe - (mu * m[4] + mv * m[5] + mw * m[6]));
+    const double dTg = k.dtxi * (in ? a.rbar[(size_t)rold * my + j] : 0.0) + k.dtgamma * rb;
+    g[10] = dTg;
+    g[9] = (dTg / (mT * (mT + dTg))) / k.dtbeta - k.ab * rb / (mT * mT);
+    double rzb = 0.0;
+    g[11] = 0.0;
+    if (WZ) {
+      rzb = mT * (m[11] * ipe - (mu * m[8] + mv * m[9] + mw * m[10]));
+      g[11] = k.dtxi * (in ? a.rzbar[(size_t)rold * my + j] : 0.0) + k.dtgamma * rzb;
+    }
+#pragma unroll
+    for (int q = 0; q < 12; ++q) gln[q][l] = g[q];
+    if (z == 0 && in) {
+#pragma unroll
+      for (int q = 0; q < 12; ++q) a.mean_out[(size_t)q * my + j] = g[q];
+      a.rbar[(size_t)rnew * my + j] = rb;
+      if (WZ) a.rzbar[(size_t)rnew * my + j] = rzb;
+    }
+  }
+  __syncthreads();
+  auto ent = [&](int m) { return x0 + TP * m; };  // x of entry m (valid if < N)
   auto grab = [&](const T2* R, T (&v)[EPT][2]) {
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
-      const int e = threadIdx.x + XL_THREADS * m;
+      const int x = ent(m);
       T2 q = T2{0, 0};
-      if (e < tot) q = R[e];
+      if (x < N) q = R[lds_i(p * N + x)];
       v[m][0] = q.x;
       v[m][1] = q.y;
     }
@@ -430,38 +525,40 @@ __global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
   auto put = [&](const T (&v)[EPT][2]) {
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
-      const int e = threadIdx.x + XL_THREADS * m;
-      if (e < tot) A[e] = T2{v[m][0], v[m][1]};
+      const int x = ent(m);
+      if (x < N) A[lds_i(p * N + x)] = T2{v[m][0], v[m][1]};
     }
   };
-  // inputs come from wbuf, outputs go to wout: pass 2 still reads inputs
-  // after pass 1 has written its outputs
   T2* const* wb = (T2* const*)a.wbuf;
   T2* const* wo = (T2* const*)a.wout;
 
   // ---- pass 1: T, rho u_i -> u_i, rho u_i u_j; CFL maxima ----
-  T tt[EPT][2], ru[EPT][2], rv[EPT][2], rw[EPT][2];
+  T tt[EPT][2], ru[EPT][2], rv[EPT][2], rw[EPT][2];  // perturbations
   grab(xl_inverse<T, T2, EPT>(a, wb[7], A, B, z, j0, pairs), tt);
   grab(xl_inverse<T, T2, EPT>(a, wb[0], A, B, z, j0, pairs), ru);
   grab(xl_inverse<T, T2, EPT>(a, wb[1], A, B, z, j0, pairs), rv);
   grab(xl_inverse<T, T2, EPT>(a, wb[2], A, B, z, j0, pairs), rw);
-  const double ipe = 1.0 / rc.peclet;
+  // profiles of rho u, rho v, rho w, T for the two lines
+  double pu[2], pv[2], pw[2], pT[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    pu[h] = prof(h, 0); pv[h] = prof(h, 1); pw[h] = prof(h, 2);
+    pT[h] = jl[h] < my ? prof(h, 7) : 1.0;
+  }
   if (a.do_cfl) {
     double cx = 0.0, cy = 0.0, cz = 0.0, cd = 0.0;
     const double hx = rc.hxalp * 0.5, hz = rc.hzbet * 0.5;
 #pragma unroll
-    for (int m = 0; m < EPT; ++m) {
-      const int e = threadIdx.x + XL_THREADS * m;
-      const int p = e / N;
+    for (int h = 0; h < 2; ++h) {
+      if (jl[h] >= my) continue;
+      const double ihy = 1.0 / a.hy[jl[h]];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int j = j0 + 2 * p + h;
-        if (e >= tot || j >= my) continue;
-        const double Tt = tt[m][h];
-        const double ihy = 1.0 / a.hy[j];
-        cx = fmax(cx, fabs((double)ru[m][h] * Tt));
-        cy = fmax(cy, fabs((double)rv[m][h] * Tt) * ihy);  // cross.f:1705-1707
-        cz = fmax(cz, fabs((double)rw[m][h] * Tt));
+      for (int m = 0; m < EPT; ++m) {
+        if (ent(m) >= N) continue;
+        const double Tt = (double)tt[m][h] + pT[h];
+        cx = fmax(cx, fabs(((double)ru[m][h] + pu[h]) * Tt));
+        cy = fmax(cy, fabs(((double)rv[m][h] + pv[h]) * Tt) * ihy);  // cross.f:1705-1707
+        cz = fmax(cz, fabs(((double)rw[m][h] + pw[h]) * Tt));
         cd = fmax(cd, 2.0 * (ipe * Tt) * (hx * hx + hz * hz + ihy * ihy));
       }
     }
@@ -479,36 +576,46 @@ __global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
     }
   }
   // outputs 0 u 1 v 2 w 3 ruu 4 rvv 5 rww 6 ruv 7 ruw 8 rvw  (cross.f:1613-1651)
+  // minus the product of the profiles, expanded exactly so that every term
+  // carries a perturbation factor:  (P+p)(Q+q)(R+r) - PQR
+  //   = p (Q+q)(R+r) + P (q (R+r) + Q r)
 #pragma unroll 1
   for (int q = 0; q < 9; ++q) {
+    opaque_regs(tt);
+    opaque_regs(ru);
+    opaque_regs(rv);
+    opaque_regs(rw);
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
-      const int e = threadIdx.x + XL_THREADS * m;
-      if (e >= tot) continue;
+      const int x = ent(m);
+      if (x >= N) continue;
       T val[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const T Tt = tt[m][h], a_ = ru[m][h], b_ = rv[m][h], c_ = rw[m][h];
-        T x;
+        const T A_ = (T)pu[h], B_ = (T)pv[h], C_ = (T)pw[h], P_ = (T)pT[h];
+        const T a_ = ru[m][h], b_ = rv[m][h], c_ = rw[m][h], t_ = tt[m][h];
+        const T TT = P_ + t_;
+        T v;
         switch (q) {
-          case 0: x = a_ * Tt; break;
-          case 1: x = b_ * Tt; break;
-          case 2: x = c_ * Tt; break;
-          case 3: x = a_ * (a_ * Tt); break;
-          case 4: x = b_ * (b_ * Tt); break;
-          case 5: x = c_ * (c_ * Tt); break;
-          case 6: x = a_ * (b_ * Tt); break;
-          case 7: x = a_ * (c_ * Tt); break;
-          default: x = b_ * (c_ * Tt); break;
+          case 0: v = a_ * TT + A_ * t_; break;
+          case 1: v = b_ * TT + B_ * t_; break;
+          case 2: v = c_ * TT + C_ * t_; break;
+          case 3: v = a_ * (A_ + a_) * TT + A_ * (a_ * TT + A_ * t_); break;
+          case 4: v = b_ * (B_ + b_) * TT + B_ * (b_ * TT + B_ * t_); break;
+          case 5: v = c_ * (C_ + c_) * TT + C_ * (c_ * TT + C_ * t_); break;
+          case 6: v = a_ * (B_ + b_) * TT + A_ * (b_ * TT + B_ * t_); break;
+          case 7: v = a_ * (C_ + c_) * TT + A_ * (c_ * TT + C_ * t_); break;
+          default: v = b_ * (C_ + c_) * TT + B_ * (c_ * TT + C_ * t_); break;
         }
-        val[h] = x;
+        val[h] = v;
       }
-      A[e] = T2{val[0], val[1]};
+      A[lds_i(p * N + x)] = T2{val[0], val[1]};
     }
     xl_forward<T, T2, EPT>(a, wo[q], A, B, z, j0, pairs);
   }
 
   // ---- pass 2: r = T (lap T / Pe - rho u . grad T)  (cross.f:1652-1673) ----
+  // acc holds the perturbation part; the profile part is per line
   T acc[EPT][2], accz[EPT][2];
   {
     T f[EPT][2];
@@ -524,66 +631,68 @@ __global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
 #pragma unroll
         for (int h = 0; h < 2; ++h) accz[m][h] = f[m][h] * (T)ipe;
     }
-    // gradient terms: T_x (4), T_y (5), T_z (6); Z_x (8), Z_y (9), Z_z (10)
+    // gradient terms: T_x (4), T_y (5), T_z (6); Z_x (8), Z_y (9), Z_z (10):
+    // (R + r)(G + g) - R G = (R + r) g + r G
 #pragma unroll 1
     for (int c = 0; c < 3; ++c) {
+      opaque_regs(ru);
+      opaque_regs(rv);
+      opaque_regs(rw);
       grab(xl_inverse<T, T2, EPT>(a, wb[4 + c], A, B, z, j0, pairs), f);
 #pragma unroll
-      for (int m = 0; m < EPT; ++m)
+      for (int h = 0; h < 2; ++h) {
+        const T R_ = (T)(c == 0 ? pu[h] : (c == 1 ? pv[h] : pw[h])), G_ = (T)prof(h, 4 + c);
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int m = 0; m < EPT; ++m) {
           const T rc_ = c == 0 ? ru[m][h] : (c == 1 ? rv[m][h] : rw[m][h]);
-          acc[m][h] -= rc_ * f[m][h];
+          acc[m][h] -= (rc_ + R_) * f[m][h] + rc_ * G_;
         }
+      }
       if constexpr (WZ) {
         grab(xl_inverse<T, T2, EPT>(a, wb[8 + c], A, B, z, j0, pairs), f);
 #pragma unroll
-        for (int m = 0; m < EPT; ++m)
+        for (int h = 0; h < 2; ++h) {
+          const T R_ = (T)(c == 0 ? pu[h] : (c == 1 ? pv[h] : pw[h])), G_ = (T)prof(h, 8 + c);
 #pragma unroll
-          for (int h = 0; h < 2; ++h) {
+          for (int m = 0; m < EPT; ++m) {
             const T rc_ = c == 0 ? ru[m][h] : (c == 1 ? rv[m][h] : rw[m][h]);
-            accz[m][h] -= rc_ * f[m][h];
+            accz[m][h] -= (rc_ + R_) * f[m][h] + rc_ * G_;
           }
+        }
       }
     }
   }
-  // time advance of T in physical space, rhst history in FP64
-  const double dt = a.sc->dt;
-  const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
-  const double xis[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
-  const double p0 = 5.0 / 66.0, p1 = -1.0;
-  const double b3 = (p1 * (gam[0] * gam[2] + gam[1] * gam[2] + xis[1] * gam[2] + gam[0] * xis[2]) -
-                     gam[0] * gam[1] * gam[2] - gam[0] * gam[1] * xis[2]) /
-                    (p1 - gam[0] * gam[1]);
-  const double betas[3] = {gam[0] - p0, p1, b3};
-  const double alphas[3] = {p0, gam[1] + xis[1] - p1, gam[2] + xis[2] - b3};
-  const int rk = a.rkstep - 1;
-  const double dtxi = dt * xis[rk], dtgamma = dt * gam[rk], dtbeta = dt * betas[rk];
-  const double ab = alphas[rk] / betas[rk];
-  // ru <- DT, rv <- drho, rw <- DZ (rho u_i are no longer needed)
+  // time advance of T in physical space (rhst history, FP64); outputs minus
+  // their profile estimates.  ru <- DT, rv <- drho, rw <- DZ.
 #pragma unroll
-  for (int m = 0; m < EPT; ++m) {
-    const int e = threadIdx.x + XL_THREADS * m;
-    const int p = e / N, n = e - p * N;
+  for (int h = 0; h < 2; ++h) {
+    const int j = jl[h];
+    const double mT = pT[h];
+    const double accm = prof(h, 3) * ipe -
+                        (pu[h] * prof(h, 4) + pv[h] * prof(h, 5) + pw[h] * prof(h, 6));
+    const double acczm = WZ ? prof(h, 11) * ipe - (pu[h] * prof(h, 8) + pv[h] * prof(h, 9) +
+                                                   pw[h] * prof(h, 10))
+                            : 0.0;
+    const double g9 = gln[9][2 * p + h], g10 = gln[10][2 * p + h], g11 = gln[11][2 * p + h];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int j = j0 + 2 * p + h;
+    for (int m = 0; m < EPT; ++m) {
+      const int x = ent(m);
       ru[m][h] = rv[m][h] = rw[m][h] = (T)0;
-      if (e >= tot || j >= my) continue;
-      const double Tt = tt[m][h];
-      const double r = Tt * (double)acc[m][h];
-      const size_t ri = ((size_t)z * my + j) * N + n;
-      const double rold = a.rhst[ri];
+      if (x >= N || j >= my) continue;
+      const double Tt = (double)tt[m][h] + mT;
+      const double r = Tt * (accm + (double)acc[m][h]);
+      const size_t ri = ((size_t)z * my + j) * N + x;
+      const double rold_ = a.rhst[ri];
       a.rhst[ri] = r;
-      const double d = dtxi * rold + dtgamma * r;
+      const double d = k.dtxi * rold_ + k.dtgamma * r;
       const double tnext = Tt + d;
-      ru[m][h] = (T)d;
-      rv[m][h] = (T)((d / (Tt * tnext)) / dtbeta - ab * r / (Tt * Tt));
+      ru[m][h] = (T)(d - g10);
+      rv[m][h] = (T)((d / (Tt * tnext)) / k.dtbeta - k.ab * r / (Tt * Tt) - g9);
       if constexpr (WZ) {
-        const double rz = Tt * (double)accz[m][h];
+        const double rz = Tt * (acczm + (double)accz[m][h]);
         const double zold = a.rhsz[ri];
         a.rhsz[ri] = rz;
-        rw[m][h] = (T)(dtxi * zold + dtgamma * rz);
+        rw[m][h] = (T)(k.dtxi * zold + k.dtgamma * rz - g11);
       }
     }
   }
@@ -604,15 +713,15 @@ __global__ __launch_bounds__(256) void k_fourz(FftPlan pl, const double2* tw, do
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = pl.n;
   double2* A = (double2*)smem;
-  double2* B = A + N;
+  double2* B = A + lds_entries(N);
   const int row = blockIdx.x;
   if (row >= nrows) return;
-  for (int it = threadIdx.x; it < N; it += blockDim.x) A[it] = c[(size_t)row * N + it];
+  for (int it = threadIdx.x; it < N; it += blockDim.x) A[lds_i(it)] = c[(size_t)row * N + it];
   __syncthreads();
   double2* R = fft_rows<double, double2>(A, B, N, 1, pl, tw, iopt < 0 ? -1 : 1);
   const double sc = iopt < 0 ? 1.0 / N : 1.0;
   for (int it = threadIdx.x; it < N; it += blockDim.x) {
-    double2 v = R[it];
+    double2 v = R[lds_i(it)];
     v.x *= sc;
     v.y *= sc;
     c[(size_t)row * N + it] = v;
@@ -623,10 +732,10 @@ __global__ __launch_bounds__(256) void k_fourz(FftPlan pl, const double2* tw, do
 // launchers
 // ------------------------------------------------------------------------
 static size_t zfft_lds(bool fp32, const FftArgs& a) {
-  return (size_t)a.J * a.rc.mgalz * (fp32 ? 8 : 16);
+  return (size_t)lds_entries(a.J * a.rc.mgalz) * (fp32 ? 8 : 16);
 }
 static size_t xline_lds(bool fp32, const FftArgs& a) {
-  return (size_t)2 * (a.L / 2) * a.rc.mgalx * (fp32 ? 8 : 16);
+  return (size_t)2 * lds_entries((a.L / 2) * a.rc.mgalx) * (fp32 ? 8 : 16);
 }
 
 int launch_zfft_inv(bool fp32, const FftArgs& a, hipStream_t st) {
@@ -670,7 +779,7 @@ int launch_xline(bool fp32, const FftArgs& a, hipStream_t st) {
 int launch_fourz(const FftPlan& pl, const double2* tw, double2* c, int nrows, int iopt,
                  hipStream_t st) {
   if (nrows <= 0) return 0;
-  hipLaunchKernelGGL(k_fourz, dim3(nrows), dim3(256), (size_t)2 * pl.n * 16, st, pl, tw, c,
+  hipLaunchKernelGGL(k_fourz, dim3(nrows), dim3(256), (size_t)2 * lds_entries(pl.n) * 16, st, pl, tw, c,
                      nrows, iopt);
   return (int)hipGetLastError();
 }

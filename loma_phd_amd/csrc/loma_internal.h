@@ -27,6 +27,12 @@ void band_apply(const double* band, int n, int p, const double* u, double* out);
 void band_lu_solve(const double* inv, const double* up, const double* lo, int n,
                    int p, double* b);
 
+// wavefront-scan tables of a fixed banded factor (loma_tables.cpp): writes
+// H[6][P][P][64] and returns the number of scan levels needed.  Lower solve:
+// lo = multipliers (row k into row k+i at lo[k*P+i-1]); upper solve: up, inv.
+int scan_table(int my, int M, int P, bool upper, const double* lo, const double* up,
+               const double* inv, double* H);
+
 void set_error(const std::string& msg);
 
 }  // namespace loma

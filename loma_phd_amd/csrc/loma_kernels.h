@@ -53,6 +53,10 @@ struct PhaseArgs {
   double2 *vor, *phi, *psi, *scal, *ten12, *ten13, *zf;
   void* spec[12];            // S-sized transform buffers (float2 or double2)
   double *u00wk, *v00wk, *w00wk, *rf0u, *rf0w;
+  // mean split of the transforms: y-profiles of the 00 mode of each input
+  // (phase A writes them, the FP32/FP64 transform carries zero there) and
+  // the per-y estimates subtracted from each output ([12][my] each)
+  double *mean_in, *mean_out;
   DevScalars* sc;
   int rkstep;
   int with_z;
@@ -70,6 +74,8 @@ struct FftArgs {
   void* wout[12];            // z-physical transform outputs [i][z][j]
   double* rhst;
   double* rhsz;
+  double *mean_in, *mean_out;  // [12][my], see PhaseArgs
+  double *rbar, *rzbar;        // [2][my] mean-profile RHS of T and Z by substep parity
   const double* hy;          // my
   DevScalars* sc;
   int nfields;
@@ -102,7 +108,7 @@ int launch_xline(bool fp32, const FftArgs& a, hipStream_t st);
 int launch_zfft_fwd(bool fp32, const FftArgs& a, hipStream_t st);
 int launch_fourz(const FftPlan& pl, const double2* tw, double2* c, int nrows, int iopt,
                  hipStream_t st);
-int launch_band5(const double* band, void* rhs, double* x, int n, int nsys, int rhs_fp32,
-                 hipStream_t st);
+int launch_band5(const double* band, const void* rhs, double* x, double* scratch, int n,
+                 int nsys, int rhs_fp32, hipStream_t st);
 
 }  // namespace loma

@@ -190,8 +190,9 @@ __device__ __forceinline__ void op_inty(const YTab& T, int lane, double (&x)[1][
 #pragma unroll
   for (int t = 0; t < M; ++t)
     if (r0 + t == T.my - 1) x[0][t] = 0.0;  // gauge row x(my) = 0
-  lsolve<M, 3, 1>(tab(T.loi, 3, 0, lane), lane, x);
-  usolve<M, 3, 1>(tab(T.upi, 3, 0, lane), tab(T.invi, 1, 0, lane), lane, x);
+  lsolve_tab<M, 3, 1>(tab(T.loi, 3, 0, lane), tab(T.hLi, 3, 0, lane), T.kLi, lane, x);
+  usolve_tab<M, 3, 1>(tab(T.upi, 3, 0, lane), tab(T.invi, 1, 0, lane), tab(T.hUi, 3, 0, lane),
+                      T.kUi, lane, x);
   const double v0 = col_value<M>(x[0], 0);
 #pragma unroll
   for (int t = 0; t < M; ++t) x[0][t] -= v0;
@@ -258,14 +259,58 @@ __device__ __forceinline__ void stage_add(double2* st, int lane, const double (&
   }
 }
 
+// Mean split of the transforms: the 00 mode of a transform input travels in
+// FP64 in `mean` (its real part; the imaginary part of a mean is zero) and
+// the transform buffer carries zero in that column, so FP32 transforms only
+// ever see perturbation-sized values.  The x-line kernel adds the profiles
+// back in physical space.
+template <int M, typename T2>
+__device__ __forceinline__ void store_spec(T2* __restrict__ col, int my, int lane, double2* st,
+                                           const double (&x)[2][M], bool m00,
+                                           double* __restrict__ mean) {
+  if (!m00) {
+    store_col<M>(col, my, lane, st, x);
+    return;
+  }
+  const int r0 = lane * M;
+  wave_sync();
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    if (r0 + t < my) mean[r0 + t] = x[0][t];
+    st[r0 + t] = make_double2(0.0, 0.0);
+  }
+  stage_out<M>(st, my, lane, col);
+}
+// ... and the outputs' 00 mode is the transformed remainder plus the FP64
+// estimate the x-line kernel subtracted (own rows of the stage)
+template <int M>
+__device__ __forceinline__ void stage_add_mean(double2* st, int lane, int my, bool m00,
+                                               const double* __restrict__ mean) {
+  if (!m00) return;
+  const int r0 = lane * M;
+#pragma unroll
+  for (int t = 0; t < M; ++t)
+    if (r0 + t < my) st[r0 + t].x += mean[r0 + t];
+}
+template <int M>
+__device__ __forceinline__ void add_mean(double (&x)[2][M], int lane, int my, bool m00,
+                                         const double* __restrict__ mean) {
+  if (!m00) return;
+  const int r0 = lane * M;
+#pragma unroll
+  for (int t = 0; t < M; ++t)
+    if (r0 + t < my) x[0][t] += mean[r0 + t];
+}
+
 // scalar-field block of phase A: BCs, then f, i kx f, df/dy, i kz f, lap f
 template <int M, typename T2>
 __device__ __forceinline__ void scalar_derivs(const PhaseArgs& a, int lane, double2* st,
                                               size_t o, bool m00, double kx, double kz,
                                               double rK, double2* fld, double bot, double top,
-                                              T2* o_f, T2* o_dx, T2* o_dy, T2* o_dz,
-                                              T2* o_lap) {
+                                              T2* const* sp, int q_f, int q_dx, int q_dy,
+                                              int q_dz, int q_lap) {
   const int my = a.rc.my, r0 = lane * M;
+  double* mi = a.mean_in;
   double F[2][M], d[2][M];
   load_col<M>(fld + o, my, lane, st, F);
 #pragma unroll
@@ -277,20 +322,20 @@ __device__ __forceinline__ void scalar_derivs(const PhaseArgs& a, int lane, doub
       fld[o + r] = make_double2(F[0][t], 0.0);
     }
   }
-  if (o_f) store_col<M>(o_f + o, my, lane, st, F);
+  if (q_f >= 0) store_spec<M>(sp[q_f] + o, my, lane, st, F, m00, mi + (size_t)q_f * my);
   mul_i<M>(F, kx, d);
-  store_col<M>(o_dx + o, my, lane, st, d);
+  store_spec<M>(sp[q_dx] + o, my, lane, st, d, m00, mi + (size_t)q_dx * my);
   mul_i<M>(F, kz, d);
-  store_col<M>(o_dz + o, my, lane, st, d);
+  store_spec<M>(sp[q_dz] + o, my, lane, st, d, m00, mi + (size_t)q_dz * my);
   op_dy1<M, 2>(a.T, lane, F, d);
-  store_col<M>(o_dy + o, my, lane, st, d);
+  store_spec<M>(sp[q_dy] + o, my, lane, st, d, m00, mi + (size_t)q_dy * my);
   op_dy2<M, 2>(a.T, lane, F, d);
 #pragma unroll
   for (int t = 0; t < M; ++t) {
     d[0][t] = d[0][t] - rK * F[0][t];
     d[1][t] = d[1][t] - rK * F[1][t];
   }
-  store_col<M>(o_lap + o, my, lane, st, d);
+  store_spec<M>(sp[q_lap] + o, my, lane, st, d, m00, mi + (size_t)q_lap * my);
 }
 
 template <int M, typename TS>
@@ -387,32 +432,34 @@ __global__ __launch_bounds__(256, 2) void k_phaseA(PhaseArgs a) {
         }
       }
     }
-    store_col<M>(sp[0] + o, my, lane, st, mx);
-    store_col<M>(sp[2] + o, my, lane, st, mz);
+    store_spec<M>(sp[0] + o, my, lane, st, mx, m00, a.mean_in);
+    store_spec<M>(sp[2] + o, my, lane, st, mz, m00, a.mean_in + 2 * (size_t)my);
     op_dy1<M, 2>(a.T, lane, ps, dv);
     // rho v = m_y + d psi/dy
     stage_add<M>(pk, lane, dv, 1.0);
     if (m00) {
+      // rho v 00 mode: carried in FP64 beside the transform
 #pragma unroll
       for (int t = 0; t < M; ++t) {
         const int r = r0 + t;
-        pk[r] = make_double2((r < my) ? a.v00wk[r] : 0.0, 0.0);
+        if (r < my) a.mean_in[(size_t)my + r] = a.v00wk[r];
+        pk[r] = make_double2(0.0, 0.0);
       }
     }
     stage_out<M>(pk, my, lane, sp[1] + o);
   }
   // (4) T BCs and derivatives                           cross.f:338-349, 521-554
   scalar_derivs<M, T2>(a, lane, st, o, m00, kx, kz, rK, a.scal, 1.0 / rc.rhobot,
-                       1.0 / rc.rhotop, sp[7], sp[4], sp[5], sp[6], sp[3]);
+                       1.0 / rc.rhotop, sp, 7, 4, 5, 6, 3);
   if (a.with_z) {
     // passive scalar: BCs and derivatives (lomaHZB cross.f:343-355, 626-666)
-    scalar_derivs<M, T2>(a, lane, st, o, m00, kx, kz, rK, a.zf, rc.zbot, rc.ztop,
-                         (T2*)nullptr, sp[8], sp[9], sp[10], sp[11]);
+    scalar_derivs<M, T2>(a, lane, st, o, m00, kx, kz, rK, a.zf, rc.zbot, rc.ztop, sp, -1, 8,
+                         9, 10, 11);
   }
 }
 
 template <int M, typename TS>
-__global__ __launch_bounds__(256) void k_phaseC(PhaseArgs a) {
+__global__ __launch_bounds__(256, 2) void k_phaseC(PhaseArgs a) {
   using T2 = typename std::conditional<sizeof(TS) == 4, float2, double2>::type;
   LOMA_STAGE2(M);
   const RunConst& rc = a.rc;
@@ -440,11 +487,13 @@ __global__ __launch_bounds__(256) void k_phaseC(PhaseArgs a) {
     double T[2][M];
     load_col<M>(a.scal + o, my, lane, st, T);
     stage_in<M>(sp[10], my, lane, st);
+    stage_add_mean<M>(st, lane, my, m00, a.mean_out + 10 * (size_t)my);
     stage_axpy<M>(st, lane, T, 1.0);
     stage_out<M>(st, my, lane, a.scal + o);
     if (a.with_z) {
       load_col<M>(a.zf + o, my, lane, st, T);
       stage_in<M>(sp[11], my, lane, st);
+      stage_add_mean<M>(st, lane, my, m00, a.mean_out + 11 * (size_t)my);
       stage_axpy<M>(st, lane, T, 1.0);
       stage_out<M>(st, my, lane, a.zf + o);
     }
@@ -599,8 +648,10 @@ __global__ __launch_bounds__(256) void k_phaseC(PhaseArgs a) {
       double* rf = c == 0 ? a.rf0u : a.rf0w;
       double* wk = c == 0 ? a.u00wk : a.w00wk;
       load_col<M>(sp[c == 0 ? 6 : 8], my, lane, st, A);
+      add_mean<M>(A, lane, my, true, a.mean_out + (size_t)(c == 0 ? 6 : 8) * my);
       op_dy1<M, 2>(a.T, lane, A, B);
       load_col<M>(sp[c == 0 ? 0 : 2], my, lane, st, A);
+      add_mean<M>(A, lane, my, true, a.mean_out + (size_t)(c == 0 ? 0 : 2) * my);
 #pragma unroll
       for (int t = 0; t < M; ++t) A[1][t] = 0.0;
       {
@@ -640,6 +691,7 @@ __global__ __launch_bounds__(256) void k_phaseC2(PhaseArgs a) {
   const bool m00 = (i == 0 && k == 0);
   double dr[2][M], ps[2][M];
   load_col<M>((const T2*)a.spec[9] + o, my, lane, st, dr);
+  add_mean<M>(dr, lane, my, m00, a.mean_out + 9 * (size_t)my);
   if (m00) {
     // rho v 00 mode = integral of drho00 + acoustic split (cross.f:1027-1072)
     double x1[1][M];
