@@ -136,6 +136,73 @@ def cpu_baseline(cfg, st, max_threads=16):
                        f"FP64 C oracle, {threads} OpenMP threads, {dt:.1f} s")
 
 
+def bench_ensemble(nruns, steps, warmup, seed=7, workers=8):
+    """Ensemble batch: nruns independent mixed-precision runs (grids drawn
+    from 128x129x64, 192x193x96, 256x257x128; density ratio U[1,8]; Re
+    U[160,800]) advanced by one graph launch per RK3 step (loma_advance_many),
+    tile shapes from the autotuner."""
+    import concurrent.futures as cf
+    from loma_phd_amd import capi, grid, ic
+    cfgs = grid.ensemble_configs(nruns, seed)
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        sts = list(ex.map(ic.initial_condition, cfgs))
+    flags = capi.LOMA_FLAG_SCALAR_Z | capi.LOMA_FLAG_AUTOTUNE
+    runs = []
+    for c, st in zip(cfgs, sts):
+        r = capi.Run(c.mgalx, c.my, c.mgalz, st["y"], c.Re, c.alp, c.bet, c.pr, c.cfl,
+                     c.ncfl, c.dt0, capi.LOMA_MIXED, flags)
+        r.upload(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"],
+                 st["w00"], z=st["z"])
+        r.use_graph(True)
+        runs.append(r)
+    del sts
+    capi.advance_many(runs, warmup)
+    t0 = time.perf_counter()
+    capi.advance_many(runs, steps)
+    wall = time.perf_counter() - t0
+    lat = []
+    for _ in range(10):
+        t1 = time.perf_counter()
+        capi.advance_many(runs, 1)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    pts = sum(c.points for c in cfgs)
+    out = dict(runs=nruns, steps=steps, grid_points=pts,
+               gpts_per_s=round(pts * steps / wall, 1),
+               ms_per_ensemble_step=round(wall * 1e3 / steps, 3),
+               p50_ms=round(float(np.percentile(lat, 50)), 3),
+               p99_ms=round(float(np.percentile(lat, 99)), 3),
+               device_bytes=int(sum(r.bytes for r in runs)),
+               grids={f"{g[0]}x{g[1]}x{g[2]}": sum(1 for c in cfgs
+                                                  if (c.mgalx, c.my, c.mgalz) == g)
+                      for g in grid.ENSEMBLE_GRIDS},
+               tiles=sorted({(c.mgalx, c.my, c.mgalz, *r.tiles()) for c, r in zip(cfgs, runs)}),
+               precision="mixed", launch="one hipGraph launch per RK3 step")
+    for r in runs:
+        r.close()
+    return out
+
+
+def bench_served(window, rate=2.0, seed=11, max_slots=32, workers=4):
+    """Served ensemble: Poisson arrivals (rate per s) over `window` seconds,
+    grids from 64x129x64 to 384x385x192, density ratios U[1,8], step budgets
+    U[200,2000], admission by device footprint; runs still resident at the
+    end of the window are cut after their step in flight."""
+    import torch
+    from loma_phd_amd.serve import HipBackend, Server, poisson_requests
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(0.8 * free)
+    reqs = poisson_requests(seed, rate, window)
+    rep = Server(HipBackend(), mem_budget=budget, max_slots=max_slots,
+                 workers=workers).run(reqs, window=window)
+    return dict(window_s=window, arrival_rate_per_s=rate, requests=len(reqs),
+                gpts_per_s=round(rep["gp_steps_per_s"], 1), steps=rep["steps"],
+                p50_ms=round(rep["p50_ms"], 3), p99_ms=round(rep["p99_ms"], 3),
+                device_p50_ms=round(rep["device_p50_ms"], 3),
+                peak_active=rep["peak_active"], peak_bytes=rep["peak_bytes"],
+                budget_bytes=budget, outcomes=rep["counts"], precision="mixed",
+                latency="host wall time from step enqueue to observed completion")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +212,11 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--ensemble-runs", type=int, default=32,
+                    help="runs in the ensemble-batch leg (0 skips it)")
+    ap.add_argument("--ensemble-steps", type=int, default=30)
+    ap.add_argument("--served-window", type=float, default=30.0,
+                    help="seconds of Poisson arrivals in the served leg (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -208,6 +280,20 @@ def main():
     if "fp64" in res and "mixed" in res:
         line["speedup_mixed_vs_fp64"] = round(res["mixed"]["gpts_per_s"] /
                                               res["fp64"]["gpts_per_s"], 3)
+    if args.ensemble_runs > 0:
+        print(f"[bench] ensemble batch of {args.ensemble_runs} runs", file=sys.stderr,
+              flush=True)
+        e = bench_ensemble(args.ensemble_runs, args.ensemble_steps, args.warmup)
+        if "fp64" in res:
+            e["vs_fp64_single"] = round(e["gpts_per_s"] / res["fp64"]["gpts_per_s"], 3)
+        line["ensemble"] = e
+    if args.served_window > 0:
+        print(f"[bench] served ensemble, {args.served_window:g} s window", file=sys.stderr,
+              flush=True)
+        sv = bench_served(args.served_window)
+        if "fp64" in res:
+            sv["vs_fp64_single"] = round(sv["gpts_per_s"] / res["fp64"]["gpts_per_s"], 3)
+        line["served"] = sv
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(cfg, st)

@@ -41,6 +41,9 @@ extern "C" {
                                  (trunk/lomacte3/src/modulos.f:87-90) */
 #define LOMA_FLAG_SCALAR_Z 2  /* transport a passive scalar Z
                                  (branches/lomaHZB/src/cross.f:2736) */
+#define LOMA_FLAG_AUTOTUNE 4  /* time the transform tile shapes for this grid
+                                 at init and keep the fastest (no reference
+                                 counterpart: a tuning knob of this library) */
 
 typedef struct loma_run loma_run;
 
@@ -158,10 +161,39 @@ int loma_fourz_dev(loma_run *run, double *c, int nrows, int iopt);
 int loma_band5_solve_dev(const double *band, void *rhs, double *x, int n,
                          int nsys, int rhs_fp32, void *stream);
 
+/* Transform tile shapes in use: y entries per z-transform workgroup (J) and
+ * lines per x-line workgroup (L).  No reference counterpart. */
+int loma_tiles(const loma_run *run, int *J, int *L);
+
 /* ---- ensembles -------------------------------------------------------- */
-/* Advance several runs together, one launch per substep kernel for all
- * runs (runs may differ in grid, Re, density ratio). */
+/* Advance several independent runs (grids, Re, density ratios may differ)
+ * nsteps full RK3 steps each.  With graphs enabled on every run, all runs'
+ * steps are captured into one hipGraph with one independent branch per run
+ * (each on the run's own stream) and one graph launch advances the whole
+ * ensemble by a step; otherwise the runs' kernels are enqueued on their own
+ * streams.  Replaces a driver loop of cross1 calls over runs (no reference
+ * counterpart for the batching itself). */
 int loma_advance_many(loma_run **runs, int nruns, int nsteps);
+
+/* ---- serving ---------------------------------------------------------- */
+/* Device bytes that loma_init would allocate for this grid, precision and
+ * flags (the sum of its requested buffer sizes, equal to the last entry of
+ * loma_dims after init), without touching the device: the admission test of
+ * a request loop.  No reference counterpart (the reference sizes its arrays
+ * at compile time, ctes3D). */
+int loma_footprint(int mgalx, int my, int mgalz, int precision, int flags,
+                   long long *bytes);
+
+/* One RK3 step in flight per run, for a request loop that interleaves many
+ * runs: loma_step_begin enqueues one step on the run's stream (its graph
+ * when graphs are enabled) and returns at once; loma_step_poll reports
+ * without blocking whether that step has finished (*done), its device time
+ * in ms and the run's non-finite flag.  Beginning a step while one is in
+ * flight, or polling with none, is LOMA_ERR_STATE.  These replace one
+ * iteration of the cross1 step loop (cross.f:225-1226) for a caller that
+ * does not wait. */
+int loma_step_begin(loma_run *run);
+int loma_step_poll(loma_run *run, int *done, double *device_ms, int *nonfinite);
 
 #ifdef __cplusplus
 }

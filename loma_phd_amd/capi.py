@@ -17,7 +17,7 @@ LIB = os.environ.get("LOMA_HIP_LIB") or os.path.join(PKG, "_lib", "libloma_hip.s
 LOMA_OK, LOMA_ERR_ARG, LOMA_ERR_NAN, LOMA_ERR_OOM = 0, 1, 2, 3
 LOMA_ERR_DEVICE, LOMA_ERR_STATE, LOMA_ERR_NODEV = 4, 5, 6
 LOMA_FP64, LOMA_MIXED = 0, 1
-LOMA_FLAG_DT_FP32, LOMA_FLAG_SCALAR_Z = 1, 2
+LOMA_FLAG_DT_FP32, LOMA_FLAG_SCALAR_Z, LOMA_FLAG_AUTOTUNE = 1, 2, 4
 
 # every symbol the header declares (checked by tests on the CPU)
 EXPORTS = [
@@ -28,6 +28,7 @@ EXPORTS = [
     "loma_deryyr2_dev", "loma_lapvdvhom_dev", "loma_solvepsi_dev", "loma_helmholtz_dev",
     "loma_inty8_dev", "loma_deryr2", "loma_deryyr2", "loma_lapvdvhom", "loma_solvepsi",
     "loma_fourz_dev", "loma_band5_solve_dev", "loma_advance_many", "loma_advance_async",
+    "loma_tiles", "loma_footprint", "loma_step_begin", "loma_step_poll",
     "loma_sync", "loma_stream", "loma_profile", "loma_profile_read",
 ]
 KERNEL_CLASSES = ["phaseA", "zfft_inv", "xline", "zfft_fwd", "phaseC", "finish", "phaseC2"]
@@ -84,6 +85,10 @@ def lib():
             "loma_fourz_dev": ([_vp, _vp, i, i], i),
             "loma_band5_solve_dev": ([_vp, _vp, _vp, i, i, i, _vp], i),
             "loma_advance_many": ([ctypes.POINTER(_vp), i, i], i),
+            "loma_tiles": ([_vp, ctypes.POINTER(i), ctypes.POINTER(i)], i),
+            "loma_footprint": ([i, i, i, i, i, ctypes.POINTER(ctypes.c_longlong)], i),
+            "loma_step_begin": ([_vp], i),
+            "loma_step_poll": ([_vp, ctypes.POINTER(i), _dp, ctypes.POINTER(i)], i),
             "loma_advance_async": ([_vp, i], i),
             "loma_sync": ([_vp], i),
             "loma_stream": ([_vp, ctypes.POINTER(_vp)], i),
@@ -125,6 +130,13 @@ def band5_solve_dev(band_ptr, rhs_ptr, x_ptr, n, nsys, rhs_fp32=False, stream=No
     (complex64 when rhs_fp32), x [nsys][n] complex128 or None for in place."""
     check(lib().loma_band5_solve_dev(band_ptr, rhs_ptr, x_ptr, n, nsys, 1 if rhs_fp32 else 0,
                                      stream))
+
+
+def footprint(mgalx, my, mgalz, precision=LOMA_FP64, flags=0) -> int:
+    """Device bytes a run of this grid allocates (host-only, no GPU needed)."""
+    b = ctypes.c_longlong(0)
+    check(lib().loma_footprint(mgalx, my, mgalz, precision, flags, ctypes.byref(b)))
+    return int(b.value)
 
 
 def device_count() -> int:
@@ -217,6 +229,19 @@ class Run:
         check(lib().loma_profile_read(self._h, _d(ms), cnt, n))
         return {k: (float(ms[q]), int(cnt[q])) for q, k in enumerate(KERNEL_CLASSES)}
 
+    def step_begin(self):
+        """Enqueue one RK3 step and return at once (one step in flight)."""
+        check(lib().loma_step_begin(self._h))
+
+    def step_poll(self):
+        """(done, device_ms, nonfinite) of the step in flight, without
+        blocking."""
+        done, nf = ctypes.c_int(0), ctypes.c_int(0)
+        ms = ctypes.c_double(0.0)
+        check(lib().loma_step_poll(self._h, ctypes.byref(done), ctypes.byref(ms),
+                                   ctypes.byref(nf)))
+        return bool(done.value), float(ms.value), bool(nf.value)
+
     def substep(self, rk):
         check(lib().loma_substep(self._h, rk))
 
@@ -249,6 +274,13 @@ class Run:
 
     def inty8_dev(self, du, u):
         check(lib().loma_inty8_dev(self._h, du, u))
+
+    def tiles(self):
+        """(J, L): y entries per z-transform workgroup, lines per x-line
+        workgroup."""
+        J, L = ctypes.c_int(0), ctypes.c_int(0)
+        check(lib().loma_tiles(self._h, ctypes.byref(J), ctypes.byref(L)))
+        return J.value, L.value
 
     def fourz_dev(self, c_ptr, nrows, iopt):
         check(lib().loma_fourz_dev(self._h, c_ptr, nrows, iopt))

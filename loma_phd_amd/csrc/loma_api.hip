@@ -22,6 +22,11 @@ using namespace loma;
 
 namespace loma {
 static thread_local std::string g_err;
+// Serialises the calls that allocate, free, use the null stream or capture a
+// graph (init, free, upload, download, graph builds): HIP invalidates a
+// stream capture when another thread makes such calls while it records, and
+// runs may be set up on one thread while another advances others.
+static std::recursive_mutex g_setup_mu;
 void set_error(const std::string& msg) { g_err = msg; }
 }  // namespace loma
 
@@ -80,6 +85,7 @@ struct loma_run {
   int J = 8, L = 4;
   int istep = 0;
   bool uploaded = false;
+  bool owns_stream = false;  // stream created by the library
   // graph
   bool use_graph = false;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -87,6 +93,10 @@ struct loma_run {
   bool prof = false;
   struct ProfEv { int cls; hipEvent_t a, b; };
   std::vector<ProfEv> prof_ev;
+  // one step in flight for a request loop (loma_step_begin / loma_step_poll)
+  hipEvent_t sev[2] = {nullptr, nullptr};
+  DevScalars* hsc = nullptr;  // pinned copy of the scalars after the step
+  bool inflight = false;
 };
 
 // kernel classes for profiling
@@ -132,6 +142,30 @@ static int pick_M(int my) {
   return -1;
 }
 
+// Device bytes loma_init requests for a grid: the sum of its allocations, in
+// the order it makes them (kept in step with loma_init; a test compares this
+// with the byte count of a live run).
+static size_t plan_bytes(int mgalx, int my, int mgalz, bool fp32, bool with_z) {
+  const int M = pick_M(my);
+  const size_t R = 64 * (size_t)M;
+  const size_t mx = 2 * (mgalx / 3), np = mx / 2, mz = 2 * (mgalz / 3) - 1;
+  const size_t nz = (mz - 1) / 2, nclass = np * (nz + 1);
+  const size_t myp = (my + 15) / 16 * 16;
+  const size_t S = np * mz * my, Sp = np * mz * myp, Wp = np * mgalz * myp;
+  const size_t P = (size_t)mgalx * my * mgalz, es = fp32 ? 8 : 16;
+  const size_t nsp = with_z ? 12 : 11, nst = with_z ? 7 : 6;
+  size_t b = 8 * (46 * R + 64 * 6 * 44);       // per-row and scan tables
+  b += 2 * 8 * 5 * (size_t)my;                 // band copies
+  b += 8 * nclass + 8 * 5 * R * nclass + 32 * nclass;  // shifts, LU, Neumann constants
+  b += nst * S * 16;                           // spectral state and histories
+  b += nsp * (Sp + 2 * Wp) * es;               // transform buffers
+  b += (with_z ? 2 : 1) * P * 8;               // physical-space RK histories
+  b += (6 + 28) * 8 * (size_t)my;              // 00-mode work and mean profiles
+  b += sizeof(DevScalars);
+  b += (fp32 ? 8 : 16) * (size_t)(mgalz + mgalx) + 16 * (size_t)mgalz;  // twiddles
+  return b;
+}
+
 extern "C" {
 
 const char* loma_version(void) { return "loma_amd 0.1 (gfx950)"; }
@@ -160,20 +194,62 @@ int loma_fd_matrices(int my, const double* y, double* dt11, double* dt12, double
   return LOMA_OK;
 }
 
+}  // extern "C"
+
+// ---- ensembles: one graph launch per step for many runs -----------------
+// The steps of all runs are captured into one hipGraph whose branches (one
+// per run, on the run's own stream) are independent, so the device overlaps
+// the runs' kernels; graphs are cached by run set and CFL pattern.
+struct EnsembleGraph {
+  std::vector<loma_run*> runs;
+  std::vector<int> icfl;
+  hipStream_t master = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+static std::mutex g_ens_mu;
+static std::vector<EnsembleGraph> g_ens;
+
+static void ensemble_forget(loma_run* r) {
+  std::lock_guard<std::mutex> lk(g_ens_mu);
+  for (size_t q = 0; q < g_ens.size();) {
+    bool has = false;
+    for (loma_run* x : g_ens[q].runs) has = has || x == r;
+    if (has) {
+      if (g_ens[q].exec) hipGraphExecDestroy(g_ens[q].exec);
+      if (g_ens[q].master) hipStreamDestroy(g_ens[q].master);
+      g_ens.erase(g_ens.begin() + q);
+    } else {
+      ++q;
+    }
+  }
+}
+
+extern "C" {
+
 int loma_free(loma_run* r) {
   if (!r) return LOMA_OK;
-  for (int q = 0; q < 2; ++q)
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
+  ensemble_forget(r);
+  if (r->stream) hipStreamSynchronize(r->stream);
+  for (int q = 0; q < 2; ++q) {
     if (r->gexec[q]) hipGraphExecDestroy(r->gexec[q]);
+    if (r->sev[q]) hipEventDestroy(r->sev[q]);
+  }
+  if (r->hsc) hipHostFree(r->hsc);
   for (void* p : r->allocs) hipFree(p);
+  if (r->owns_stream && r->stream) hipStreamDestroy(r->stream);
   delete r;
   return LOMA_OK;
 }
+
+static int autotune_tiles(loma_run* r);
 
 int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, double Re,
               double alp, double bet, double pr, double cfl, int ncfl, double dt0,
               int precision, int flags) {
   if (!out || !y) { set_error("loma_init: null argument"); return LOMA_ERR_ARG; }
   *out = nullptr;
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
   if (mgalx < 8 || mgalz < 8 || my < 16 || my > 1024) {
     set_error("loma_init: grid out of range (my in [16,1024], mgalx,mgalz >= 8)");
     return LOMA_ERR_ARG;
@@ -213,6 +289,7 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   RunConst& c = r->rc;
   c.mgalx = mgalx; c.my = my; c.mgalz = mgalz; c.mx = r->mx; c.mz = r->mz;
   c.nplanes = r->nplanes; c.nz = r->nz;
+  c.myp = (my + 15) / 16 * 16;  // 128-byte aligned y runs in the transform buffers
   c.alp = alp; c.bet = bet; c.ire = 1.0 / Re; c.peclet = Re * pr;
   c.hxalp = alp * (mgalx - 1) / M_PI;  // cross.f:1500
   c.hzbet = bet * (mgalz - 1) / M_PI;
@@ -314,14 +391,16 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
     }
     if (launch_helm_factor(r->dt21, r->dt22, my, M, dsh, r->nclass, r->helm_lu, 0) ||
         launch_helm_bc(M, r->T, r->helm_lu, r->helm_bc, r->nclass, 0) ||
-        hipDeviceSynchronize() != hipSuccess) {
+        hipStreamSynchronize(nullptr) != hipSuccess) {
       set_error("loma_init: factor launch failed"); loma_free(r); return LOMA_ERR_DEVICE;
     }
   }
   // ---- state and work buffers ----
   const size_t S = (size_t)r->nplanes * r->mz * my;
-  const size_t W = (size_t)r->nplanes * mgalz * my;
   const size_t P = (size_t)mgalx * my * mgalz;
+  // transform buffers use the padded y stride myp
+  const size_t Sp = (size_t)r->nplanes * r->mz * c.myp;
+  const size_t Wp = (size_t)r->nplanes * mgalz * c.myp;
   const size_t es = r->fp32 ? 8 : 16;
   double2** st[6] = {&r->vor, &r->phi, &r->psi, &r->scal, &r->ten12, &r->ten13};
   for (auto p : st)
@@ -329,9 +408,9 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   if (r->with_z && (rc = dalloc(r, &r->zf, S * 16))) { loma_free(r); return rc; }
   const int nsp = r->with_z ? 12 : 11;
   for (int q = 0; q < nsp; ++q) {
-    if ((rc = dalloc(r, &r->spec[q], S * es))) { loma_free(r); return rc; }
-    if ((rc = dalloc(r, &r->wbuf[q], W * es))) { loma_free(r); return rc; }
-    if ((rc = dalloc(r, &r->wout[q], W * es))) { loma_free(r); return rc; }
+    if ((rc = dalloc(r, &r->spec[q], Sp * es))) { loma_free(r); return rc; }
+    if ((rc = dalloc(r, &r->wbuf[q], Wp * es))) { loma_free(r); return rc; }
+    if ((rc = dalloc(r, &r->wout[q], Wp * es))) { loma_free(r); return rc; }
   }
   if ((rc = dalloc(r, &r->rhst, P * 8))) { loma_free(r); return rc; }
   if (r->with_z && (rc = dalloc(r, &r->rhsz, P * 8))) { loma_free(r); return rc; }
@@ -413,8 +492,11 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
     }
     r->L = 2 * pairs;
   }
-  if (hipDeviceSynchronize() != hipSuccess) {
+  if (hipStreamSynchronize(nullptr) != hipSuccess) {
     set_error("loma_init: device error during set-up"); loma_free(r); return LOMA_ERR_DEVICE;
+  }
+  if (flags & LOMA_FLAG_AUTOTUNE) {
+    if ((rc = autotune_tiles(r))) { loma_free(r); return rc; }
   }
   *out = r;
   return LOMA_OK;
@@ -432,6 +514,12 @@ int loma_dims(const loma_run* r, long long* out) {
 
 int loma_set_stream(loma_run* r, void* stream) {
   if (!r) return LOMA_ERR_ARG;
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
+  if (r->owns_stream && r->stream && r->stream != (hipStream_t)stream) {
+    hipStreamSynchronize(r->stream);
+    hipStreamDestroy(r->stream);
+  }
+  r->owns_stream = false;
   r->stream = (hipStream_t)stream;
   for (int q = 0; q < 2; ++q)
     if (r->gexec[q]) { hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
@@ -463,6 +551,7 @@ int loma_upload(loma_run* r, const void* vor, const void* phi, const void* psi,
     set_error("loma_upload: null argument");
     return LOMA_ERR_ARG;
   }
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
   const size_t S = (size_t)r->nplanes * r->mz * r->my;
   std::vector<double> buf;
   const void* src[4] = {vor, phi, psi, scal};
@@ -496,6 +585,17 @@ int loma_upload(loma_run* r, const void* vor, const void* phi, const void* psi,
   s.dt = (r->flags & LOMA_FLAG_DT_FP32) ? (double)(float)r->dt0 : r->dt0;
   s.time = 0.0;
   HIPCHK(hipMemcpy(r->sc, &s, sizeof(s), hipMemcpyHostToDevice));
+  // the copies and fills above are on the null stream; the run's own stream
+  // does not wait for it
+  HIPCHK(hipStreamSynchronize(nullptr));
+  // events and pinned scalars of the one-step-in-flight interface, made here
+  // (under the set-up lock) rather than in the stepping loop
+  if (!r->sev[0]) {
+    HIPCHK(hipEventCreate(&r->sev[0]));
+    HIPCHK(hipEventCreate(&r->sev[1]));
+  }
+  if (!r->hsc) HIPCHK(hipHostMalloc((void**)&r->hsc, sizeof(DevScalars), hipHostMallocDefault));
+  r->inflight = false;
   r->istep = 0;
   r->uploaded = true;
   for (int q = 0; q < 2; ++q)
@@ -505,12 +605,14 @@ int loma_upload(loma_run* r, const void* vor, const void* phi, const void* psi,
 
 int loma_upload_z(loma_run* r, const void* z, int fp64) {
   if (!r || !z || !r->with_z) { set_error("loma_upload_z: run has no scalar Z"); return LOMA_ERR_ARG; }
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
   const size_t S = (size_t)r->nplanes * r->mz * r->my;
   std::vector<double> buf;
   host_to_c128(z, fp64, S, buf);
   r->rc.zbot = buf[0];
   r->rc.ztop = buf[2 * (r->my - 1)];
   HIPCHK(hipMemcpy(r->zf, buf.data(), S * 16, hipMemcpyHostToDevice));
+  HIPCHK(hipStreamSynchronize(nullptr));
   for (int q = 0; q < 2; ++q)
     if (r->gexec[q]) { hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
   return LOMA_OK;
@@ -519,6 +621,7 @@ int loma_upload_z(loma_run* r, const void* z, int fp64) {
 int loma_download(loma_run* r, void* vor, void* phi, void* psi, void* scal, double* u00,
                   double* v00, double* w00, int fp64) {
   if (!r || !r->uploaded) { set_error("loma_download: nothing uploaded"); return LOMA_ERR_STATE; }
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
   HIPCHK(hipStreamSynchronize(r->stream));
   const size_t S = (size_t)r->nplanes * r->mz * r->my;
   std::vector<double> buf(2 * S);
@@ -537,6 +640,7 @@ int loma_download(loma_run* r, void* vor, void* phi, void* psi, void* scal, doub
 
 int loma_download_z(loma_run* r, void* z, int fp64) {
   if (!r || !r->with_z || !z) { set_error("loma_download_z: run has no scalar Z"); return LOMA_ERR_ARG; }
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
   HIPCHK(hipStreamSynchronize(r->stream));
   const size_t S = (size_t)r->nplanes * r->mz * r->my;
   std::vector<double> buf(2 * S);
@@ -619,6 +723,80 @@ static int enqueue_substep(loma_run* r, int rkstep, int icfl) {
   return LOMA_OK;
 }
 
+// ---- tile autotuner ------------------------------------------------------
+// Times the transform kernels for the admissible tile shapes of a run's grid
+// (y entries per z-transform workgroup J, line pairs per x-line workgroup)
+// and keeps the fastest; results are cached per (grid, precision, Z).  Runs
+// on the run's own (not yet uploaded) work buffers.
+struct TileKey {
+  int mgalx, my, mgalz, fp32, z;
+  bool operator==(const TileKey& o) const {
+    return mgalx == o.mgalx && my == o.my && mgalz == o.mgalz && fp32 == o.fp32 && z == o.z;
+  }
+};
+static std::mutex g_tile_mu;
+static std::vector<std::pair<TileKey, std::pair<int, int>>> g_tiles;
+
+static int time_launch(loma_run* r, int which, float* best_ms) {
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  float best = 1e30f;
+  for (int rep = 0; rep < 3; ++rep) {
+    FftArgs fa = fft_args(r, 1, 0);
+    HIPCHK(hipEventRecord(a, r->stream));
+    int e = 0;
+    if (which == 0) {
+      fa.nfields = r->with_z ? 12 : 8;
+      e = launch_zfft_inv(r->fp32, fa, r->stream);
+      fa.nfields = r->with_z ? 12 : 11;
+      if (!e) e = launch_zfft_fwd(r->fp32, fa, r->stream);
+    } else {
+      e = launch_xline(r->fp32, fa, r->stream);
+    }
+    if (e) { hipEventDestroy(a); hipEventDestroy(b); return LOMA_ERR_DEVICE; }
+    HIPCHK(hipEventRecord(b, r->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    if (rep > 0) best = std::min(best, ms);  // first launch warms up
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  *best_ms = best;
+  return LOMA_OK;
+}
+
+static int autotune_tiles(loma_run* r) {
+  const TileKey key{r->mgalx, r->my, r->mgalz, r->fp32 ? 1 : 0, r->with_z ? 1 : 0};
+  {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    for (auto& e : g_tiles)
+      if (e.first == key) { r->J = e.second.first; r->L = e.second.second; return LOMA_OK; }
+  }
+  const int J0 = r->J, L0 = r->L;
+  int bestJ = J0, bestL = L0;
+  float best = 1e30f, ms = 0.0f;
+  for (int J = J0; J >= 1 && J >= J0 / 4; J /= 2) {
+    r->J = J;
+    int e = time_launch(r, 0, &ms);
+    if (e) return e;
+    if (ms < best) { best = ms; bestJ = J; }
+  }
+  r->J = bestJ;
+  best = 1e30f;
+  for (int L = L0; L >= 2 && L >= L0 / 4; L /= 2) {
+    r->L = L;
+    int e = time_launch(r, 1, &ms);
+    if (e) return e;
+    if (ms < best) { best = ms; bestL = L; }
+  }
+  r->L = bestL;
+  std::lock_guard<std::mutex> lk(g_tile_mu);
+  g_tiles.push_back({key, {bestJ, bestL}});
+  return LOMA_OK;
+}
+
 static int enqueue_step(loma_run* r, int icfl) {
   for (int rk = 1; rk <= 3; ++rk) {
     int e = enqueue_substep(r, rk, icfl);
@@ -638,9 +816,44 @@ int loma_substep(loma_run* r, int rkstep) {
   return LOMA_OK;
 }
 
+static int ensure_stream(loma_run* r);
+
+// Capture one full step (with or without the CFL pass) into the run's graph.
+static int build_graph(loma_run* r, int icfl) {
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
+  hipGraphExec_t& ge = r->gexec[icfl];
+  if (ge) return LOMA_OK;
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+  int e = enqueue_step(r, icfl);
+  hipError_t ce = hipStreamEndCapture(r->stream, &g);
+  if (e) return e;
+  HIPCHK(ce);
+  hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  HIPCHK(ie);
+  return LOMA_OK;
+}
+
 int loma_use_graph(loma_run* r, int enable) {
   if (!r) return LOMA_ERR_ARG;
   r->use_graph = enable != 0;
+  if (r->use_graph && r->uploaded && !r->prof) {
+    // capture now, so that a later step only launches
+    int e = ensure_stream(r);
+    for (int icfl = 0; icfl < 2 && !e; ++icfl) e = build_graph(r, icfl);
+    if (e) return e;
+  }
+  return LOMA_OK;
+}
+
+// A run without a caller-set stream gets one of its own, non-blocking with
+// respect to the null stream, so that set-up and transfers of other runs on
+// the null stream neither wait for nor delay its steps.
+static int ensure_stream(loma_run* r) {
+  if (r->stream) return LOMA_OK;
+  HIPCHK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+  r->owns_stream = true;
   return LOMA_OK;
 }
 
@@ -649,18 +862,11 @@ static int run_steps(loma_run* r, int nsteps) {
     // cross.f:238: the step size is re-estimated every ncfl steps
     const int icfl = ((r->istep) % r->ncfl == 0) ? 1 : 0;
     if (r->use_graph && !r->prof) {
-      hipGraphExec_t& ge = r->gexec[icfl];
-      if (!ge) {
-        hipGraph_t g;
-        HIPCHK(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
-        int e = enqueue_step(r, icfl);
-        hipError_t ce = hipStreamEndCapture(r->stream, &g);
+      if (!r->gexec[icfl]) {
+        int e = build_graph(r, icfl);
         if (e) return e;
-        HIPCHK(ce);
-        HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-        hipGraphDestroy(g);
       }
-      HIPCHK(hipGraphLaunch(ge, r->stream));
+      HIPCHK(hipGraphLaunch(r->gexec[icfl], r->stream));
     } else {
       int e = enqueue_step(r, icfl);
       if (e) return e;
@@ -673,9 +879,9 @@ static int run_steps(loma_run* r, int nsteps) {
 int loma_advance(loma_run* r, int nsteps) {
   if (!r || !r->uploaded) { set_error("loma_advance: nothing uploaded"); return LOMA_ERR_STATE; }
   if (nsteps < 0) { set_error("loma_advance: nsteps < 0"); return LOMA_ERR_ARG; }
-  if (r->stream == nullptr && r->use_graph) {
-    // graphs need a non-default stream
-    HIPCHK(hipStreamCreate(&r->stream));
+  if (r->use_graph) {  // graphs need a non-default stream
+    int e = ensure_stream(r);
+    if (e) return e;
   }
   int e = run_steps(r, nsteps);
   if (e) return e;
@@ -689,7 +895,10 @@ int loma_advance(loma_run* r, int nsteps) {
 int loma_advance_async(loma_run* r, int nsteps) {
   if (!r || !r->uploaded) { set_error("loma_advance_async: nothing uploaded"); return LOMA_ERR_STATE; }
   if (nsteps < 0) { set_error("loma_advance_async: nsteps < 0"); return LOMA_ERR_ARG; }
-  if (r->stream == nullptr && r->use_graph) HIPCHK(hipStreamCreate(&r->stream));
+  if (r->use_graph) {
+    int e = ensure_stream(r);
+    if (e) return e;
+  }
   return run_steps(r, nsteps);
 }
 
@@ -730,15 +939,126 @@ int loma_stream(loma_run* r, void** stream) {
 }
 
 int loma_advance_many(loma_run** runs, int nruns, int nsteps) {
-  if (!runs || nruns <= 0) return LOMA_ERR_ARG;
-  for (int n = 0; n < nsteps; ++n)
-    for (int q = 0; q < nruns; ++q) {
-      loma_run* r = runs[q];
-      if (!r || !r->uploaded) { set_error("loma_advance_many: run not uploaded"); return LOMA_ERR_STATE; }
-      int e = run_steps(r, 1);
-      if (e) return e;
+  if (!runs || nruns <= 0 || nsteps < 0) return LOMA_ERR_ARG;
+  bool graph = true;
+  for (int q = 0; q < nruns; ++q) {
+    loma_run* r = runs[q];
+    if (!r || !r->uploaded) { set_error("loma_advance_many: run not uploaded"); return LOMA_ERR_STATE; }
+    for (int p = 0; p < q; ++p)
+      if (runs[p] == r) { set_error("loma_advance_many: run listed twice"); return LOMA_ERR_ARG; }
+    int e = ensure_stream(r);  // every run needs a stream of its own
+    if (e) return e;
+    graph = graph && r->use_graph && !r->prof;
+  }
+  for (int n = 0; n < nsteps; ++n) {
+    std::vector<int> icfl(nruns);
+    for (int q = 0; q < nruns; ++q) icfl[q] = (runs[q]->istep % runs[q]->ncfl == 0) ? 1 : 0;
+    if (!graph) {
+      // independent streams: the runs still overlap on the device
+      for (int q = 0; q < nruns; ++q) {
+        int e = enqueue_step(runs[q], icfl[q]);
+        if (e) return e;
+      }
+    } else {
+      // lock order everywhere: set-up, then ensembles
+      std::lock_guard<std::recursive_mutex> lk2(g_setup_mu);
+      std::lock_guard<std::mutex> lk(g_ens_mu);
+      EnsembleGraph* eg = nullptr;
+      for (auto& c : g_ens)
+        if ((int)c.runs.size() == nruns && c.icfl == icfl &&
+            std::equal(c.runs.begin(), c.runs.end(), runs))
+          eg = &c;
+      if (!eg) {
+        EnsembleGraph c;
+        c.runs.assign(runs, runs + nruns);
+        c.icfl = icfl;
+        HIPCHK(hipStreamCreate(&c.master));
+        hipEvent_t fork;
+        std::vector<hipEvent_t> join(nruns);
+        HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        for (auto& e : join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPCHK(hipStreamBeginCapture(c.master, hipStreamCaptureModeThreadLocal));
+        int err = hipEventRecord(fork, c.master);
+        for (int q = 0; q < nruns && !err; ++q) {
+          err = hipStreamWaitEvent(runs[q]->stream, fork, 0);
+          if (!err) err = enqueue_step(runs[q], icfl[q]);
+          if (!err) err = hipEventRecord(join[q], runs[q]->stream);
+          if (!err) err = hipStreamWaitEvent(c.master, join[q], 0);
+        }
+        hipGraph_t g;
+        hipError_t ce = hipStreamEndCapture(c.master, &g);
+        hipEventDestroy(fork);
+        for (auto& e : join) hipEventDestroy(e);
+        if (err) { hipStreamDestroy(c.master); set_error("loma_advance_many: capture failed"); return LOMA_ERR_DEVICE; }
+        HIPCHK(ce);
+        hipError_t ie = hipGraphInstantiate(&c.exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        HIPCHK(ie);
+        g_ens.push_back(c);
+        eg = &g_ens.back();
+      }
+      HIPCHK(hipGraphLaunch(eg->exec, eg->master));
     }
-  for (int q = 0; q < nruns; ++q) HIPCHK(hipStreamSynchronize(runs[q]->stream));
+    for (int q = 0; q < nruns; ++q) runs[q]->istep++;
+  }
+  // wait for every run (graph launches complete on their master streams)
+  HIPCHK(hipDeviceSynchronize());
+  for (int q = 0; q < nruns; ++q) {
+    DevScalars s;
+    HIPCHK(hipMemcpy(&s, runs[q]->sc, sizeof(s), hipMemcpyDeviceToHost));
+    if (s.nan) { set_error("loma_advance_many: non-finite state in a run"); return LOMA_ERR_NAN; }
+  }
+  return LOMA_OK;
+}
+
+int loma_tiles(const loma_run* r, int* J, int* L) {
+  if (!r || !J || !L) return LOMA_ERR_ARG;
+  *J = r->J;
+  *L = r->L;
+  return LOMA_OK;
+}
+
+int loma_footprint(int mgalx, int my, int mgalz, int precision, int flags,
+                   long long* bytes) {
+  if (!bytes || mgalx < 8 || mgalz < 8 || my < 16 || my > 1024 || pick_M(my) < 0 ||
+      (precision != LOMA_FP64 && precision != LOMA_MIXED)) {
+    set_error("loma_footprint: grid or precision out of range");
+    return LOMA_ERR_ARG;
+  }
+  *bytes = (long long)plan_bytes(mgalx, my, mgalz, precision == LOMA_MIXED,
+                                 (flags & LOMA_FLAG_SCALAR_Z) != 0);
+  return LOMA_OK;
+}
+
+int loma_step_begin(loma_run* r) {
+  if (!r || !r->uploaded) { set_error("loma_step_begin: nothing uploaded"); return LOMA_ERR_STATE; }
+  if (r->inflight) { set_error("loma_step_begin: a step is in flight"); return LOMA_ERR_STATE; }
+  if (!r->stream) {
+    std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
+    int e = ensure_stream(r);
+    if (e) return e;
+  }
+  HIPCHK(hipEventRecord(r->sev[0], r->stream));
+  int e = run_steps(r, 1);
+  if (e) return e;
+  HIPCHK(hipMemcpyAsync(r->hsc, r->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, r->stream));
+  HIPCHK(hipEventRecord(r->sev[1], r->stream));
+  r->inflight = true;
+  return LOMA_OK;
+}
+
+int loma_step_poll(loma_run* r, int* done, double* device_ms, int* nonfinite) {
+  if (!r || !done) return LOMA_ERR_ARG;
+  if (!r->inflight) { set_error("loma_step_poll: no step in flight"); return LOMA_ERR_STATE; }
+  const hipError_t q = hipEventQuery(r->sev[1]);
+  if (q == hipErrorNotReady) { *done = 0; return LOMA_OK; }
+  HIPCHK(q);
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, r->sev[0], r->sev[1]));
+  *done = 1;
+  if (device_ms) *device_ms = ms;
+  if (nonfinite) *nonfinite = r->hsc->nan;
+  r->inflight = false;
   return LOMA_OK;
 }
 

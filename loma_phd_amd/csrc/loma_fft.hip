@@ -123,7 +123,7 @@ __device__ __forceinline__ void dft5(T2* v, int s) {
 // LDS layout of FFT work buffers: one pad entry after every 8, so that the
 // stride-R butterfly writes of the first passes spread over the banks.
 #ifndef LOMA_LDS_PAD
-#define LOMA_LDS_PAD 1
+#define LOMA_LDS_PAD 0
 #endif
 // waves per SIMD the x-line and z-transform kernels are compiled for
 #ifndef LOMA_XL_WPE
@@ -260,17 +260,120 @@ __device__ void fft_rows_ip(T2* a, int N, int nrows, const FftPlan& pl,
   }
 }
 
+// Compile-time plans for the production lengths: the same radix sequence as
+// make_plan (8s, then 4, 2, 3, 5), with every stride, index split and
+// twiddle step a constant, so the index arithmetic folds into shifts and
+// multiplies.  Results are identical to the run-time plan.
+constexpr int ct_radix(int m) {
+  return m % 8 == 0 ? 8 : (m % 4 == 0 ? 4 : (m % 2 == 0 ? 2 : (m % 3 == 0 ? 3 : 5)));
+}
+
+template <typename T, typename T2, int N, int Ns, int NT>
+__device__ __forceinline__ T2* fft_rows_ct(T2* a, T2* b, int nrows, const T2* __restrict__ tw,
+                                           int s) {
+  if constexpr (Ns >= N) {
+    return a;
+  } else {
+    constexpr int R = ct_radix(N / Ns);
+    constexpr int nb = N / R;
+    constexpr int tstep = N / (Ns * R);
+    // the pass's index arithmetic starts here, not hoisted into earlier
+    // passes where it would hold registers across them
+    int t0 = threadIdx.x;
+    asm volatile("" : "+v"(t0));
+    const int total = nrows * nb;
+    for (int it = t0; it < total; it += NT) {
+      const int row = it / nb, j = it - row * nb;
+      stockham_item<R, T, T2>(a, b, row * N, j, nb, Ns, tstep, tw, s);
+    }
+    __syncthreads();
+    return fft_rows_ct<T, T2, N, Ns * R, NT>(b, a, nrows, tw, s);
+  }
+}
+
+template <typename T, typename T2, int E, int N, int Ns, int NT>
+__device__ __forceinline__ void fft_rows_ip_ct(T2* a, int nrows, const T2* __restrict__ tw,
+                                               int s, T2* v) {
+  if constexpr (Ns < N) {
+    constexpr int R = ct_radix(N / Ns);
+    constexpr int nb = N / R;
+    constexpr int tstep = N / (Ns * R);
+    constexpr int nitems = (E + R - 1) / R;
+    const int total = nrows * nb;
+    int t0 = threadIdx.x;
+    asm volatile("" : "+v"(t0));
+#pragma unroll
+    for (int k = 0; k < nitems; ++k) {
+      const int it = t0 + k * NT;
+      if (it < total) {
+        const int row = it / nb, j = it - row * nb;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[k * R + r] = a[lds_i(row * N + j + r * nb)];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < nitems; ++k) {
+      const int it = t0 + k * NT;
+      if (it < total) {
+        const int row = it / nb, j = it - row * nb;
+        const int jm = j % Ns;
+        T2* w = v + k * R;
+        if (Ns > 1) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            T2 t = tw[jm * r * tstep];
+            if (s > 0) t.y = -t.y;
+            w[r] = cmul(w[r], t);
+          }
+        }
+        if (R == 2) dft2<T, T2>(w);
+        if (R == 3) dft3<T, T2>(w, s);
+        if (R == 4) dft4<T, T2>(w, s);
+        if (R == 5) dft5<T, T2>(w, s);
+        if (R == 8) dft8<T, T2>(w, s);
+        const int idxD = row * N + (j / Ns) * Ns * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[lds_i(idxD + r * Ns)] = w[r];
+      }
+    }
+    __syncthreads();
+    fft_rows_ip_ct<T, T2, E, N, Ns * R, NT>(a, nrows, tw, s, v);
+  }
+}
+
+// NN > 0: compile-time length NN; NN = 0: the run-time plan
+template <typename T, typename T2, int NN, int NT>
+__device__ __forceinline__ T2* fft_any(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
+                                       const T2* __restrict__ tw, int s) {
+  if constexpr (NN > 0)
+    return fft_rows_ct<T, T2, NN, 1, NT>(a, b, nrows, tw, s);
+  else
+    return fft_rows<T, T2>(a, b, N, nrows, pl, tw, s);
+}
+template <typename T, typename T2, int E, int NN, int NT>
+__device__ __forceinline__ void fft_any_ip(T2* a, int N, int nrows, const FftPlan& pl,
+                                           const T2* __restrict__ tw, int s) {
+  if constexpr (NN > 0) {
+    T2 v[E + 8];
+    fft_rows_ip_ct<T, T2, E, NN, 1, NT>(a, nrows, tw, s, v);
+  } else {
+    fft_rows_ip<T, T2, E>(a, N, nrows, pl, tw, s);
+  }
+}
+
 // ------------------------------------------------------------------------
 // z transforms: spectral [i][k][j] <-> z-physical [i][z][j]
 // (localyz2zy + fourz(+1), fou3D.f:103-146 / fourz(-1) + localzy2yz)
 // ------------------------------------------------------------------------
 // One workgroup: J consecutive y entries of one kx plane of one field, all
 // z; one LDS buffer, in-place FFT (J * mgalz <= 256 * E).
-template <typename T, int E>
+template <typename T, int E, int NN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE))) void k_zfft_inv(FftArgs a) {
   using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int N = a.rc.mgalz, J = a.J, my = a.rc.my, mz = a.rc.mz, nz = a.rc.nz;
+  const int N = NN ? NN : a.rc.mgalz, J = a.J, my = a.rc.my, myp = a.rc.myp, mz = a.rc.mz,
+            nz = a.rc.nz;
   T2* A = (T2*)smem;
   const int nyb = (my + J - 1) / J;
   const int i = blockIdx.x / nyb, j0 = (blockIdx.x - i * nyb) * J;
@@ -287,22 +390,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
     const int k = it / J, jj = it - k * J, j = j0 + jj;
     const int z = (k <= nz) ? k : k + N - mz;
     T2 v = T2{0, 0};
-    if (j < my) v = src[((size_t)i * mz + k) * my + j];
+    if (j < my) v = src[((size_t)i * mz + k) * myp + j];
     A[lds_i(jj * N + z)] = v;
   }
   __syncthreads();
-  fft_rows_ip<T, T2, E>(A, N, J, a.plz, (const T2*)a.twz, +1);
+  fft_any_ip<T, T2, E, NN, 256>(A, N, J, a.plz, (const T2*)a.twz, +1);
   for (int it = threadIdx.x; it < N * J; it += blockDim.x) {
     const int z = it / J, jj = it - z * J, j = j0 + jj;
-    if (j < my) dst[((size_t)i * N + z) * my + j] = A[lds_i(jj * N + z)];
+    if (j < my) dst[((size_t)i * N + z) * myp + j] = A[lds_i(jj * N + z)];
   }
 }
 
-template <typename T, int E>
+template <typename T, int E, int NN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE))) void k_zfft_fwd(FftArgs a) {
   using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int N = a.rc.mgalz, J = a.J, my = a.rc.my, mz = a.rc.mz, nz = a.rc.nz;
+  const int N = NN ? NN : a.rc.mgalz, J = a.J, my = a.rc.my, myp = a.rc.myp, mz = a.rc.mz,
+            nz = a.rc.nz;
   T2* A = (T2*)smem;
   const int nyb = (my + J - 1) / J;
   const int i = blockIdx.x / nyb, j0 = (blockIdx.x - i * nyb) * J;
@@ -312,11 +416,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
   for (int it = threadIdx.x; it < N * J; it += blockDim.x) {
     const int z = it / J, jj = it - z * J, j = j0 + jj;
     T2 v = T2{0, 0};
-    if (j < my) v = src[((size_t)i * N + z) * my + j];
+    if (j < my) v = src[((size_t)i * N + z) * myp + j];
     A[lds_i(jj * N + z)] = v;
   }
   __syncthreads();
-  fft_rows_ip<T, T2, E>(A, N, J, a.plz, (const T2*)a.twz, -1);
+  fft_any_ip<T, T2, E, NN, 256>(A, N, J, a.plz, (const T2*)a.twz, -1);
   const T sc = (T)(1.0 / (double)N);
   for (int it = threadIdx.x; it < mz * J; it += blockDim.x) {
     const int k = it / J, jj = it - k * J, j = j0 + jj;
@@ -325,7 +429,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
       T2 v = A[lds_i(jj * N + z)];
       v.x *= sc;
       v.y *= sc;
-      dst[((size_t)i * mz + k) * my + j] = v;
+      dst[((size_t)i * mz + k) * myp + j] = v;
     }
   }
 }
@@ -346,7 +450,12 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double 
   atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
-constexpr int XL_THREADS = 256;
+// 256 threads per x-line workgroup (512 measured slower on MI355X: more
+// spills at the 128-VGPR budget that 8 waves per SIMD pair impose)
+#ifndef LOMA_XL_THREADS
+#define LOMA_XL_THREADS 256
+#endif
+constexpr int XL_THREADS = LOMA_XL_THREADS;
 
 // Workgroups are dealt to the 8 XCDs round-robin; this maps the launch so
 // that tiles t, t+1, ... (neighbouring y runs that share cache lines) run on
@@ -360,10 +469,10 @@ __host__ inline int xcd_groups(int n) { return 8 * ((n + 7) / 8); }
 // spectral lines of field `src` (W layout [i][z][j]) -> Hermitian-packed
 // complex rows in A (row p = lines j0+2p, j0+2p+1), then inverse FFT.
 // Returns the buffer holding the physical pairs.
-template <typename T, typename T2, int EPT>
+template <typename T, typename T2, int EPT, int NN>
 __device__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src, T2* A, T2* B, int z,
                           int j0, int pairs) {
-  const int N = a.rc.mgalx, np = a.rc.nplanes, my = a.rc.my, mgz = a.rc.mgalz;
+  const int N = NN ? NN : a.rc.mgalx, np = a.rc.nplanes, my = a.rc.my, mgz = a.rc.mgalz;
   // the dealiased band np..N-np is zero
   const int nzero = N - 2 * np + 1;
   for (int it = threadIdx.x; it < pairs * nzero; it += blockDim.x) {
@@ -373,7 +482,7 @@ __device__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src, T2* A, T
   for (int it = threadIdx.x; it < np * pairs; it += blockDim.x) {
     const int ii = it / pairs, p = it - ii * pairs;
     const int ja = j0 + 2 * p, jb = ja + 1;
-    const size_t base = ((size_t)ii * mgz + z) * my;
+    const size_t base = ((size_t)ii * mgz + z) * a.rc.myp;
     T2 ca = T2{0, 0}, cb = T2{0, 0};
     if (ja < my) ca = src[base + ja];
     if (jb < my) cb = src[base + jb];
@@ -387,17 +496,17 @@ __device__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src, T2* A, T
     }
   }
   __syncthreads();
-  return fft_rows<T, T2>(A, B, N, pairs, a.plx, (const T2*)a.twx, +1);
+  return fft_any<T, T2, NN, XL_THREADS>(A, B, N, pairs, a.plx, (const T2*)a.twx, +1);
 }
 
 // forward FFT of A (filled by the caller, then synchronised here) and
 // Hermitian unpacking of the pairs into field `dst` (scaled by 1/N)
-template <typename T, typename T2, int EPT>
+template <typename T, typename T2, int EPT, int NN>
 __device__ void xl_forward(const FftArgs& a, T2* __restrict__ dst, T2* A, T2* B, int z,
                            int j0, int pairs) {
-  const int N = a.rc.mgalx, np = a.rc.nplanes, my = a.rc.my, mgz = a.rc.mgalz;
+  const int N = NN ? NN : a.rc.mgalx, np = a.rc.nplanes, my = a.rc.my, mgz = a.rc.mgalz;
   __syncthreads();
-  const T2* R = fft_rows<T, T2>(A, B, N, pairs, a.plx, (const T2*)a.twx, -1);
+  const T2* R = fft_any<T, T2, NN, XL_THREADS>(A, B, N, pairs, a.plx, (const T2*)a.twx, -1);
   const T hs = (T)(0.5 / (double)N);
   for (int it = threadIdx.x; it < np * pairs; it += blockDim.x) {
     const int ii = it / pairs, p = it - ii * pairs;
@@ -409,7 +518,7 @@ __device__ void xl_forward(const FftArgs& a, T2* __restrict__ dst, T2* A, T2* B,
     T2 ya{(zk.x + zc.x) * hs, (zk.y + zc.y) * hs};
     T2 yb{(zk.y - zc.y) * hs, -(zk.x - zc.x) * hs};  // (zk - zc) / (2i)
     if (ii == 0) { ya.y = 0; yb.y = 0; }
-    const size_t base = ((size_t)ii * mgz + z) * my;
+    const size_t base = ((size_t)ii * mgz + z) * a.rc.myp;
     if (ja < my) dst[base + ja] = ya;
     if (jb < my) dst[base + jb] = yb;
   }
@@ -450,13 +559,16 @@ __device__ __forceinline__ XlRk xl_rk(double dt, int rkstep) {
   return XlRk{dt * xis[rk], dt * gam[rk], dt * betas[rk], alphas[rk] / betas[rk]};
 }
 
-template <typename T, bool WZ, int EPT>
-__global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(LOMA_XL_WPE))) void k_xline(FftArgs a) {
+// FP64 is compiled for two waves per SIMD: at three its register demand
+// spills heavily, and the Z-transport build gave wrong results at 3.
+template <typename T, bool WZ, int EPT, int NN>
+__global__ __launch_bounds__(XL_THREADS)
+__attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 : LOMA_XL_WPE)))) void k_xline(FftArgs a) {
   using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double gln[12][XL_MAXL];   // output estimates per line
   const RunConst& rc = a.rc;
-  const int N = rc.mgalx, my = rc.my;
+  const int N = NN ? NN : rc.mgalx, my = rc.my;
   const int pairs = a.L / 2;            // a power of two dividing XL_THREADS
   const int nyb = (my + a.L - 1) / a.L;
   const int tile = xcd_tile(blockIdx.x, rc.mgalz * nyb);
@@ -534,10 +646,10 @@ __global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(LOMA
 
   // ---- pass 1: T, rho u_i -> u_i, rho u_i u_j; CFL maxima ----
   T tt[EPT][2], ru[EPT][2], rv[EPT][2], rw[EPT][2];  // perturbations
-  grab(xl_inverse<T, T2, EPT>(a, wb[7], A, B, z, j0, pairs), tt);
-  grab(xl_inverse<T, T2, EPT>(a, wb[0], A, B, z, j0, pairs), ru);
-  grab(xl_inverse<T, T2, EPT>(a, wb[1], A, B, z, j0, pairs), rv);
-  grab(xl_inverse<T, T2, EPT>(a, wb[2], A, B, z, j0, pairs), rw);
+  grab(xl_inverse<T, T2, EPT, NN>(a, wb[7], A, B, z, j0, pairs), tt);
+  grab(xl_inverse<T, T2, EPT, NN>(a, wb[0], A, B, z, j0, pairs), ru);
+  grab(xl_inverse<T, T2, EPT, NN>(a, wb[1], A, B, z, j0, pairs), rv);
+  grab(xl_inverse<T, T2, EPT, NN>(a, wb[2], A, B, z, j0, pairs), rw);
   // profiles of rho u, rho v, rho w, T for the two lines
   double pu[2], pv[2], pw[2], pT[2];
 #pragma unroll
@@ -611,7 +723,7 @@ __global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(LOMA
       }
       A[lds_i(p * N + x)] = T2{val[0], val[1]};
     }
-    xl_forward<T, T2, EPT>(a, wo[q], A, B, z, j0, pairs);
+    xl_forward<T, T2, EPT, NN>(a, wo[q], A, B, z, j0, pairs);
   }
 
   // ---- pass 2: r = T (lap T / Pe - rho u . grad T)  (cross.f:1652-1673) ----
@@ -619,13 +731,13 @@ __global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(LOMA
   T acc[EPT][2], accz[EPT][2];
   {
     T f[EPT][2];
-    grab(xl_inverse<T, T2, EPT>(a, wb[3], A, B, z, j0, pairs), f);  // lap T
+    grab(xl_inverse<T, T2, EPT, NN>(a, wb[3], A, B, z, j0, pairs), f);  // lap T
 #pragma unroll
     for (int m = 0; m < EPT; ++m)
 #pragma unroll
       for (int h = 0; h < 2; ++h) acc[m][h] = f[m][h] * (T)ipe;
     if constexpr (WZ) {
-      grab(xl_inverse<T, T2, EPT>(a, wb[11], A, B, z, j0, pairs), f);  // lap Z
+      grab(xl_inverse<T, T2, EPT, NN>(a, wb[11], A, B, z, j0, pairs), f);  // lap Z
 #pragma unroll
       for (int m = 0; m < EPT; ++m)
 #pragma unroll
@@ -638,7 +750,7 @@ __global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(LOMA
       opaque_regs(ru);
       opaque_regs(rv);
       opaque_regs(rw);
-      grab(xl_inverse<T, T2, EPT>(a, wb[4 + c], A, B, z, j0, pairs), f);
+      grab(xl_inverse<T, T2, EPT, NN>(a, wb[4 + c], A, B, z, j0, pairs), f);
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const T R_ = (T)(c == 0 ? pu[h] : (c == 1 ? pv[h] : pw[h])), G_ = (T)prof(h, 4 + c);
@@ -649,7 +761,7 @@ __global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(LOMA
         }
       }
       if constexpr (WZ) {
-        grab(xl_inverse<T, T2, EPT>(a, wb[8 + c], A, B, z, j0, pairs), f);
+        grab(xl_inverse<T, T2, EPT, NN>(a, wb[8 + c], A, B, z, j0, pairs), f);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const T R_ = (T)(c == 0 ? pu[h] : (c == 1 ? pv[h] : pw[h])), G_ = (T)prof(h, 8 + c);
@@ -698,12 +810,12 @@ __global__ __launch_bounds__(XL_THREADS) __attribute__((amdgpu_waves_per_eu(LOMA
   }
   // outputs 9 drho, 10 DT, 11 DZ
   put(rv);
-  xl_forward<T, T2, EPT>(a, wo[9], A, B, z, j0, pairs);
+  xl_forward<T, T2, EPT, NN>(a, wo[9], A, B, z, j0, pairs);
   put(ru);
-  xl_forward<T, T2, EPT>(a, wo[10], A, B, z, j0, pairs);
+  xl_forward<T, T2, EPT, NN>(a, wo[10], A, B, z, j0, pairs);
   if constexpr (WZ) {
     put(rw);
-    xl_forward<T, T2, EPT>(a, wo[11], A, B, z, j0, pairs);
+    xl_forward<T, T2, EPT, NN>(a, wo[11], A, B, z, j0, pairs);
   }
 }
 
@@ -738,15 +850,30 @@ static size_t xline_lds(bool fp32, const FftArgs& a) {
   return (size_t)2 * lds_entries((a.L / 2) * a.rc.mgalx) * (fp32 ? 8 : 16);
 }
 
+// kernel instance for a transform length: compile-time plans for the
+// production lengths, the run-time plan otherwise
+#define LOMA_BY_LENGTH(n, LAUNCH)              \
+  switch (n) {                                 \
+    case 64: { constexpr int NN = 64; LAUNCH; break; }   \
+    case 96: { constexpr int NN = 96; LAUNCH; break; }   \
+    case 128: { constexpr int NN = 128; LAUNCH; break; } \
+    case 192: { constexpr int NN = 192; LAUNCH; break; } \
+    case 256: { constexpr int NN = 256; LAUNCH; break; } \
+    case 384: { constexpr int NN = 384; LAUNCH; break; } \
+    case 512: { constexpr int NN = 512; LAUNCH; break; } \
+    default: { constexpr int NN = 0; LAUNCH; break; }    \
+  }
+
 int launch_zfft_inv(bool fp32, const FftArgs& a, hipStream_t st) {
   const int nyb = (a.rc.my + a.J - 1) / a.J;
   dim3 g(a.rc.nplanes * nyb, a.nfields), b(256);
   const size_t lds = zfft_lds(fp32, a);
   // J * mgalz <= 256 * E: E = 16 in FP32, 8 in FP64 (checked at init)
-  if (fp32)
-    hipLaunchKernelGGL((k_zfft_inv<float, 16>), g, b, lds, st, a);
-  else
-    hipLaunchKernelGGL((k_zfft_inv<double, 8>), g, b, lds, st, a);
+  if (fp32) {
+    LOMA_BY_LENGTH(a.rc.mgalz, hipLaunchKernelGGL((k_zfft_inv<float, 16, NN>), g, b, lds, st, a));
+  } else {
+    LOMA_BY_LENGTH(a.rc.mgalz, hipLaunchKernelGGL((k_zfft_inv<double, 8, NN>), g, b, lds, st, a));
+  }
   return (int)hipGetLastError();
 }
 
@@ -754,10 +881,11 @@ int launch_zfft_fwd(bool fp32, const FftArgs& a, hipStream_t st) {
   const int nyb = (a.rc.my + a.J - 1) / a.J;
   dim3 g(a.rc.nplanes * nyb, a.nfields), b(256);
   const size_t lds = zfft_lds(fp32, a);
-  if (fp32)
-    hipLaunchKernelGGL((k_zfft_fwd<float, 16>), g, b, lds, st, a);
-  else
-    hipLaunchKernelGGL((k_zfft_fwd<double, 8>), g, b, lds, st, a);
+  if (fp32) {
+    LOMA_BY_LENGTH(a.rc.mgalz, hipLaunchKernelGGL((k_zfft_fwd<float, 16, NN>), g, b, lds, st, a));
+  } else {
+    LOMA_BY_LENGTH(a.rc.mgalz, hipLaunchKernelGGL((k_zfft_fwd<double, 8, NN>), g, b, lds, st, a));
+  }
   return (int)hipGetLastError();
 }
 
@@ -765,13 +893,21 @@ int launch_xline(bool fp32, const FftArgs& a, hipStream_t st) {
   const int nyb = (a.rc.my + a.L - 1) / a.L;
   dim3 g(xcd_groups(a.rc.mgalz * nyb)), b(XL_THREADS);
   const size_t lds = xline_lds(fp32, a);
-  // entries per thread: pairs * mgalx <= XL_THREADS * EPT (checked at init)
+  // entries per thread: pairs * mgalx <= XL_THREADS * EPT, i.e. at most
+  // 1024 (FP64) / 2048 (FP32) entries per workgroup (checked at init)
+  constexpr int E64 = 1024 / XL_THREADS, E32 = 2048 / XL_THREADS;
   if (fp32) {
-    if (a.with_z) hipLaunchKernelGGL((k_xline<float, true, 8>), g, b, lds, st, a);
-    else hipLaunchKernelGGL((k_xline<float, false, 8>), g, b, lds, st, a);
+    if (a.with_z) {
+      LOMA_BY_LENGTH(a.rc.mgalx, hipLaunchKernelGGL((k_xline<float, true, E32, NN>), g, b, lds, st, a));
+    } else {
+      LOMA_BY_LENGTH(a.rc.mgalx, hipLaunchKernelGGL((k_xline<float, false, E32, NN>), g, b, lds, st, a));
+    }
   } else {
-    if (a.with_z) hipLaunchKernelGGL((k_xline<double, true, 4>), g, b, lds, st, a);
-    else hipLaunchKernelGGL((k_xline<double, false, 4>), g, b, lds, st, a);
+    if (a.with_z) {
+      LOMA_BY_LENGTH(a.rc.mgalx, hipLaunchKernelGGL((k_xline<double, true, E64, NN>), g, b, lds, st, a));
+    } else {
+      LOMA_BY_LENGTH(a.rc.mgalx, hipLaunchKernelGGL((k_xline<double, false, E64, NN>), g, b, lds, st, a));
+    }
   }
   return (int)hipGetLastError();
 }

@@ -28,6 +28,7 @@ struct DevScalars {
 // per-run physics constants passed by value to kernels
 struct RunConst {
   int mgalx, my, mgalz, mx, mz, nplanes, nz;
+  int myp;  // y stride of the transform buffers (my rounded up to 16)
   double alp, bet, ire, peclet;
   double hxalp, hzbet;   // 2/dx and 2/dz as in phys2 (cross.f:1500-1501)
   double rhobot, rhotop, zbot, ztop;

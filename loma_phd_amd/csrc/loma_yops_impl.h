@@ -305,7 +305,7 @@ __device__ __forceinline__ void add_mean(double (&x)[2][M], int lane, int my, bo
 // scalar-field block of phase A: BCs, then f, i kx f, df/dy, i kz f, lap f
 template <int M, typename T2>
 __device__ __forceinline__ void scalar_derivs(const PhaseArgs& a, int lane, double2* st,
-                                              size_t o, bool m00, double kx, double kz,
+                                              size_t o, size_t os, bool m00, double kx, double kz,
                                               double rK, double2* fld, double bot, double top,
                                               T2* const* sp, int q_f, int q_dx, int q_dy,
                                               int q_dz, int q_lap) {
@@ -322,20 +322,20 @@ __device__ __forceinline__ void scalar_derivs(const PhaseArgs& a, int lane, doub
       fld[o + r] = make_double2(F[0][t], 0.0);
     }
   }
-  if (q_f >= 0) store_spec<M>(sp[q_f] + o, my, lane, st, F, m00, mi + (size_t)q_f * my);
+  if (q_f >= 0) store_spec<M>(sp[q_f] + os, my, lane, st, F, m00, mi + (size_t)q_f * my);
   mul_i<M>(F, kx, d);
-  store_spec<M>(sp[q_dx] + o, my, lane, st, d, m00, mi + (size_t)q_dx * my);
+  store_spec<M>(sp[q_dx] + os, my, lane, st, d, m00, mi + (size_t)q_dx * my);
   mul_i<M>(F, kz, d);
-  store_spec<M>(sp[q_dz] + o, my, lane, st, d, m00, mi + (size_t)q_dz * my);
+  store_spec<M>(sp[q_dz] + os, my, lane, st, d, m00, mi + (size_t)q_dz * my);
   op_dy1<M, 2>(a.T, lane, F, d);
-  store_spec<M>(sp[q_dy] + o, my, lane, st, d, m00, mi + (size_t)q_dy * my);
+  store_spec<M>(sp[q_dy] + os, my, lane, st, d, m00, mi + (size_t)q_dy * my);
   op_dy2<M, 2>(a.T, lane, F, d);
 #pragma unroll
   for (int t = 0; t < M; ++t) {
     d[0][t] = d[0][t] - rK * F[0][t];
     d[1][t] = d[1][t] - rK * F[1][t];
   }
-  store_spec<M>(sp[q_lap] + o, my, lane, st, d, m00, mi + (size_t)q_lap * my);
+  store_spec<M>(sp[q_lap] + os, my, lane, st, d, m00, mi + (size_t)q_lap * my);
 }
 
 template <int M, typename TS>
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256, 2) void k_phaseA(PhaseArgs a) {
   const double rK = kx * kx + kz * kz;
   const double dt = a.sc->dt;
   const double dtxi = dt * c_xi[a.rkstep - 1];
-  const size_t o = (size_t)w * my;
+  const size_t o = (size_t)w * my, os = (size_t)w * rc.myp;
   const double* lu = a.helm_lu + (size_t)helm_class(rc, i, k) * a.lu_stride;
   const bool m00 = (i == 0 && k == 0);
   T2* const* sp = (T2* const*)a.spec;
@@ -432,8 +432,8 @@ __global__ __launch_bounds__(256, 2) void k_phaseA(PhaseArgs a) {
         }
       }
     }
-    store_spec<M>(sp[0] + o, my, lane, st, mx, m00, a.mean_in);
-    store_spec<M>(sp[2] + o, my, lane, st, mz, m00, a.mean_in + 2 * (size_t)my);
+    store_spec<M>(sp[0] + os, my, lane, st, mx, m00, a.mean_in);
+    store_spec<M>(sp[2] + os, my, lane, st, mz, m00, a.mean_in + 2 * (size_t)my);
     op_dy1<M, 2>(a.T, lane, ps, dv);
     // rho v = m_y + d psi/dy
     stage_add<M>(pk, lane, dv, 1.0);
@@ -446,14 +446,14 @@ __global__ __launch_bounds__(256, 2) void k_phaseA(PhaseArgs a) {
         pk[r] = make_double2(0.0, 0.0);
       }
     }
-    stage_out<M>(pk, my, lane, sp[1] + o);
+    stage_out<M>(pk, my, lane, sp[1] + os);
   }
   // (4) T BCs and derivatives                           cross.f:338-349, 521-554
-  scalar_derivs<M, T2>(a, lane, st, o, m00, kx, kz, rK, a.scal, 1.0 / rc.rhobot,
+  scalar_derivs<M, T2>(a, lane, st, o, os, m00, kx, kz, rK, a.scal, 1.0 / rc.rhobot,
                        1.0 / rc.rhotop, sp, 7, 4, 5, 6, 3);
   if (a.with_z) {
     // passive scalar: BCs and derivatives (lomaHZB cross.f:343-355, 626-666)
-    scalar_derivs<M, T2>(a, lane, st, o, m00, kx, kz, rK, a.zf, rc.zbot, rc.ztop, sp, -1, 8,
+    scalar_derivs<M, T2>(a, lane, st, o, os, m00, kx, kz, rK, a.zf, rc.zbot, rc.ztop, sp, -1, 8,
                          9, 10, 11);
   }
 }
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(256, 2) void k_phaseC(PhaseArgs a) {
   //             9 drho, 10 DT, 11 DZ
   const T2* sp[12];
 #pragma unroll
-  for (int q = 0; q < 12; ++q) sp[q] = (const T2*)a.spec[q] + o;
+  for (int q = 0; q < 12; ++q) sp[q] = (const T2*)a.spec[q] + (size_t)w * rc.myp;
 
   // (6a) T += DT, Z += DZ                               cross.f:619-627
   {
@@ -690,7 +690,7 @@ __global__ __launch_bounds__(256) void k_phaseC2(PhaseArgs a) {
   const size_t o = (size_t)w * my;
   const bool m00 = (i == 0 && k == 0);
   double dr[2][M], ps[2][M];
-  load_col<M>((const T2*)a.spec[9] + o, my, lane, st, dr);
+  load_col<M>((const T2*)a.spec[9] + (size_t)w * rc.myp, my, lane, st, dr);
   add_mean<M>(dr, lane, my, m00, a.mean_out + 9 * (size_t)my);
   if (m00) {
     // rho v 00 mode = integral of drho00 + acoustic split (cross.f:1027-1072)

@@ -30,10 +30,9 @@ def momentum(state, ops: HostOps, alp, bet):
     kz = np.where(k <= nz, bet * k, -bet * (mz - k))
     rK = kx[:, None] ** 2 + kz[None, :] ** 2
     v = np.zeros_like(phi)
-    for i in range(np_):
-        for kk in range(mz):
-            if rK[i, kk] != 0.0:
-                v[i, kk] = ops.helm(phi[i, kk], rK[i, kk])
+    nzm = rK != 0.0
+    if np.any(nzm):
+        v[nzm] = ops.helm_batch(phi[nzm], rK[nzm])
     dv = _along_y(ops, ops.dy1, v)
     with np.errstate(divide="ignore", invalid="ignore"):
         ta = np.where(rK > 0, 1j * kx[:, None] / rK, 0.0)[:, :, None]

@@ -83,6 +83,24 @@ class MixingLayerConfig:
 # the five workloads (configs A-E)
 CONFIG_A = MixingLayerConfig(mgalx=64, my=129, mgalz=64, Re=160.0, s=2.0)
 CONFIG_B = MixingLayerConfig(mgalx=512, my=513, mgalz=256, Re=800.0, s=4.0)
+
+# ensemble batch: 32 runs, grids drawn by seed from these, density ratios
+# U[1,8], Re U[160,800], independent seeds, mixed precision
+ENSEMBLE_GRIDS = [(128, 129, 64), (192, 193, 96), (256, 257, 128)]
+
+
+def ensemble_configs(nruns: int = 32, seed: int = 7, grids=ENSEMBLE_GRIDS,
+                     ratio=(1.0, 8.0), re=(160.0, 800.0)):
+    """Seeded draw of the ensemble batch's run configurations."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(nruns):
+        g = grids[int(rng.integers(len(grids)))]
+        out.append(MixingLayerConfig(mgalx=g[0], my=g[1], mgalz=g[2],
+                                     s=float(rng.uniform(*ratio)),
+                                     Re=float(rng.uniform(*re)),
+                                     seed=int(rng.integers(2**31))))
+    return out
 ENSEMBLE_GRIDS = [(128, 129, 64), (192, 193, 96), (256, 257, 128)]
 SERVE_GRIDS = [(64, 129, 64), (128, 129, 64), (192, 193, 96), (256, 257, 128),
                (384, 385, 192)]

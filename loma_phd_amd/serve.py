@@ -1,0 +1,327 @@
+"""Request loop for many device-resident mixing-layer runs (continuous batching).
+
+Run requests arrive over time.  Each names a grid, a density ratio, a
+Reynolds number, a seed and a step budget.  The loop
+
+  * prepares each request's initial condition on the host as it arrives
+    (a worker thread, so the loop keeps polling meanwhile);
+  * admits requests in arrival order when a slot is free and the run's device
+    footprint (loma_footprint: the exact bytes loma_init will allocate) fits
+    in the memory budget beside the runs already resident;
+  * keeps every admitted run's spectral state on the GPU and keeps one RK3
+    step in flight per run, each run on its own non-blocking stream
+    (loma_step_begin / loma_step_poll), so the steps of all resident runs
+    overlap on the device and a newly admitted run joins the mix at once;
+  * retires a run when its budget is spent (final state downloaded, device
+    memory freed), or evicts it when its state turns non-finite.
+
+It reports per-step latency (host wall time from enqueue to observed
+completion, p50/p99) beside aggregate grid-point-steps/s.
+
+The reference has no request loop: its driver advances one run per MPI job
+(trunk/lomacte3/src/main.f:215-236).  One served run is that driver's
+cross1 time loop (cross.f:225-1226), one step per loma_step_begin.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import dataclasses
+import math
+import time
+from collections import deque
+from typing import Any, Callable
+
+import numpy as np
+
+from .grid import MixingLayerConfig
+
+
+@dataclasses.dataclass
+class Request:
+    """One run request: when it arrives (s after the loop starts), what to
+    run and for how many RK3 steps."""
+    rid: int
+    arrival: float
+    cfg: MixingLayerConfig
+    steps: int
+    precision: int = 1  # LOMA_MIXED
+
+    @property
+    def points(self) -> int:
+        return self.cfg.points
+
+
+@dataclasses.dataclass
+class Outcome:
+    rid: int
+    status: str = "waiting"  # done | nonfinite | rejected | cut | failed
+    steps_done: int = 0
+    footprint: int = 0
+    admit_s: float = math.nan
+    end_s: float = math.nan
+    error: str = ""
+    state: Any = None
+
+
+class HipBackend:
+    """Runs requests on the GPU through the C ABI (capi.Run)."""
+
+    def __init__(self, keep_state: bool = False, use_graph: bool = True):
+        from . import capi  # loads libloma_hip.so; fails loudly if it is missing
+        self.capi = capi
+        self.keep_state = keep_state
+        self.use_graph = use_graph
+
+    def flags(self, req: Request) -> int:
+        return self.capi.LOMA_FLAG_SCALAR_Z if req.cfg.scalar_z else 0
+
+    def footprint(self, req: Request) -> int:
+        c = req.cfg
+        return self.capi.footprint(c.mgalx, c.my, c.mgalz, req.precision, self.flags(req))
+
+    def prepare(self, req: Request):
+        from .ic import initial_condition
+        return initial_condition(req.cfg)
+
+    def start(self, req: Request, ic):
+        c = req.cfg
+        run = self.capi.Run(c.mgalx, c.my, c.mgalz, ic["y"], c.Re, c.alp, c.bet, pr=c.pr,
+                            cfl=c.cfl, ncfl=c.ncfl, dt0=c.dt0, precision=req.precision,
+                            flags=self.flags(req))
+        try:
+            run.upload(ic["vor"], ic["phi"], ic["psi"], ic["scal"], ic["u00"], ic["v00"],
+                       ic["w00"], z=ic["z"] if c.scalar_z else None)
+            run.use_graph(self.use_graph)
+        except Exception:
+            run.close()
+            raise
+        return run
+
+    def begin(self, run):
+        run.step_begin()
+
+    def poll(self, run):
+        return run.step_poll()
+
+    def finish(self, run):
+        try:
+            return run.download() if self.keep_state else None
+        finally:
+            run.close()
+
+    def free(self, run):
+        run.close()
+
+
+class _Inline:
+    """Executor stand-in that runs jobs at submission (workers=0)."""
+
+    def submit(self, fn, *args):
+        f: cf.Future = cf.Future()
+        try:
+            f.set_result(fn(*args))
+        except BaseException as e:  # noqa: BLE001 - handed to the caller
+            f.set_exception(e)
+        return f
+
+    def shutdown(self, wait=True):
+        pass
+
+
+@dataclasses.dataclass
+class _Live:
+    req: Request
+    out: Outcome
+    run: Any = None
+    t_begin: float = 0.0
+
+
+class Server:
+    """Continuous-batching request loop over one GPU.
+
+    mem_budget: device bytes the resident runs may hold together.
+    max_slots: resident runs at most (each holds one stream and one step in
+    flight).
+    workers: host threads for initial conditions, run set-up and retirement
+    (0 runs them inline in the loop).
+    """
+
+    def __init__(self, backend, mem_budget: int, max_slots: int = 32, workers: int = 2,
+                 clock: Callable[[], float] = time.perf_counter, idle_sleep: float = 1e-4):
+        self.b = backend
+        self.budget = int(mem_budget)
+        self.max_slots = int(max_slots)
+        self.workers = workers
+        self.clock = clock
+        self.idle_sleep = idle_sleep
+
+    def run(self, requests, window: float | None = None) -> dict:
+        """Serve `requests` (arrival times relative to the start).  With
+        `window`, admission stops at that time and runs still resident are cut
+        after their step in flight; otherwise every request is served to its
+        end.  Returns the report dict."""
+        reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        pool = (cf.ThreadPoolExecutor(max_workers=self.workers) if self.workers > 0
+                else _Inline())
+        outs = {r.rid: Outcome(r.rid) for r in reqs}
+        lat, dev_ms = [], []
+        pending = deque(reqs)          # not yet arrived
+        waiting: deque = deque()       # arrived, in arrival order: (req, prep future)
+        starting: list = []            # admitted, set-up in flight: (_Live, future)
+        active: list[_Live] = []       # resident, one step in flight
+        retiring: list = []            # (_Live, future) download / free in flight
+        used = 0
+        peak_used = 0
+        peak_active = 0
+        gp_steps = 0
+        t0 = self.clock()
+        closed = False
+        try:
+            while pending or waiting or starting or active or retiring:
+                now = self.clock() - t0
+                busy = False
+                if window is not None and now >= window and not closed:
+                    closed = True
+                    for r in pending:
+                        outs[r.rid].status = "cut"
+                    pending.clear()
+                    for r, f in waiting:
+                        f.cancel()
+                        outs[r.rid].status = "cut"
+                    waiting.clear()
+                # arrivals: start preparing the initial condition
+                while pending and pending[0].arrival <= now:
+                    r = pending.popleft()
+                    outs[r.rid].footprint = self.b.footprint(r)
+                    waiting.append((r, pool.submit(self.b.prepare, r)))
+                    busy = True
+                # admission in arrival order
+                while waiting and waiting[0][1].done():
+                    r, f = waiting[0]
+                    o = outs[r.rid]
+                    if o.footprint > self.budget or f.exception() is not None:
+                        waiting.popleft()
+                        o.status = "rejected" if o.footprint > self.budget else "failed"
+                        o.error = "" if f.exception() is None else repr(f.exception())
+                        o.end_s = now
+                        busy = True
+                        continue
+                    nres = len(active) + len(starting)
+                    if nres >= self.max_slots or used + o.footprint > self.budget:
+                        break
+                    waiting.popleft()
+                    used += o.footprint
+                    peak_used = max(peak_used, used)
+                    o.admit_s = now
+                    lv = _Live(r, o)
+                    starting.append((lv, pool.submit(self.b.start, r, f.result())))
+                    busy = True
+                # set-up finished: first step
+                for item in list(starting):
+                    lv, f = item
+                    if not f.done():
+                        continue
+                    starting.remove(item)
+                    busy = True
+                    if f.exception() is not None:
+                        lv.out.status, lv.out.error = "failed", repr(f.exception())
+                        lv.out.end_s = self.clock() - t0
+                        used -= lv.out.footprint
+                        continue
+                    lv.run = f.result()
+                    if closed or lv.req.steps <= 0:
+                        lv.out.status = "cut" if closed else "done"
+                        retiring.append((lv, pool.submit(self.b.finish, lv.run)))
+                        continue
+                    self.b.begin(lv.run)
+                    lv.t_begin = self.clock()
+                    active.append(lv)
+                peak_active = max(peak_active, len(active))
+                # poll the steps in flight
+                for lv in list(active):
+                    done, ms, nonfinite = self.b.poll(lv.run)
+                    if not done:
+                        continue
+                    t = self.clock()
+                    busy = True
+                    lat.append(t - lv.t_begin)
+                    dev_ms.append(ms)
+                    lv.out.steps_done += 1
+                    gp_steps += lv.req.points
+                    if nonfinite:
+                        lv.out.status = "nonfinite"
+                        active.remove(lv)
+                        retiring.append((lv, pool.submit(self.b.free, lv.run)))
+                    elif lv.out.steps_done >= lv.req.steps or closed:
+                        lv.out.status = "done" if lv.out.steps_done >= lv.req.steps else "cut"
+                        active.remove(lv)
+                        retiring.append((lv, pool.submit(self.b.finish, lv.run)))
+                    else:
+                        self.b.begin(lv.run)
+                        lv.t_begin = self.clock()
+                # retirement finished: memory returns to the budget
+                for item in list(retiring):
+                    lv, f = item
+                    if not f.done():
+                        continue
+                    retiring.remove(item)
+                    busy = True
+                    used -= lv.out.footprint
+                    lv.out.end_s = self.clock() - t0
+                    if f.exception() is not None:
+                        lv.out.status, lv.out.error = "failed", repr(f.exception())
+                    elif lv.out.status in ("done", "cut"):
+                        lv.out.state = f.result()
+                if not busy:
+                    time.sleep(self.idle_sleep)
+        finally:
+            # an exception in the loop: leave no run resident
+            for lv in active:
+                try:
+                    self.b.free(lv.run)
+                except Exception:  # noqa: BLE001
+                    pass
+            pool.shutdown(wait=True)
+        wall = self.clock() - t0
+        lat_ms = np.asarray(lat) * 1e3
+        return dict(
+            outcomes=outs,
+            wall_s=wall,
+            steps=int(lat_ms.size),
+            grid_point_steps=int(gp_steps),
+            gp_steps_per_s=gp_steps / wall if wall > 0 else 0.0,
+            p50_ms=float(np.percentile(lat_ms, 50)) if lat_ms.size else math.nan,
+            p99_ms=float(np.percentile(lat_ms, 99)) if lat_ms.size else math.nan,
+            device_p50_ms=float(np.percentile(dev_ms, 50)) if dev_ms else math.nan,
+            peak_bytes=int(peak_used),
+            peak_active=int(peak_active),
+            counts={s: sum(1 for o in outs.values() if o.status == s)
+                    for s in ("done", "cut", "nonfinite", "rejected", "failed")},
+        )
+
+
+# ---- the served-ensemble workload ----------------------------------------
+SERVED_GRIDS = [(64, 129, 64), (128, 129, 64), (192, 193, 96), (256, 257, 128),
+                (384, 385, 192)]
+
+
+def poisson_requests(seed: int, rate: float, horizon: float, grids=SERVED_GRIDS,
+                     ratio=(1.0, 8.0), re=(160.0, 800.0), steps=(200, 2000),
+                     precision: int = 1, scalar_z: bool = True):
+    """Requests arriving as a Poisson process of `rate` per second over
+    `horizon` seconds; grid, density ratio U[ratio], Re U[re] and step budget
+    U[steps] drawn per request from one seeded generator."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t, out, rid = 0.0, [], 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= horizon:
+            break
+        g = grids[int(rng.integers(len(grids)))]
+        cfg = MixingLayerConfig(mgalx=g[0], my=g[1], mgalz=g[2],
+                                s=float(rng.uniform(*ratio)), Re=float(rng.uniform(*re)),
+                                seed=int(rng.integers(2**31)), scalar_z=scalar_z)
+        out.append(Request(rid, t, cfg, int(rng.integers(steps[0], steps[1] + 1)),
+                           precision))
+        rid += 1
+    return out
