@@ -27,10 +27,23 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+# HBM bytes per launch measured with rocprofv3 on the 512x513x256 step
+# without Z: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors read off a
+# plain 1 GiB device copy in the same runs (python3 -m
+# loma_phd_amd.profile_step; summary in profiles/rk3_step_512x513x256_hbm_traffic.md).
+MEASURED_TRAFFIC = {
+    "fp64": {"phaseA": 4.481e9, "zfft_inv": 4.776e9, "xline": 8.032e9,
+             "zfft_fwd": 6.582e9, "phaseC": 4.724e9, "phaseC2": 1.715e9},
+    "mixed": {"phaseA": 3.600e9, "zfft_inv": 2.408e9, "xline": 7.056e9,
+              "zfft_fwd": 3.327e9, "phaseC": 3.506e9, "phaseC2": 1.592e9},
+}
+
 
 def algorithmic_bytes(run, with_z: bool):
     """Compulsory HBM bytes per launch of each kernel class (one substep)
-    and per RK3 step; see DESIGN.md for the derivation."""
+    and per RK3 step: one read of every field a kernel consumes and one write
+    of every field it produces (S, W, P sized; es bytes per complex transform
+    entry), nothing else."""
     S, W, P = run.S, run.W, run.P
     es = 8 if run.precision == 1 else 16
     z = 1 if with_z else 0
@@ -103,15 +116,17 @@ def bench_precision(cfg, st, precision, steps, warmup, prof_steps):
     return out
 
 
-def roofline_of(res):
+def roofline_of(res, traffic=None):
     # dominant kernel = largest total time per step
     per_step = {k: res["kernels_ms"][k] * (3 if k != "finish" else 1)
                 for k in res["kernels_ms"]}
     dom = max((k for k in per_step if k in res["kernel_bytes"]), key=lambda k: per_step[k])
     t = res["kernels_ms"][dom] * 1e-3
     achieved = res["kernel_bytes"][dom] / t / 1e9
+    tr = traffic.get(dom) if traffic else None
     return dom, dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                     traffic=int(tr) if tr else None,
                      kernel=dom, bytes_per_launch=int(res["kernel_bytes"][dom]),
                      avg_launch_ms=round(res["kernels_ms"][dom], 4))
 
@@ -198,6 +213,8 @@ def bench_served(window, rate=2.0, seed=11, max_slots=32, workers=4):
                 gpts_per_s=round(rep["gp_steps_per_s"], 1), steps=rep["steps"],
                 p50_ms=round(rep["p50_ms"], 3), p99_ms=round(rep["p99_ms"], 3),
                 device_p50_ms=round(rep["device_p50_ms"], 3),
+                by_grid={g: {k: (round(v, 3) if isinstance(v, float) else v)
+                             for k, v in d.items()} for g, d in rep["by_grid"].items()},
                 peak_active=rep["peak_active"], peak_bytes=rep["peak_bytes"],
                 budget_bytes=budget, outcomes=rep["counts"], precision="mixed",
                 latency="host wall time from step enqueue to observed completion")
@@ -243,7 +260,9 @@ def main():
               flush=True)
     head = res.get("mixed") or res["fp64"]
     mixed = "mixed" in res
-    dom, roof = roofline_of(head)
+    measured = (args.config == "B" and not cfg.scalar_z)
+    tr = {p: (MEASURED_TRAFFIC[p] if measured else None) for p in ("fp64", "mixed")}
+    dom, roof = roofline_of(head, tr["mixed" if mixed else "fp64"])
     S, P = head["S"], head["P"]
     line = {
         "metric": "grid-pt*RK3-steps/s (mixed vs FP64, 1 GPU)",
@@ -276,7 +295,7 @@ def main():
                           model_bytes_per_step(S, P, prec == "mixed") /
                           (r["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
                       "kernel_ms": {k: round(v, 4) for k, v in r["kernels_ms"].items()},
-                      "dominant": roofline_of(r)[1]}
+                      "dominant": roofline_of(r, tr[prec])[1]}
     if "fp64" in res and "mixed" in res:
         line["speedup_mixed_vs_fp64"] = round(res["mixed"]["gpts_per_s"] /
                                               res["fp64"]["gpts_per_s"], 3)

@@ -154,7 +154,7 @@ static size_t plan_bytes(int mgalx, int my, int mgalz, bool fp32, bool with_z) {
   const size_t S = np * mz * my, Sp = np * mz * myp, Wp = np * mgalz * myp;
   const size_t P = (size_t)mgalx * my * mgalz, es = fp32 ? 8 : 16;
   const size_t nsp = with_z ? 12 : 11, nst = with_z ? 7 : 6;
-  size_t b = 8 * (46 * R + 64 * 6 * 44);       // per-row and scan tables
+  size_t b = 8 * (size_t)ytab_offset(YT_COUNT, M);  // per-row and scan tables
   b += 2 * 8 * 5 * (size_t)my;                 // band copies
   b += 8 * nclass + 8 * 5 * R * nclass + 32 * nclass;  // shifts, LU, Neumann constants
   b += nst * S * 16;                           // spectral state and histories
@@ -314,53 +314,47 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
       wbd[(size_t)j * 2 + 1] = d[my - 1] * f.fmap[my - 1];
     }
   }
-  const int NT = 15;
-  const int nb[NT] = {7, 7, 1, 3, 3, 1, 5, 5, 1, 2, 2, 1, 3, 3, 2};
-  size_t off[NT], tot = 0;
-  for (int q = 0; q < NT; ++q) { off[q] = tot; tot += (size_t)nb[q] * R; }
-  // scan tables of the fixed factors: L1, U1 (P=3), L2, U2 (P=2), Li, Ui (P=3)
-  const int hp[6] = {3, 3, 2, 2, 3, 3};
-  size_t hoff[6];
-  for (int q = 0; q < 6; ++q) { hoff[q] = tot; tot += (size_t)6 * hp[q] * hp[q] * 64; }
+  // per-row tables and scan compositions at ytab_offset(id, M), the layout
+  // the kernels address with constant offsets
+  const size_t tot = ytab_offset(YT_COUNT, M);
   std::vector<double> h(tot, 0.0);
   auto lm = [&](int row, int w, int idx) -> size_t {
     const int l = row / M, t = row - l * M;
     return (size_t)(t * w + idx) * 64 + l;
   };
   // row-indexed table -> lane-major
-  auto put = [&](int q, const std::vector<double>& src, int w) {
+  auto put = [&](int id, const std::vector<double>& src) {
+    const int w = ytab_width(id);
+    const size_t o = ytab_offset(id, M);
     for (int j = 0; j < my; ++j)
-      for (int d = 0; d < w; ++d) h[off[q] + lm(j, w, d)] = src[(size_t)j * w + d];
+      for (int d = 0; d < w; ++d) h[o + lm(j, w, d)] = src[(size_t)j * w + d];
   };
   // lower multipliers: lo[k][i-1] (row k into row k+i) -> destination row k+i
-  auto put_lo = [&](int q, const std::vector<double>& src, int w) {
+  auto put_lo = [&](int id, const std::vector<double>& src) {
+    const int w = ytab_width(id);
+    const size_t o = ytab_offset(id, M);
     for (int j = 0; j < my; ++j)
       for (int i = 1; i <= w; ++i)
-        if (j - i >= 0) h[off[q] + lm(j, w, i - 1)] = src[(size_t)(j - i) * w + i - 1];
+        if (j - i >= 0) h[o + lm(j, w, i - 1)] = src[(size_t)(j - i) * w + i - 1];
   };
-  put(0, f.b1, 7); put(1, f.a1, 7); put(2, f.inv1, 1); put(3, f.up1, 3); put_lo(4, f.lo1, 3);
-  put(5, f.fmap, 1); put(6, f.b2, 5); put(7, f.a2, 5); put(8, f.inv2, 1); put(9, f.up2, 2);
-  put_lo(10, f.lo2, 2); put(11, f.invi, 1); put(12, f.upi, 3); put_lo(13, f.loi, 3);
-  put(14, wbd, 2);
+  put(YT_B1, f.b1); put(YT_A1, f.a1); put(YT_INV1, f.inv1); put(YT_UP1, f.up1);
+  put_lo(YT_LO1, f.lo1); put(YT_FMAP, f.fmap); put(YT_B2, f.b2); put(YT_A2, f.a2);
+  put(YT_INV2, f.inv2); put(YT_UP2, f.up2); put_lo(YT_LO2, f.lo2); put(YT_INVI, f.invi);
+  put(YT_UPI, f.upi); put_lo(YT_LOI, f.loi); put(YT_WBD, wbd);
+  auto H = [&](int id) { return h.data() + ytab_offset(id, M); };
   int lev[6];
-  lev[0] = scan_table(my, M, 3, false, f.lo1.data(), nullptr, nullptr, h.data() + hoff[0]);
-  lev[1] = scan_table(my, M, 3, true, nullptr, f.up1.data(), f.inv1.data(), h.data() + hoff[1]);
-  lev[2] = scan_table(my, M, 2, false, f.lo2.data(), nullptr, nullptr, h.data() + hoff[2]);
-  lev[3] = scan_table(my, M, 2, true, nullptr, f.up2.data(), f.inv2.data(), h.data() + hoff[3]);
-  lev[4] = scan_table(my, M, 3, false, f.loi.data(), nullptr, nullptr, h.data() + hoff[4]);
-  lev[5] = scan_table(my, M, 3, true, nullptr, f.upi.data(), f.invi.data(), h.data() + hoff[5]);
+  lev[0] = scan_table(my, M, 3, false, f.lo1.data(), nullptr, nullptr, H(YT_HL1));
+  lev[1] = scan_table(my, M, 3, true, nullptr, f.up1.data(), f.inv1.data(), H(YT_HU1));
+  lev[2] = scan_table(my, M, 2, false, f.lo2.data(), nullptr, nullptr, H(YT_HL2));
+  lev[3] = scan_table(my, M, 2, true, nullptr, f.up2.data(), f.inv2.data(), H(YT_HU2));
+  lev[4] = scan_table(my, M, 3, false, f.loi.data(), nullptr, nullptr, H(YT_HLI));
+  lev[5] = scan_table(my, M, 3, true, nullptr, f.upi.data(), f.invi.data(), H(YT_HUI));
   if ((rc = dalloc(r, &r->tab, tot * sizeof(double)))) { loma_free(r); return rc; }
   if (hipMemcpy(r->tab, h.data(), tot * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
     set_error("loma_init: table upload failed"); loma_free(r); return LOMA_ERR_DEVICE;
   }
   YTab& T = r->T;
-  T.b1 = r->tab + off[0]; T.a1 = r->tab + off[1]; T.inv1 = r->tab + off[2];
-  T.up1 = r->tab + off[3]; T.lo1 = r->tab + off[4]; T.fmap = r->tab + off[5];
-  T.b2 = r->tab + off[6]; T.a2 = r->tab + off[7]; T.inv2 = r->tab + off[8];
-  T.up2 = r->tab + off[9]; T.lo2 = r->tab + off[10]; T.invi = r->tab + off[11];
-  T.upi = r->tab + off[12]; T.loi = r->tab + off[13]; T.wbd = r->tab + off[14];
-  T.hL1 = r->tab + hoff[0]; T.hU1 = r->tab + hoff[1]; T.hL2 = r->tab + hoff[2];
-  T.hU2 = r->tab + hoff[3]; T.hLi = r->tab + hoff[4]; T.hUi = r->tab + hoff[5];
+  T.base = r->tab;
   T.kL1 = lev[0]; T.kU1 = lev[1]; T.kL2 = lev[2]; T.kU2 = lev[3]; T.kLi = lev[4];
   T.kUi = lev[5];
   std::memcpy(r->scan_levels, lev, sizeof(lev));

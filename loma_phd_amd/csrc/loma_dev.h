@@ -520,42 +520,66 @@ __device__ __forceinline__ double col_value(const double (&x)[M], int row) {
   return __shfl(mine, l);
 }
 
-// Device tables of one y grid, lane-major, zero padded to 64*M rows.
+// Tables of one y grid, in order, in one allocation.  Per-row tables hold
+// nb entries for each of 64*M rows (lane-major); the scan compositions of the
+// fixed factors hold 6 levels of P x P maps for 64 lanes.
+enum YTabId {
+  YT_B1, YT_A1, YT_INV1, YT_UP1, YT_LO1, YT_FMAP,  // d/deta: B1, A1, LU of A1, fmap
+  YT_B2, YT_A2, YT_INV2, YT_UP2, YT_LO2,           // d2/dy2: B2, A2, LU of A2
+  YT_INVI, YT_UPI, YT_LOI,                         // integration: LU of B1
+  YT_WBD,                                          // boundary d/dy functionals
+  YT_NROW,                                         // (count of per-row tables)
+  YT_HL1 = YT_NROW, YT_HU1, YT_HL2, YT_HU2, YT_HLI, YT_HUI,
+  YT_COUNT
+};
+__host__ __device__ constexpr int ytab_width(int id) {
+  // entries per row of the per-row tables; P of the scan tables
+  return id == YT_B1 || id == YT_A1 ? 7
+       : id == YT_B2 || id == YT_A2 ? 5
+       : id == YT_UP1 || id == YT_LO1 || id == YT_UPI || id == YT_LOI ? 3
+       : id == YT_UP2 || id == YT_LO2 || id == YT_WBD ? 2
+       : id == YT_HL1 || id == YT_HU1 || id == YT_HLI || id == YT_HUI ? 3
+       : id == YT_HL2 || id == YT_HU2 ? 2 : 1;
+}
+// element offset of table `id`; ytab_offset(YT_COUNT, M) is the total size
+__host__ __device__ constexpr int ytab_offset(int id, int M) {
+  int off = 0;
+  for (int q = 0; q < id; ++q) {
+    const int w = ytab_width(q);
+    off += q < YT_NROW ? w * 64 * M : 6 * w * w * 64;
+  }
+  return off;
+}
+
+// Device tables of one y grid: one allocation, lane-major tables of 64*M
+// rows each at the offsets ytab_offset(id, M) (loma_kernels.h), so that every
+// table access goes through one buffer resource with a constant offset.
 struct YTab {
-  const double* b1;    // [7]  RHS of d/deta (dt12)
-  const double* a1;    // [7]  LHS of d/deta (dt11, unfactored)
-  const double* inv1;  // [1]  LU of a1
-  const double* up1;   // [3]
-  const double* lo1;   // [3]  (by destination row)
-  const double* fmap;  // [1]
-  const double* b2;    // [5]  RHS of d2/dy2 (dt22)
-  const double* a2;    // [5]  LHS of d2/dy2 (dt21, unfactored)
-  const double* inv2;  // LU of a2
-  const double* up2;   // [2]
-  const double* lo2;   // [2]
-  const double* invi;  // LU of b1 (integration)
-  const double* upi;   // [3]
-  const double* loi;   // [3]
-  const double* wbd;   // [2]  d/dy at row 0 and at row my-1 as row functionals
-  // scan compositions [6*P][P] per lane (see lsolve_tab) and level counts
-  const double *hL1, *hU1, *hL2, *hU2, *hLi, *hUi;
+  const double* base;
+  // scan level counts of the fixed factors (see lsolve_tab)
   int kL1, kU1, kL2, kU2, kLi, kUi;
   int my;
 };
+
+// view of table `id` (ncol entries per row) for this lane
+template <int M>
+__device__ __forceinline__ TabRef ytab(const YTab& T, int id, int ncol, int lane) {
+  return TabRef{make_rsrc(T.base, 0x7fffffff), ncol, 0, lane * 8, ytab_offset(id, M) * 8};
+}
 
 // first derivative du = fmap * A1^{-1} B1 u   (deryr2, cfdiff.f:518-601)
 template <int M, int NC>
 __device__ __forceinline__ void op_dy1(const YTab& T, int lane, const double (&u)[NC][M],
                                        double (&du)[NC][M], bool apply_fmap = true) {
-  bmatvec<M, 3, NC>(tab(T.b1, 7, 0, lane), lane, u, du);
+  bmatvec<M, 3, NC>(ytab<M>(T, YT_B1, 7, lane), lane, u, du);
   op_fence();
-  lsolve_tab<M, 3, NC>(tab(T.lo1, 3, 0, lane), tab(T.hL1, 3, 0, lane), T.kL1, lane, du);
+  lsolve_tab<M, 3, NC>(ytab<M>(T, YT_LO1, 3, lane), ytab<M>(T, YT_HL1, 3, lane), T.kL1, lane, du);
   op_fence();
-  usolve_tab<M, 3, NC>(tab(T.up1, 3, 0, lane), tab(T.inv1, 1, 0, lane),
-                       tab(T.hU1, 3, 0, lane), T.kU1, lane, du);
+  usolve_tab<M, 3, NC>(ytab<M>(T, YT_UP1, 3, lane), ytab<M>(T, YT_INV1, 1, lane),
+                       ytab<M>(T, YT_HU1, 3, lane), T.kU1, lane, du);
   op_fence();
   if (apply_fmap) {
-    const TabRef fm = tab(T.fmap, 1, 0, lane);
+    const TabRef fm = ytab<M>(T, YT_FMAP, 1, lane);
 #pragma unroll
     for (int t = 0; t < M; ++t) {
       const double f = tget(fm, t, 0);
@@ -570,12 +594,12 @@ __device__ __forceinline__ void op_dy1(const YTab& T, int lane, const double (&u
 template <int M, int NC>
 __device__ __forceinline__ void op_dy2(const YTab& T, int lane, const double (&u)[NC][M],
                                        double (&d2u)[NC][M]) {
-  bmatvec<M, 2, NC>(tab(T.b2, 5, 0, lane), lane, u, d2u);
+  bmatvec<M, 2, NC>(ytab<M>(T, YT_B2, 5, lane), lane, u, d2u);
   op_fence();
-  lsolve_tab<M, 2, NC>(tab(T.lo2, 2, 0, lane), tab(T.hL2, 2, 0, lane), T.kL2, lane, d2u);
+  lsolve_tab<M, 2, NC>(ytab<M>(T, YT_LO2, 2, lane), ytab<M>(T, YT_HL2, 2, lane), T.kL2, lane, d2u);
   op_fence();
-  usolve_tab<M, 2, NC>(tab(T.up2, 2, 0, lane), tab(T.inv2, 1, 0, lane),
-                       tab(T.hU2, 2, 0, lane), T.kU2, lane, d2u);
+  usolve_tab<M, 2, NC>(ytab<M>(T, YT_UP2, 2, lane), ytab<M>(T, YT_INV2, 1, lane),
+                       ytab<M>(T, YT_HU2, 2, lane), T.kU2, lane, d2u);
   op_fence();
 }
 
@@ -587,7 +611,7 @@ template <int M, int NC>
 __device__ __forceinline__ void op_helm(const YTab& T, const double* __restrict__ lu,
                                         int lane, const double (&f)[NC][M],
                                         double (&v)[NC][M]) {
-  bmatvec<M, 2, NC>(tab(T.a2, 5, 0, lane), lane, f, v);
+  bmatvec<M, 2, NC>(ytab<M>(T, YT_A2, 5, lane), lane, f, v);
   const int r0 = lane * M;
 #pragma unroll
   for (int t = 0; t < M; ++t) {
@@ -646,18 +670,26 @@ __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, int voff, doubl
   __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, 0, 0);
 }
 
+// Offset of row t*64 + lane of a column, opaque to the optimiser: the
+// constant part must stay in the vector offset, which the range check
+// covers (split off into the scalar offset it would escape the check).
+template <typename T2>
+__device__ __forceinline__ int col_voff(int lane, int t) {
+  int v = (t * WAVE + lane) * (int)sizeof(T2);
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // HBM column (T2 = float2 or double2; `col` wavefront-uniform) -> LDS stage.
-// Row p = t*64 + lane; the resource of block t starts at row t*64 and ends at
-// row my, so rows >= my read as 0 without a test.  All M loads are issued
-// before the first is used.
+// Row p = t*64 + lane; one resource spans rows 0..my-1, so rows >= my read
+// as 0 without a test.  All M loads are issued before the first is used.
 template <int M, typename T2>
 __device__ __forceinline__ void stage_in(const T2* __restrict__ col, int my, int lane,
                                          double2* st) {
+  const auto r = make_rsrc(col, my * (int)sizeof(T2));
   double2 q[M];
 #pragma unroll
-  for (int t = 0; t < M; ++t)
-    q[t] = buf_ld(make_rsrc(col + t * WAVE, (my - t * WAVE) * (int)sizeof(T2)),
-                  lane * (int)sizeof(T2), (T2*)nullptr);
+  for (int t = 0; t < M; ++t) q[t] = buf_ld(r, col_voff<T2>(lane, t), (T2*)nullptr);
   wave_sync();
 #pragma unroll
   for (int t = 0; t < M; ++t) st[t * WAVE + lane] = q[t];
@@ -672,10 +704,9 @@ __device__ __forceinline__ void stage_out(const double2* st, int my, int lane,
 #pragma unroll
   for (int t = 0; t < M; ++t) v[t] = st[t * WAVE + lane];
   wave_sync();
+  const auto r = make_rsrc(col, my * (int)sizeof(T2));
 #pragma unroll
-  for (int t = 0; t < M; ++t)
-    buf_st(make_rsrc(col + t * WAVE, (my - t * WAVE) * (int)sizeof(T2)),
-           lane * (int)sizeof(T2), v[t], (T2*)nullptr);
+  for (int t = 0; t < M; ++t) buf_st(r, col_voff<T2>(lane, t), v[t], (T2*)nullptr);
 }
 // stage -> registers (own chunk)
 template <int M>

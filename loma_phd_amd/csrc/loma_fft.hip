@@ -127,7 +127,7 @@ __device__ __forceinline__ void dft5(T2* v, int s) {
 #endif
 // waves per SIMD the x-line and z-transform kernels are compiled for
 #ifndef LOMA_XL_WPE
-#define LOMA_XL_WPE 3
+#define LOMA_XL_WPE 2
 #endif
 #ifndef LOMA_ZF_WPE
 #define LOMA_ZF_WPE 4
@@ -466,33 +466,67 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 }
 __host__ inline int xcd_groups(int n) { return 8 * ((n + 7) / 8); }
 
-// spectral lines of field `src` (W layout [i][z][j]) -> Hermitian-packed
-// complex rows in A (row p = lines j0+2p, j0+2p+1), then inverse FFT.
-// Returns the buffer holding the physical pairs.
-template <typename T, typename T2, int EPT, int NN>
+// Per-thread share of one input field's lines: XR (kx, line pair) items of
+// two complex coefficients, all loaded before the first is packed.
+// (Fetching the next field during the current field's FFT was measured:
+// no gain in FP64, and in FP32 the held registers cost more than the hidden
+// latency.)
+template <typename T2, int XR>
+struct XlIn {
+  T2 a[XR], b[XR];
+};
+
+template <typename T2, int XR, int NN>
+__device__ __forceinline__ void xl_fetch(const FftArgs& a, const T2* __restrict__ src, int z,
+                                         int j0, int pairs, XlIn<T2, XR>& v) {
+  const int np = NN ? NN / 3 : a.rc.nplanes, my = a.rc.my, mgz = a.rc.mgalz;
+  const int lp = __builtin_ctz(pairs);  // pairs is a power of two
+#pragma unroll
+  for (int r = 0; r < XR; ++r) {
+    const int it = threadIdx.x + r * XL_THREADS;
+    T2 ca = T2{0, 0}, cb = T2{0, 0};
+    if (it < np * pairs) {
+      const int ii = it >> lp, p = it & (pairs - 1);
+      const int ja = j0 + 2 * p, jb = ja + 1;
+      const size_t base = ((size_t)ii * mgz + z) * a.rc.myp;
+      if (ja < my) ca = src[base + ja];
+      if (jb < my) cb = src[base + jb];
+    }
+    v.a[r] = ca;
+    v.b[r] = cb;
+  }
+}
+
+// lines of field `src` (W layout [i][z][j]) -> Hermitian-packed complex
+// rows in A (row p = lines j0+2p, j0+2p+1), then the inverse FFT.  Returns
+// the buffer holding the physical pairs.
+template <typename T, typename T2, int XR, int NN>
 __device__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src, T2* A, T2* B, int z,
                           int j0, int pairs) {
-  const int N = NN ? NN : a.rc.mgalx, np = a.rc.nplanes, my = a.rc.my, mgz = a.rc.mgalz;
+  XlIn<T2, XR> v;
+  xl_fetch<T2, XR, NN>(a, src, z, j0, pairs, v);
+  const int N = NN ? NN : a.rc.mgalx, np = NN ? NN / 3 : a.rc.nplanes;
+  const int lp = __builtin_ctz(pairs);
   // the dealiased band np..N-np is zero
   const int nzero = N - 2 * np + 1;
-  for (int it = threadIdx.x; it < pairs * nzero; it += blockDim.x) {
+  for (int it = threadIdx.x; it < pairs * nzero; it += XL_THREADS) {
     const int p = it / nzero, q = it - p * nzero;
     A[lds_i(p * N + np + q)] = T2{0, 0};
   }
-  for (int it = threadIdx.x; it < np * pairs; it += blockDim.x) {
-    const int ii = it / pairs, p = it - ii * pairs;
-    const int ja = j0 + 2 * p, jb = ja + 1;
-    const size_t base = ((size_t)ii * mgz + z) * a.rc.myp;
-    T2 ca = T2{0, 0}, cb = T2{0, 0};
-    if (ja < my) ca = src[base + ja];
-    if (jb < my) cb = src[base + jb];
-    const int rb = p * N;
-    if (ii == 0) {
-      // the imaginary part of the kx=0 coefficient is ignored (rft, rftsingle.f:87-91)
-      A[lds_i(rb)] = T2{ca.x, cb.x};
-    } else {
-      A[lds_i(rb + ii)] = T2{ca.x - cb.y, ca.y + cb.x};           // X_a + i X_b
-      A[lds_i(rb + N - ii)] = T2{ca.x + cb.y, -ca.y + cb.x};      // conj(X_a) + i conj(X_b)
+#pragma unroll
+  for (int r = 0; r < XR; ++r) {
+    const int it = threadIdx.x + r * XL_THREADS;
+    if (it < np * pairs) {
+      const int ii = it >> lp, p = it & (pairs - 1);
+      const T2 ca = v.a[r], cb = v.b[r];
+      const int rb = p * N;
+      if (ii == 0) {
+        // the imaginary part of the kx=0 coefficient is ignored (rft, rftsingle.f:87-91)
+        A[lds_i(rb)] = T2{ca.x, cb.x};
+      } else {
+        A[lds_i(rb + ii)] = T2{ca.x - cb.y, ca.y + cb.x};           // X_a + i X_b
+        A[lds_i(rb + N - ii)] = T2{ca.x + cb.y, -ca.y + cb.x};      // conj(X_a) + i conj(X_b)
+      }
     }
   }
   __syncthreads();
@@ -501,15 +535,17 @@ __device__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src, T2* A, T
 
 // forward FFT of A (filled by the caller, then synchronised here) and
 // Hermitian unpacking of the pairs into field `dst` (scaled by 1/N)
-template <typename T, typename T2, int EPT, int NN>
+template <typename T, typename T2, int NN>
 __device__ void xl_forward(const FftArgs& a, T2* __restrict__ dst, T2* A, T2* B, int z,
                            int j0, int pairs) {
-  const int N = NN ? NN : a.rc.mgalx, np = a.rc.nplanes, my = a.rc.my, mgz = a.rc.mgalz;
+  const int N = NN ? NN : a.rc.mgalx, np = NN ? NN / 3 : a.rc.nplanes, my = a.rc.my,
+            mgz = a.rc.mgalz;
+  const int lp = __builtin_ctz(pairs);
   __syncthreads();
   const T2* R = fft_any<T, T2, NN, XL_THREADS>(A, B, N, pairs, a.plx, (const T2*)a.twx, -1);
   const T hs = (T)(0.5 / (double)N);
-  for (int it = threadIdx.x; it < np * pairs; it += blockDim.x) {
-    const int ii = it / pairs, p = it - ii * pairs;
+  for (int it = threadIdx.x; it < np * pairs; it += XL_THREADS) {
+    const int ii = it >> lp, p = it & (pairs - 1);
     const int ja = j0 + 2 * p, jb = ja + 1;
     const int rb = p * N;
     const T2 zk = R[lds_i(rb + ii)];
@@ -559,8 +595,9 @@ __device__ __forceinline__ XlRk xl_rk(double dt, int rkstep) {
   return XlRk{dt * xis[rk], dt * gam[rk], dt * betas[rk], alphas[rk] / betas[rk]};
 }
 
-// FP64 is compiled for two waves per SIMD: at three its register demand
-// spills heavily, and the Z-transport build gave wrong results at 3.
+// Compiled for two waves per SIMD (LOMA_XL_WPE; FP64 always): at three,
+// FP64 spills heavily (and its Z-transport build gave wrong results), and
+// FP32 spills enough to measure slower (3.33 vs 3.22 ms at 512x513x256).
 template <typename T, bool WZ, int EPT, int NN>
 __global__ __launch_bounds__(XL_THREADS)
 __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 : LOMA_XL_WPE)))) void k_xline(FftArgs a) {
@@ -646,10 +683,12 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
 
   // ---- pass 1: T, rho u_i -> u_i, rho u_i u_j; CFL maxima ----
   T tt[EPT][2], ru[EPT][2], rv[EPT][2], rw[EPT][2];  // perturbations
-  grab(xl_inverse<T, T2, EPT, NN>(a, wb[7], A, B, z, j0, pairs), tt);
-  grab(xl_inverse<T, T2, EPT, NN>(a, wb[0], A, B, z, j0, pairs), ru);
-  grab(xl_inverse<T, T2, EPT, NN>(a, wb[1], A, B, z, j0, pairs), rv);
-  grab(xl_inverse<T, T2, EPT, NN>(a, wb[2], A, B, z, j0, pairs), rw);
+  // XR items of the np * pairs input coefficients per thread
+  constexpr int XR = (EPT + 2) / 3;
+  grab(xl_inverse<T, T2, XR, NN>(a, wb[7], A, B, z, j0, pairs), tt);
+  grab(xl_inverse<T, T2, XR, NN>(a, wb[0], A, B, z, j0, pairs), ru);
+  grab(xl_inverse<T, T2, XR, NN>(a, wb[1], A, B, z, j0, pairs), rv);
+  grab(xl_inverse<T, T2, XR, NN>(a, wb[2], A, B, z, j0, pairs), rw);
   // profiles of rho u, rho v, rho w, T for the two lines
   double pu[2], pv[2], pw[2], pT[2];
 #pragma unroll
@@ -723,7 +762,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
       }
       A[lds_i(p * N + x)] = T2{val[0], val[1]};
     }
-    xl_forward<T, T2, EPT, NN>(a, wo[q], A, B, z, j0, pairs);
+    xl_forward<T, T2, NN>(a, wo[q], A, B, z, j0, pairs);
   }
 
   // ---- pass 2: r = T (lap T / Pe - rho u . grad T)  (cross.f:1652-1673) ----
@@ -731,13 +770,13 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
   T acc[EPT][2], accz[EPT][2];
   {
     T f[EPT][2];
-    grab(xl_inverse<T, T2, EPT, NN>(a, wb[3], A, B, z, j0, pairs), f);  // lap T
+    grab(xl_inverse<T, T2, XR, NN>(a, wb[3], A, B, z, j0, pairs), f);  // lap T
 #pragma unroll
     for (int m = 0; m < EPT; ++m)
 #pragma unroll
       for (int h = 0; h < 2; ++h) acc[m][h] = f[m][h] * (T)ipe;
     if constexpr (WZ) {
-      grab(xl_inverse<T, T2, EPT, NN>(a, wb[11], A, B, z, j0, pairs), f);  // lap Z
+      grab(xl_inverse<T, T2, XR, NN>(a, wb[11], A, B, z, j0, pairs), f);  // lap Z
 #pragma unroll
       for (int m = 0; m < EPT; ++m)
 #pragma unroll
@@ -750,7 +789,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
       opaque_regs(ru);
       opaque_regs(rv);
       opaque_regs(rw);
-      grab(xl_inverse<T, T2, EPT, NN>(a, wb[4 + c], A, B, z, j0, pairs), f);
+      grab(xl_inverse<T, T2, XR, NN>(a, wb[4 + c], A, B, z, j0, pairs), f);  // T_x, T_y, T_z
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const T R_ = (T)(c == 0 ? pu[h] : (c == 1 ? pv[h] : pw[h])), G_ = (T)prof(h, 4 + c);
@@ -761,7 +800,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
         }
       }
       if constexpr (WZ) {
-        grab(xl_inverse<T, T2, EPT, NN>(a, wb[8 + c], A, B, z, j0, pairs), f);
+        grab(xl_inverse<T, T2, XR, NN>(a, wb[8 + c], A, B, z, j0, pairs), f);  // Z_x, Z_y, Z_z
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const T R_ = (T)(c == 0 ? pu[h] : (c == 1 ? pv[h] : pw[h])), G_ = (T)prof(h, 8 + c);
@@ -810,12 +849,12 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
   }
   // outputs 9 drho, 10 DT, 11 DZ
   put(rv);
-  xl_forward<T, T2, EPT, NN>(a, wo[9], A, B, z, j0, pairs);
+  xl_forward<T, T2, NN>(a, wo[9], A, B, z, j0, pairs);
   put(ru);
-  xl_forward<T, T2, EPT, NN>(a, wo[10], A, B, z, j0, pairs);
+  xl_forward<T, T2, NN>(a, wo[10], A, B, z, j0, pairs);
   if constexpr (WZ) {
     put(rw);
-    xl_forward<T, T2, EPT, NN>(a, wo[11], A, B, z, j0, pairs);
+    xl_forward<T, T2, NN>(a, wo[11], A, B, z, j0, pairs);
   }
 }
 

@@ -48,7 +48,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <int M>
 __device__ __forceinline__ void bd_dots(const YTab& T, int lane, const double (&x)[2][M],
                                         double (&out)[4]) {
-  const TabRef w = tab(T.wbd, 2, 0, lane);
+  const TabRef w = ytab<M>(T, YT_WBD, 2, lane);
   double s0r = 0.0, s0i = 0.0, s1r = 0.0, s1i = 0.0;
 #pragma unroll
   for (int t = 0; t < M; ++t) {
@@ -85,7 +85,7 @@ __device__ __forceinline__ void solve_neumann(const YTab& T, const double* lu,
   const double ci1 = (-dp[1] * a22 + a12 * dp[3]) / det;
   const double ci2 = (-a11 * dp[3] + a21 * dp[1]) / det;
   // second solve with Dirichlet end values (c1, c2)
-  bmatvec<M, 2, 2>(tab(T.a2, 5, 0, lane), lane, f, v);
+  bmatvec<M, 2, 2>(ytab<M>(T, YT_A2, 5, lane), lane, f, v);
   const int r0 = lane * M;
 #pragma unroll
   for (int t = 0; t < M; ++t) {
@@ -183,15 +183,16 @@ template <int M>
 __device__ __forceinline__ void op_inty(const YTab& T, int lane, double (&x)[1][M]) {
   double f[1][M];
   const int r0 = lane * M;
-  const TabRef fm = tab(T.fmap, 1, 0, lane);
+  const TabRef fm = ytab<M>(T, YT_FMAP, 1, lane);
 #pragma unroll
   for (int t = 0; t < M; ++t) f[0][t] = (r0 + t < T.my) ? x[0][t] / tget(fm, t, 0) : 0.0;
-  bmatvec<M, 3, 1>(tab(T.a1, 7, 0, lane), lane, f, x);
+  bmatvec<M, 3, 1>(ytab<M>(T, YT_A1, 7, lane), lane, f, x);
 #pragma unroll
   for (int t = 0; t < M; ++t)
     if (r0 + t == T.my - 1) x[0][t] = 0.0;  // gauge row x(my) = 0
-  lsolve_tab<M, 3, 1>(tab(T.loi, 3, 0, lane), tab(T.hLi, 3, 0, lane), T.kLi, lane, x);
-  usolve_tab<M, 3, 1>(tab(T.upi, 3, 0, lane), tab(T.invi, 1, 0, lane), tab(T.hUi, 3, 0, lane),
+  lsolve_tab<M, 3, 1>(ytab<M>(T, YT_LOI, 3, lane), ytab<M>(T, YT_HLI, 3, lane), T.kLi, lane, x);
+  usolve_tab<M, 3, 1>(ytab<M>(T, YT_UPI, 3, lane), ytab<M>(T, YT_INVI, 1, lane),
+                      ytab<M>(T, YT_HUI, 3, lane),
                       T.kUi, lane, x);
   const double v0 = col_value<M>(x[0], 0);
 #pragma unroll
@@ -219,6 +220,15 @@ __device__ __forceinline__ void mode_of(const RunConst& rc, int w, int& i, int& 
   k = w - i * rc.mz;
   kx = rc.alp * i;
   kz = (k <= rc.nz) ? rc.bet * k : -rc.bet * (rc.mz - k);
+}
+// Wavefront sequence index -> mode index i*mz + k.  Within a kx plane the
+// order is k = 0, then the pairs (q, mz - q): +kz and -kz share a Helmholtz
+// class, so the second wavefront of a pair finds the class's LU tables in
+// the cache the first one filled.
+__device__ __forceinline__ int mode_seq(const RunConst& rc, int seq) {
+  const int i = seq / rc.mz, s = seq - i * rc.mz;
+  const int k = s == 0 ? 0 : ((s & 1) ? (s + 1) >> 1 : rc.mz - (s >> 1));
+  return i * rc.mz + k;
 }
 __device__ __forceinline__ int helm_class(const RunConst& rc, int i, int k) {
   const int ka = (k <= rc.nz) ? k : rc.mz - k;
@@ -343,9 +353,10 @@ __global__ __launch_bounds__(256, 2) void k_phaseA(PhaseArgs a) {
   using T2 = typename std::conditional<sizeof(TS) == 4, float2, double2>::type;
   LOMA_STAGE2(M);
   const RunConst& rc = a.rc;
-  const int w = wave_uniform(a.mode_base + blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int seq = wave_uniform(a.mode_base + blockIdx.x * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
-  if (w >= a.mode_base + a.nmodes) return;
+  if (seq >= a.mode_base + a.nmodes) return;
+  const int w = mode_seq(rc, seq);
   int i, k;
   double kx, kz;
   mode_of(rc, w, i, k, kx, kz);
@@ -679,9 +690,10 @@ __global__ __launch_bounds__(256) void k_phaseC2(PhaseArgs a) {
   using T2 = typename std::conditional<sizeof(TS) == 4, float2, double2>::type;
   LOMA_STAGE(M);
   const RunConst& rc = a.rc;
-  const int w = wave_uniform(a.mode_base + blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int seq = wave_uniform(a.mode_base + blockIdx.x * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
-  if (w >= a.mode_base + a.nmodes) return;
+  if (seq >= a.mode_base + a.nmodes) return;
+  const int w = mode_seq(rc, seq);
   int i, k;
   double kx, kz;
   mode_of(rc, w, i, k, kx, kz);

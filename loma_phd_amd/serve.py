@@ -164,7 +164,7 @@ class Server:
         pool = (cf.ThreadPoolExecutor(max_workers=self.workers) if self.workers > 0
                 else _Inline())
         outs = {r.rid: Outcome(r.rid) for r in reqs}
-        lat, dev_ms = [], []
+        lat, dev_ms, lat_grid = [], [], []
         pending = deque(reqs)          # not yet arrived
         waiting: deque = deque()       # arrived, in arrival order: (req, prep future)
         starting: list = []            # admitted, set-up in flight: (_Live, future)
@@ -246,6 +246,8 @@ class Server:
                     busy = True
                     lat.append(t - lv.t_begin)
                     dev_ms.append(ms)
+                    c = lv.req.cfg
+                    lat_grid.append(f"{c.mgalx}x{c.my}x{c.mgalz}")
                     lv.out.steps_done += 1
                     gp_steps += lv.req.points
                     if nonfinite:
@@ -284,6 +286,12 @@ class Server:
             pool.shutdown(wait=True)
         wall = self.clock() - t0
         lat_ms = np.asarray(lat) * 1e3
+        by_grid = {}
+        for g in sorted(set(lat_grid), key=lambda x: [int(v) for v in x.split("x")]):
+            sel = np.asarray([q == g for q in lat_grid])
+            by_grid[g] = dict(steps=int(sel.sum()),
+                              p50_ms=float(np.percentile(lat_ms[sel], 50)),
+                              p99_ms=float(np.percentile(lat_ms[sel], 99)))
         return dict(
             outcomes=outs,
             wall_s=wall,
@@ -293,6 +301,7 @@ class Server:
             p50_ms=float(np.percentile(lat_ms, 50)) if lat_ms.size else math.nan,
             p99_ms=float(np.percentile(lat_ms, 99)) if lat_ms.size else math.nan,
             device_p50_ms=float(np.percentile(dev_ms, 50)) if dev_ms else math.nan,
+            by_grid=by_grid,
             peak_bytes=int(peak_used),
             peak_active=int(peak_active),
             counts={s: sum(1 for o in outs.values() if o.status == s)

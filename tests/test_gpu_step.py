@@ -1,8 +1,8 @@
 """GPU parity of the full RK3 time advance against the FP64 oracle.
 
 Same seeded initial condition on both sides; the HIP path (FP64) must stay
-within 1e-10 relative L2 of the oracle per field after 10 steps (the
-north-star tolerance), and the mixed FP32/FP64 path within 1e-5.
+within 1e-10 relative L2 of the oracle per field after 10 steps, and the
+mixed FP32/FP64 path within 1e-5.
 """
 import numpy as np
 import pytest
