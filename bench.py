@@ -207,8 +207,12 @@ def bench_served(window, rate=2.0, seed=11, max_slots=32, workers=4):
     free, _ = torch.cuda.mem_get_info()
     budget = int(0.8 * free)
     reqs = poisson_requests(seed, rate, window)
-    rep = Server(HipBackend(), mem_budget=budget, max_slots=max_slots,
-                 workers=workers).run(reqs, window=window)
+    be = HipBackend()
+    try:
+        rep = Server(be, mem_budget=budget, max_slots=max_slots,
+                     workers=workers).run(reqs, window=window)
+    finally:
+        be.close()
     return dict(window_s=window, arrival_rate_per_s=rate, requests=len(reqs),
                 gpts_per_s=round(rep["gp_steps_per_s"], 1), steps=rep["steps"],
                 p50_ms=round(rep["p50_ms"], 3), p99_ms=round(rep["p99_ms"], 3),
@@ -216,6 +220,9 @@ def bench_served(window, rate=2.0, seed=11, max_slots=32, workers=4):
                 by_grid={g: {k: (round(v, 3) if isinstance(v, float) else v)
                              for k, v in d.items()} for g, d in rep["by_grid"].items()},
                 peak_active=rep["peak_active"], peak_bytes=rep["peak_bytes"],
+                slow_loop_passes=rep["slow_loop_passes"],
+                slowest_loop_pass_ms=round(rep["slowest_loop_pass_ms"], 1),
+                reuse="finished runs kept per grid and re-targeted",
                 budget_bytes=budget, outcomes=rep["counts"], precision="mixed",
                 latency="host wall time from step enqueue to observed completion")
 

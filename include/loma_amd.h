@@ -135,6 +135,18 @@ int loma_helmholtz_dev(loma_run *run, const double *f, double *x,
                        const double *rK, double shift, int ncols);
 /* inty8 (laps.f:484) on one real column of my doubles */
 int loma_inty8_dev(loma_run *run, const double *du, double *u);
+/* visc (trunk/mixing/src/laps.f:535-857) with the mixing-layer boundary
+ * functions (laps.f:893-1060), for ncols mode columns: implicit viscous
+ * step of Omega_y and phi (Helmholtz shift rk1[col], Dirichlet) and v
+ * ((D2 - rk2[col]) v = phi, sqrt(rk2) v -+ dv/dy = 0 at the bottom/top).
+ * In/out roles as the reference's: vorwk in forcing, out omega; phiwk in
+ * forcing, out phi; v and dvdy in the history terms, out the new v, dv/dy;
+ * vor, phi out dalbe*solution + dtri*forcing + dtxi*(v, dvdy) (dalbe != 0)
+ * or the solution.  rk1, rk2: device double[ncols]. */
+int loma_visc_dev(loma_run *run, double *vorwk, double *vor, double *phi,
+                  double *v, double *dvdy, double *phiwk, const double *rk1,
+                  const double *rk2, double dalbe, double dtri, double dtxi,
+                  int ncols);
 
 /* ---- per-operator entry points (host pointers, float, reference shapes) */
 int loma_deryr2(loma_run *run, const float *u, float *du, int m);   /* cfdiff.f:518 */
@@ -148,6 +160,11 @@ int loma_solvepsi(loma_run *run, const float *phi, float *v, float *dvdy,
 /* fourz (fou3D.f:80) on nrows complex rows of length mgalz, FFTPACK
  * convention: iopt<0 forward scaled by 1/n with e^{-i}, iopt>=0 inverse. */
 int loma_fourz_dev(loma_run *run, double *c, int nrows, int iopt);
+/* visc with the reference's argument list and real(4) (2,my) columns
+ * (trunk/mixing/src/laps.f:535; the trailing integer flag is unused) */
+int loma_visc(loma_run *run, float *vorwk, float *vor, float *phi, float *v,
+              float *dvdy, float *phiwk, double rk1, double rk2, double dalbe,
+              double dtri, double dtxi);
 
 /* ---- batched banded solves in isolation ------------------------------- */
 /* nsys independent pentadiagonal systems of size n >= 3 (row-major band
@@ -160,6 +177,14 @@ int loma_fourz_dev(loma_run *run, double *c, int nrows, int iopt);
  * per device and grown on demand. */
 int loma_band5_solve_dev(const double *band, void *rhs, double *x, int n,
                          int nsys, int rhs_fp32, void *stream);
+
+/* Re-target a run at new physics without re-allocating: Reynolds number,
+ * Prandtl number, CFL number and first time step (the grid, precision and
+ * flags stay).  The run must be uploaded again before it advances.  For a
+ * request loop that keeps finished runs of a grid for the next request of
+ * the same grid.  No reference counterpart (initcr, main.f:461-800, sets
+ * these once per job). */
+int loma_set_physics(loma_run *run, double Re, double pr, double cfl, double dt0);
 
 /* Transform tile shapes in use: y entries per z-transform workgroup (J) and
  * lines per x-line workgroup (L).  No reference counterpart. */

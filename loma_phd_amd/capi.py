@@ -29,6 +29,7 @@ EXPORTS = [
     "loma_inty8_dev", "loma_deryr2", "loma_deryyr2", "loma_lapvdvhom", "loma_solvepsi",
     "loma_fourz_dev", "loma_band5_solve_dev", "loma_advance_many", "loma_advance_async",
     "loma_tiles", "loma_footprint", "loma_step_begin", "loma_step_poll",
+    "loma_visc_dev", "loma_visc", "loma_set_physics",
     "loma_sync", "loma_stream", "loma_profile", "loma_profile_read",
 ]
 KERNEL_CLASSES = ["phaseA", "zfft_inv", "xline", "zfft_fwd", "phaseC", "finish", "phaseC2"]
@@ -78,6 +79,9 @@ def lib():
             "loma_solvepsi_dev": ([_vp, _vp, _vp, _vp, _vp, i], i),
             "loma_helmholtz_dev": ([_vp, _vp, _vp, _vp, d, i], i),
             "loma_inty8_dev": ([_vp, _vp, _vp], i),
+            "loma_visc_dev": ([_vp] * 9 + [d, d, d, i], i),
+            "loma_visc": ([_vp] + [_fp] * 6 + [d] * 5, i),
+            "loma_set_physics": ([_vp, d, d, d, d], i),
             "loma_deryr2": ([_vp, _fp, _fp, i], i),
             "loma_deryyr2": ([_vp, _fp, _fp, i], i),
             "loma_lapvdvhom": ([_vp, _fp, _fp, _fp, d], i),
@@ -250,6 +254,12 @@ class Run:
         check(lib().loma_scalars(self._h, _d(o)))
         return dict(dt=o[0], time=o[1], dtr=o[2], istep=int(o[3]), nan=int(o[4]))
 
+    def set_physics(self, Re, pr=0.7, cfl=0.5, dt0=1e-4):
+        """New Re, Pr, CFL and first dt on the same grid and buffers; upload a
+        state before advancing."""
+        check(lib().loma_set_physics(self._h, float(Re), float(pr), float(cfl), float(dt0)))
+        self.Re, self.pr = Re, pr
+
     def use_graph(self, on=True):
         check(lib().loma_use_graph(self._h, 1 if on else 0))
 
@@ -271,6 +281,26 @@ class Run:
 
     def helmholtz_dev(self, f, x, rK, shift, ncols):
         check(lib().loma_helmholtz_dev(self._h, f, x, rK, float(shift), ncols))
+
+    def visc_dev(self, vorwk, vor, phi, v, dvdy, phiwk, rk1, rk2, dalbe, dtri, dtxi, ncols):
+        """visc (trunk/mixing/src/laps.f:535) on ncols device columns; rk1, rk2
+        device double arrays."""
+        check(lib().loma_visc_dev(self._h, vorwk, vor, phi, v, dvdy, phiwk, rk1, rk2,
+                                  float(dalbe), float(dtri), float(dtxi), ncols))
+
+    def visc(self, vorwk, v, dvdy, phiwk, rk1, rk2, dalbe, dtri, dtxi):
+        """visc with the reference's real(4) (2,my) columns on the host;
+        returns dict(vorwk, vor, phi, v, dvdy, phiwk) as complex64."""
+        a = {k: np.ascontiguousarray(x, dtype=np.complex64).copy() for k, x in
+             (("vorwk", vorwk), ("v", v), ("dvdy", dvdy), ("phiwk", phiwk))}
+        vor = np.zeros_like(a["vorwk"])
+        phi = np.zeros_like(a["vorwk"])
+        fp = lambda x: x.view(np.float32).ctypes.data_as(_fp)
+        check(lib().loma_visc(self._h, fp(a["vorwk"]), fp(vor), fp(phi), fp(a["v"]),
+                              fp(a["dvdy"]), fp(a["phiwk"]), float(rk1), float(rk2),
+                              float(dalbe), float(dtri), float(dtxi)))
+        return dict(vorwk=a["vorwk"], vor=vor, phi=phi, v=a["v"], dvdy=a["dvdy"],
+                    phiwk=a["phiwk"])
 
     def inty8_dev(self, du, u):
         check(lib().loma_inty8_dev(self._h, du, u))

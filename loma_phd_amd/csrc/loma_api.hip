@@ -829,6 +829,25 @@ static int build_graph(loma_run* r, int icfl) {
   return LOMA_OK;
 }
 
+int loma_set_physics(loma_run* r, double Re, double pr, double cfl, double dt0) {
+  if (!r || !(Re > 0.0) || !(pr > 0.0) || !(cfl > 0.0) || !(dt0 > 0.0)) {
+    set_error("loma_set_physics: Re, pr, cfl and dt0 must be positive");
+    return LOMA_ERR_ARG;
+  }
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
+  if (r->stream) HIPCHK(hipStreamSynchronize(r->stream));
+  r->Re = Re; r->pr = pr; r->cfl = cfl; r->dt0 = dt0;
+  r->rc.ire = 1.0 / Re;
+  r->rc.peclet = Re * pr;
+  r->rc.cfl = cfl;
+  // the step graphs hold the run constants by value
+  for (int q = 0; q < 2; ++q)
+    if (r->gexec[q]) { hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
+  ensemble_forget(r);
+  r->uploaded = false;  // a new state must follow
+  return LOMA_OK;
+}
+
 int loma_use_graph(loma_run* r, int enable) {
   if (!r) return LOMA_ERR_ARG;
   r->use_graph = enable != 0;
@@ -1127,6 +1146,35 @@ int loma_helmholtz_dev(loma_run* r, const double* f, double* x, const double* rK
                        double shift, int ncols) {
   return solve_cols(r, f, x, nullptr, rK, shift, ncols, 2);
 }
+int loma_visc_dev(loma_run* r, double* vorwk, double* vor, double* phi, double* v,
+                  double* dvdy, double* phiwk, const double* rk1, const double* rk2,
+                  double dalbe, double dtri, double dtxi, int ncols) {
+  if (!r || ncols < 0 || !vorwk || !vor || !phi || !v || !dvdy || !phiwk || !rk1 || !rk2)
+    return LOMA_ERR_ARG;
+  if (ncols == 0) return LOMA_OK;
+  double *lu = nullptr, *bc = nullptr;
+  const size_t L = (size_t)r->lu_stride * ncols;
+  HIPCHK(hipMalloc(&lu, sizeof(double) * 2 * L));
+  if (hipMalloc(&bc, sizeof(double) * 4 * (size_t)ncols) != hipSuccess) {
+    hipFree(lu);
+    set_error("loma_visc_dev: allocation failed");
+    return LOMA_ERR_OOM;
+  }
+  int rc = LOMA_OK;
+  if (launch_helm_factor(r->dt21, r->dt22, r->my, r->M, rk1, ncols, lu, r->stream) ||
+      launch_helm_factor(r->dt21, r->dt22, r->my, r->M, rk2, ncols, lu + L, r->stream) ||
+      launch_helm_bc(r->M, r->T, lu + L, bc, ncols, r->stream) ||
+      launch_visc(r->M, r->T, lu, lu + L, bc, vorwk, vor, phi, v, dvdy, phiwk, rk2, dalbe,
+                  dtri, dtxi, ncols, r->stream) ||
+      hipStreamSynchronize(r->stream) != hipSuccess) {
+    set_error("loma_visc_dev: device failure");
+    rc = LOMA_ERR_DEVICE;
+  }
+  hipFree(lu);
+  hipFree(bc);
+  return rc;
+}
+
 int loma_inty8_dev(loma_run* r, const double* du, double* u) {
   if (!r) return LOMA_ERR_ARG;
   LCHK(launch_inty8(r->M, r->T, du, u, r->stream));
@@ -1181,6 +1229,31 @@ int loma_lapvdvhom(loma_run* r, const float* phi, float* v, float* dvdy, double 
 }
 int loma_solvepsi(loma_run* r, const float* phi, float* v, float* dvdy, double rK) {
   return host_solve(r, phi, v, dvdy, rK, 1);
+}
+
+int loma_visc(loma_run* r, float* vorwk, float* vor, float* phi, float* v, float* dvdy,
+              float* phiwk, double rk1, double rk2, double dalbe, double dtri, double dtxi) {
+  if (!r || !vorwk || !vor || !phi || !v || !dvdy || !phiwk) return LOMA_ERR_ARG;
+  const size_t n = (size_t)2 * r->my;
+  float* cols[6] = {vorwk, vor, phi, v, dvdy, phiwk};
+  std::vector<double> h(6 * n + 2);
+  for (int c = 0; c < 6; ++c)
+    for (size_t q = 0; q < n; ++q) h[c * n + q] = cols[c][q];
+  h[6 * n] = rk1;
+  h[6 * n + 1] = rk2;
+  double* d = nullptr;
+  HIPCHK(hipMalloc(&d, h.size() * sizeof(double)));
+  int rc = hipMemcpy(d, h.data(), h.size() * 8, hipMemcpyHostToDevice) == hipSuccess
+               ? loma_visc_dev(r, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n,
+                               d + 6 * n, d + 6 * n + 1, dalbe, dtri, dtxi, 1)
+               : LOMA_ERR_DEVICE;
+  if (rc == LOMA_OK && hipMemcpy(h.data(), d, h.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = LOMA_ERR_DEVICE;
+  hipFree(d);
+  if (rc == LOMA_OK)
+    for (int c = 0; c < 6; ++c)
+      for (size_t q = 0; q < n; ++q) cols[c][q] = (float)h[c * n + q];
+  return rc;
 }
 
 int loma_fourz_dev(loma_run* r, double* c, int nrows, int iopt) {

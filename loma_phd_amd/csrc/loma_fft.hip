@@ -132,8 +132,16 @@ __device__ __forceinline__ void dft5(T2* v, int s) {
 #ifndef LOMA_ZF_WPE
 #define LOMA_ZF_WPE 4
 #endif
-__device__ __forceinline__ int lds_i(int i) { return LOMA_LDS_PAD ? i + (i >> 3) : i; }
-__host__ __device__ inline int lds_entries(int n) { return LOMA_LDS_PAD ? n + (n >> 3) + 1 : n; }
+template <int PAD = LOMA_LDS_PAD>
+__device__ __forceinline__ int lds_i(int i) { return PAD ? i + (i >> 3) : i; }
+__host__ __device__ inline int lds_entries(int n, int pad = LOMA_LDS_PAD) {
+  return pad ? n + (n >> 3) + 1 : n;
+}
+// The FP64 z transforms use the padded layout: measured at 512x513x256 it
+// takes them from 1.43 / 1.89 ms to 1.28 / 1.70 ms (inverse / forward); the
+// FP32 ones and the x lines do not gain (FP32 x lines lose 6%).
+template <typename T>
+constexpr int zf_pad() { return sizeof(T) == 8 ? 1 : 0; }
 
 template <int R, typename T, typename T2>
 __device__ __forceinline__ void stockham_item(const T2* __restrict__ src, T2* __restrict__ dst,
@@ -197,7 +205,7 @@ __device__ T2* fft_rows(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
 // In-place variant for one LDS buffer: each pass loads its items into
 // registers, synchronises, writes the butterflies back, synchronises.
 // nrows * N <= blockDim.x * E (E = complex entries per thread).
-template <int R, typename T, typename T2>
+template <int R, typename T, typename T2, int PAD>
 __device__ __forceinline__ void radix_pass_ip(T2* a, int N, int nrows, int Ns, int tstep,
                                               const T2* __restrict__ tw, int s, T2* v,
                                               int nitems) {
@@ -209,7 +217,7 @@ __device__ __forceinline__ void radix_pass_ip(T2* a, int N, int nrows, int Ns, i
     if (it < total) {
       const int row = it / nb, j = it - row * nb;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[k * R + r] = a[lds_i(row * N + j + r * nb)];
+      for (int r = 0; r < R; ++r) v[k * R + r] = a[lds_i<PAD>(row * N + j + r * nb)];
     }
   }
   __syncthreads();
@@ -235,13 +243,13 @@ __device__ __forceinline__ void radix_pass_ip(T2* a, int N, int nrows, int Ns, i
       if (R == 8) dft8<T, T2>(w, s);
       const int idxD = row * N + (j / Ns) * Ns * R + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) a[lds_i(idxD + r * Ns)] = w[r];
+      for (int r = 0; r < R; ++r) a[lds_i<PAD>(idxD + r * Ns)] = w[r];
     }
   }
   __syncthreads();
 }
 
-template <typename T, typename T2, int E>
+template <typename T, typename T2, int E, int PAD>
 __device__ void fft_rows_ip(T2* a, int N, int nrows, const FftPlan& pl,
                             const T2* __restrict__ tw, int s) {
   T2 v[E + 8];
@@ -250,11 +258,11 @@ __device__ void fft_rows_ip(T2* a, int N, int nrows, const FftPlan& pl,
     const int R = pl.rad[p];
     const int tstep = N / (Ns * R);
     switch (R) {
-      case 2: radix_pass_ip<2, T, T2>(a, N, nrows, Ns, tstep, tw, s, v, (E + 1) / 2); break;
-      case 3: radix_pass_ip<3, T, T2>(a, N, nrows, Ns, tstep, tw, s, v, (E + 2) / 3); break;
-      case 4: radix_pass_ip<4, T, T2>(a, N, nrows, Ns, tstep, tw, s, v, (E + 3) / 4); break;
-      case 5: radix_pass_ip<5, T, T2>(a, N, nrows, Ns, tstep, tw, s, v, (E + 4) / 5); break;
-      default: radix_pass_ip<8, T, T2>(a, N, nrows, Ns, tstep, tw, s, v, (E + 7) / 8); break;
+      case 2: radix_pass_ip<2, T, T2, PAD>(a, N, nrows, Ns, tstep, tw, s, v, (E + 1) / 2); break;
+      case 3: radix_pass_ip<3, T, T2, PAD>(a, N, nrows, Ns, tstep, tw, s, v, (E + 2) / 3); break;
+      case 4: radix_pass_ip<4, T, T2, PAD>(a, N, nrows, Ns, tstep, tw, s, v, (E + 3) / 4); break;
+      case 5: radix_pass_ip<5, T, T2, PAD>(a, N, nrows, Ns, tstep, tw, s, v, (E + 4) / 5); break;
+      default: radix_pass_ip<8, T, T2, PAD>(a, N, nrows, Ns, tstep, tw, s, v, (E + 7) / 8); break;
     }
     Ns *= R;
   }
@@ -291,7 +299,7 @@ __device__ __forceinline__ T2* fft_rows_ct(T2* a, T2* b, int nrows, const T2* __
   }
 }
 
-template <typename T, typename T2, int E, int N, int Ns, int NT>
+template <typename T, typename T2, int E, int N, int Ns, int NT, int PAD>
 __device__ __forceinline__ void fft_rows_ip_ct(T2* a, int nrows, const T2* __restrict__ tw,
                                                int s, T2* v) {
   if constexpr (Ns < N) {
@@ -308,7 +316,7 @@ __device__ __forceinline__ void fft_rows_ip_ct(T2* a, int nrows, const T2* __res
       if (it < total) {
         const int row = it / nb, j = it - row * nb;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[k * R + r] = a[lds_i(row * N + j + r * nb)];
+        for (int r = 0; r < R; ++r) v[k * R + r] = a[lds_i<PAD>(row * N + j + r * nb)];
       }
     }
     __syncthreads();
@@ -334,11 +342,11 @@ __device__ __forceinline__ void fft_rows_ip_ct(T2* a, int nrows, const T2* __res
         if (R == 8) dft8<T, T2>(w, s);
         const int idxD = row * N + (j / Ns) * Ns * R + jm;
 #pragma unroll
-        for (int r = 0; r < R; ++r) a[lds_i(idxD + r * Ns)] = w[r];
+        for (int r = 0; r < R; ++r) a[lds_i<PAD>(idxD + r * Ns)] = w[r];
       }
     }
     __syncthreads();
-    fft_rows_ip_ct<T, T2, E, N, Ns * R, NT>(a, nrows, tw, s, v);
+    fft_rows_ip_ct<T, T2, E, N, Ns * R, NT, PAD>(a, nrows, tw, s, v);
   }
 }
 
@@ -351,14 +359,14 @@ __device__ __forceinline__ T2* fft_any(T2* a, T2* b, int N, int nrows, const Fft
   else
     return fft_rows<T, T2>(a, b, N, nrows, pl, tw, s);
 }
-template <typename T, typename T2, int E, int NN, int NT>
+template <typename T, typename T2, int E, int NN, int NT, int PAD>
 __device__ __forceinline__ void fft_any_ip(T2* a, int N, int nrows, const FftPlan& pl,
                                            const T2* __restrict__ tw, int s) {
   if constexpr (NN > 0) {
     T2 v[E + 8];
-    fft_rows_ip_ct<T, T2, E, NN, 1, NT>(a, nrows, tw, s, v);
+    fft_rows_ip_ct<T, T2, E, NN, 1, NT, PAD>(a, nrows, tw, s, v);
   } else {
-    fft_rows_ip<T, T2, E>(a, N, nrows, pl, tw, s);
+    fft_rows_ip<T, T2, E, PAD>(a, N, nrows, pl, tw, s);
   }
 }
 
@@ -374,6 +382,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = NN ? NN : a.rc.mgalz, J = a.J, my = a.rc.my, myp = a.rc.myp, mz = a.rc.mz,
             nz = a.rc.nz;
+  constexpr int PAD = zf_pad<T>();
   T2* A = (T2*)smem;
   const int nyb = (my + J - 1) / J;
   const int i = blockIdx.x / nyb, j0 = (blockIdx.x - i * nyb) * J;
@@ -384,20 +393,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
   const int nzero = N - mz;
   for (int it = threadIdx.x; it < nzero * J; it += blockDim.x) {
     const int q = it / J, jj = it - q * J;
-    A[lds_i(jj * N + nz + 1 + q)] = T2{0, 0};
+    A[lds_i<PAD>(jj * N + nz + 1 + q)] = T2{0, 0};
   }
   for (int it = threadIdx.x; it < mz * J; it += blockDim.x) {
     const int k = it / J, jj = it - k * J, j = j0 + jj;
     const int z = (k <= nz) ? k : k + N - mz;
     T2 v = T2{0, 0};
     if (j < my) v = src[((size_t)i * mz + k) * myp + j];
-    A[lds_i(jj * N + z)] = v;
+    A[lds_i<PAD>(jj * N + z)] = v;
   }
   __syncthreads();
-  fft_any_ip<T, T2, E, NN, 256>(A, N, J, a.plz, (const T2*)a.twz, +1);
+  fft_any_ip<T, T2, E, NN, 256, PAD>(A, N, J, a.plz, (const T2*)a.twz, +1);
   for (int it = threadIdx.x; it < N * J; it += blockDim.x) {
     const int z = it / J, jj = it - z * J, j = j0 + jj;
-    if (j < my) dst[((size_t)i * N + z) * myp + j] = A[lds_i(jj * N + z)];
+    if (j < my) dst[((size_t)i * N + z) * myp + j] = A[lds_i<PAD>(jj * N + z)];
   }
 }
 
@@ -407,6 +416,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = NN ? NN : a.rc.mgalz, J = a.J, my = a.rc.my, myp = a.rc.myp, mz = a.rc.mz,
             nz = a.rc.nz;
+  constexpr int PAD = zf_pad<T>();
   T2* A = (T2*)smem;
   const int nyb = (my + J - 1) / J;
   const int i = blockIdx.x / nyb, j0 = (blockIdx.x - i * nyb) * J;
@@ -417,16 +427,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
     const int z = it / J, jj = it - z * J, j = j0 + jj;
     T2 v = T2{0, 0};
     if (j < my) v = src[((size_t)i * N + z) * myp + j];
-    A[lds_i(jj * N + z)] = v;
+    A[lds_i<PAD>(jj * N + z)] = v;
   }
   __syncthreads();
-  fft_any_ip<T, T2, E, NN, 256>(A, N, J, a.plz, (const T2*)a.twz, -1);
+  fft_any_ip<T, T2, E, NN, 256, PAD>(A, N, J, a.plz, (const T2*)a.twz, -1);
   const T sc = (T)(1.0 / (double)N);
   for (int it = threadIdx.x; it < mz * J; it += blockDim.x) {
     const int k = it / J, jj = it - k * J, j = j0 + jj;
     if (j < my) {
       const int z = (k <= nz) ? k : k + N - mz;
-      T2 v = A[lds_i(jj * N + z)];
+      T2 v = A[lds_i<PAD>(jj * N + z)];
       v.x *= sc;
       v.y *= sc;
       dst[((size_t)i * mz + k) * myp + j] = v;
@@ -883,7 +893,7 @@ __global__ __launch_bounds__(256) void k_fourz(FftPlan pl, const double2* tw, do
 // launchers
 // ------------------------------------------------------------------------
 static size_t zfft_lds(bool fp32, const FftArgs& a) {
-  return (size_t)lds_entries(a.J * a.rc.mgalz) * (fp32 ? 8 : 16);
+  return (size_t)lds_entries(a.J * a.rc.mgalz, fp32 ? 0 : 1) * (fp32 ? 8 : 16);
 }
 static size_t xline_lds(bool fp32, const FftArgs& a) {
   return (size_t)2 * lds_entries((a.L / 2) * a.rc.mgalx) * (fp32 ? 8 : 16);

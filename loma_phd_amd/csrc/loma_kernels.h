@@ -98,6 +98,10 @@ int launch_lapvdvhom(int M, const YTab& T, const double* lu, const double* bc,
                      const double* phi, double* v, double* dv, const double* rK,
                      int ncols, int mode, hipStream_t st);
 int launch_inty8(int M, const YTab& T, const double* du, double* u, hipStream_t st);
+int launch_visc(int M, const YTab& T, const double* lu1, const double* lu2, const double* bc2,
+                double* vorwk, double* vor, double* phi, double* v, double* dvdy,
+                double* phiwk, const double* rk2, double dalbe, double dtri, double dtxi,
+                int ncols, hipStream_t st);
 
 int launch_phaseA(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
 int launch_phaseC(int M, bool fp32, const PhaseArgs& a, hipStream_t st);

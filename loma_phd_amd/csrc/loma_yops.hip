@@ -17,6 +17,10 @@ int launch_helm_bc_m(const YTab&, const double*, int, double*, int, hipStream_t)
 template <int M>
 int launch_inty8_m(const YTab&, const double*, double*, hipStream_t);
 template <int M>
+int launch_visc_m(const YTab&, const double*, const double*, int, const double*, double*,
+                  double*, double*, double*, double*, double*, const double*, double, double,
+                  double, int, hipStream_t);
+template <int M>
 int launch_phase_m(int, bool, const PhaseArgs&, hipStream_t);
 
 RkCoef rk_coefficients() {
@@ -69,6 +73,14 @@ int launch_helm_bc(int M, const YTab& T, const double* lu, double* bc, int nclas
 }
 int launch_inty8(int M, const YTab& T, const double* du, double* u, hipStream_t st) {
   LOMA_DISPATCH_M(M, launch_inty8_m<MM>(T, du, u, st));
+}
+int launch_visc(int M, const YTab& T, const double* lu1, const double* lu2, const double* bc2,
+                double* vorwk, double* vor, double* phi, double* v, double* dvdy,
+                double* phiwk, const double* rk2, double dalbe, double dtri, double dtxi,
+                int ncols, hipStream_t st) {
+  if (ncols <= 0) return 0;
+  LOMA_DISPATCH_M(M, launch_visc_m<MM>(T, lu1, lu2, 5 * WAVE * MM, bc2, vorwk, vor, phi, v,
+                                       dvdy, phiwk, rk2, dalbe, dtri, dtxi, ncols, st));
 }
 int launch_phaseA(int M, bool fp32, const PhaseArgs& a, hipStream_t st) {
   if (a.nmodes <= 0) return 0;

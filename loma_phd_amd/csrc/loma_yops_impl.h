@@ -175,6 +175,104 @@ __global__ __launch_bounds__(256) void k_lapvdvhom(YTab T, const double* lu, int
   }
 }
 
+// visc (trunk/mixing/src/laps.f:535-857) for a batch of mode columns, with
+// the mixing-layer boundary functions (laps.f:893-1060).  lu1 / lu2: per
+// column LU of the Dirichlet Helmholtz matrices dt22 - rk1 dt21 and
+// dt22 - rk2 dt21; bc2: per column {w0.h1, w0.h2, w1.h1, w1.h2} of the rk2
+// homogeneous solutions (w0, w1 the d/dy functionals at the ends).
+//   omega = (D2 - rk1)^{-1} vorwk with omega = 0 at both ends (C_ome = 1,
+//     C_dyo = 0 make the 2x2 boundary system the identity);
+//   phi   = (D2 - rk1)^{-1} phiwk, zero at both ends (C_phi rows bot2/top2);
+//   v     = (D2 - rk2)^{-1} phi with the Robin rows sqrt(rk2) v - dv/dy = 0
+//     (bottom) and sqrt(rk2) v + dv/dy = 0 (top): of the reference's 4x4
+//     system the phi rows fix the phi coefficients to zero and leave this
+//     2x2 for the end values (c3, c4) of v, solved here in closed form;
+//   vor = dalbe omega + dtri vorwk + dtxi v_in, phi likewise with dvdy_in
+//     (dalbe != 0), else omega and phi; v = dvdy = 0 when rk2 = 0.
+template <int M>
+__global__ __launch_bounds__(256) void k_visc(YTab T, const double* lu1, const double* lu2,
+                                              int lu_stride, const double* bc2,
+                                              double2* vorwk, double2* vor, double2* phi,
+                                              double2* v, double2* dvdy, double2* phiwk,
+                                              const double* rk2, double dalbe, double dtri,
+                                              double dtxi, int ncols) {
+  LOMA_STAGE(M);
+  const int col = wave_uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  if (col >= ncols) return;
+  const int my = T.my, r0 = lane * M;
+  const size_t o = (size_t)col * my;
+  const double* L1 = lu1 + (size_t)col * lu_stride;
+  const double* L2 = lu2 + (size_t)col * lu_stride;
+  double f[2][M], x[2][M];
+  // omega and vor
+  load_col<M>(vorwk + o, my, lane, st, f);
+  op_helm<M, 2>(T, L1, lane, f, x);
+  store_col<M>(vorwk + o, my, lane, st, x);
+  if (dalbe != 0.0) {
+    double g[2][M];
+    load_col<M>(v + o, my, lane, st, g);
+#pragma unroll
+    for (int t = 0; t < M; ++t)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) x[c][t] = dalbe * x[c][t] + dtri * f[c][t] + dtxi * g[c][t];
+  }
+  store_col<M>(vor + o, my, lane, st, x);
+  // phi
+  load_col<M>(phiwk + o, my, lane, st, f);
+  op_helm<M, 2>(T, L1, lane, f, x);
+  store_col<M>(phiwk + o, my, lane, st, x);
+  {
+    double g[2][M];
+    if (dalbe != 0.0) {
+      load_col<M>(dvdy + o, my, lane, st, g);
+#pragma unroll
+      for (int t = 0; t < M; ++t)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) g[c][t] = dalbe * x[c][t] + dtri * f[c][t] + dtxi * g[c][t];
+    } else {
+#pragma unroll
+      for (int t = 0; t < M; ++t) g[0][t] = x[0][t], g[1][t] = x[1][t];
+    }
+    store_col<M>(phi + o, my, lane, st, g);
+  }
+  // v, dv/dy
+  const double k2 = rk2[col];
+  if (k2 == 0.0) {
+#pragma unroll
+    for (int t = 0; t < M; ++t) f[0][t] = f[1][t] = 0.0;
+    store_col<M>(v + o, my, lane, st, f);
+    store_col<M>(dvdy + o, my, lane, st, f);
+    return;
+  }
+  {
+    double vp[2][M], dp[4];
+    op_helm<M, 2>(T, L2, lane, x, vp);
+    bd_dots<M>(T, lane, vp, dp);  // {w0.vp re, im, w1.vp re, im}
+    const double* bc = bc2 + (size_t)col * 4;
+    const double k = sqrt(k2);
+    const double m11 = k - bc[0], m12 = -bc[1], m21 = bc[2], m22 = k + bc[3];
+    const double det = m11 * m22 - m12 * m21;
+    const double c3r = (dp[0] * m22 + m12 * dp[2]) / det;
+    const double c4r = (-m11 * dp[2] - m21 * dp[0]) / det;
+    const double c3i = (dp[1] * m22 + m12 * dp[3]) / det;
+    const double c4i = (-m11 * dp[3] - m21 * dp[1]) / det;
+    // second Dirichlet solve with end values (c3, c4)
+    bmatvec<M, 2, 2>(ytab<M>(T, YT_A2, 5, lane), lane, x, vp);
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+      const int r = r0 + t;
+      if (r == 0) { vp[0][t] = c3r; vp[1][t] = c3i; }
+      if (r == my - 1) { vp[0][t] = c4r; vp[1][t] = c4i; }
+    }
+    lsolve<M, 2, 2>(tab(L2, 5, 3, lane), lane, vp);
+    usolve<M, 2, 2>(tab(L2, 5, 1, lane), tab(L2, 5, 0, lane), lane, vp);
+    store_col<M>(v + o, my, lane, st, vp);
+    op_dy1<M, 2>(T, lane, vp, f);
+    store_col<M>(dvdy + o, my, lane, st, f);
+  }
+}
+
 // inty8 (laps.f:484-575): u = B1^{-1} A1 (du/fmap) - value at row 0, with
 // the last row of B1 replaced by the gauge row u(my-1) = 0 (the reference's
 // B1 is singular up to rounding; this is the well-posed form of the same
@@ -748,6 +846,17 @@ int launch_lapvdvhom_m(const YTab& T, const double* lu, int lu_stride, const dou
   return (int)hipGetLastError();
 }
 template <int M>
+int launch_visc_m(const YTab& T, const double* lu1, const double* lu2, int lu_stride,
+                  const double* bc2, double* vorwk, double* vor, double* phi, double* v,
+                  double* dvdy, double* phiwk, const double* rk2, double dalbe, double dtri,
+                  double dtxi, int ncols, hipStream_t st) {
+  dim3 g((ncols + 3) / 4), b(256);
+  hipLaunchKernelGGL((k_visc<M>), g, b, 0, st, T, lu1, lu2, lu_stride, bc2, (double2*)vorwk,
+                     (double2*)vor, (double2*)phi, (double2*)v, (double2*)dvdy,
+                     (double2*)phiwk, rk2, dalbe, dtri, dtxi, ncols);
+  return (int)hipGetLastError();
+}
+template <int M>
 int launch_helm_bc_m(const YTab& T, const double* lu, int lu_stride, double* bc, int nclass,
                      hipStream_t st) {
   dim3 g((nclass + 3) / 4), b(256);
@@ -783,6 +892,10 @@ int launch_phase_m(int which, bool fp32, const PhaseArgs& a, hipStream_t st) {
   template int launch_helm_bc_m<MV>(const YTab&, const double*, int, double*, int,        \
                                     hipStream_t);                                         \
   template int launch_inty8_m<MV>(const YTab&, const double*, double*, hipStream_t);      \
+  template int launch_visc_m<MV>(const YTab&, const double*, const double*, int,          \
+                                 const double*, double*, double*, double*, double*,       \
+                                 double*, double*, const double*, double, double, double, \
+                                 int, hipStream_t);                                       \
   template int launch_phase_m<MV>(int, bool, const PhaseArgs&, hipStream_t);
 
 }  // namespace loma

@@ -27,6 +27,7 @@ from __future__ import annotations
 import concurrent.futures as cf
 import dataclasses
 import math
+import threading
 import time
 from collections import deque
 from typing import Any, Callable
@@ -63,14 +64,87 @@ class Outcome:
     state: Any = None
 
 
-class HipBackend:
-    """Runs requests on the GPU through the C ABI (capi.Run)."""
+class RunPool:
+    """Finished runs kept for the next request of the same grid.
 
-    def __init__(self, keep_state: bool = False, use_graph: bool = True):
+    A kept run holds its device memory (grid tables, Helmholtz factors,
+    buffers), so a later request of its grid skips set-up, allocation and
+    release.  `make_room` is the admission test: it takes a kept run of the
+    request's grid if there is one (its bytes move from the pool to the
+    caller), or else frees kept runs of other grids, least recently kept
+    first, until the request's footprint fits beside what stays kept.
+    """
+
+    def __init__(self, close: Callable[[Any], None]):
+        self._close = close
+        self._idle: list = []      # (key, run, bytes), oldest first
+        self._taken: dict = {}     # rid -> run reserved for that request
+        self._lock = threading.Lock()
+
+    def idle_bytes(self) -> int:
+        with self._lock:
+            return sum(b for _, _, b in self._idle)
+
+    def make_room(self, rid, key, nbytes: int, room: int) -> bool:
+        with self._lock:
+            for q, (k, run, b) in enumerate(self._idle):
+                if k == key:
+                    del self._idle[q]
+                    self._taken[rid] = run
+                    return True
+            evict = []
+            while self._idle and nbytes + sum(b for _, _, b in self._idle) > room:
+                evict.append(self._idle.pop(0)[1])
+            fits = nbytes + sum(b for _, _, b in self._idle) <= room
+        for run in evict:
+            self._close(run)
+        return fits
+
+    def take(self, rid):
+        with self._lock:
+            return self._taken.pop(rid, None)
+
+    def put(self, key, run, nbytes: int):
+        with self._lock:
+            self._idle.append((key, run, nbytes))
+
+    def close_all(self):
+        with self._lock:
+            runs = [r for _, r, _ in self._idle] + list(self._taken.values())
+            self._idle, self._taken = [], {}
+        for run in runs:
+            self._close(run)
+
+
+class HipBackend:
+    """Runs requests on the GPU through the C ABI (capi.Run).  With `reuse`,
+    finished runs are kept per grid (RunPool) and re-targeted at the next
+    request of that grid (loma_set_physics + upload)."""
+
+    def __init__(self, keep_state: bool = False, use_graph: bool = True, reuse: bool = True):
         from . import capi  # loads libloma_hip.so; fails loudly if it is missing
         self.capi = capi
         self.keep_state = keep_state
         self.use_graph = use_graph
+        self.pool = RunPool(lambda run: run.close()) if reuse else None
+
+    def key(self, req: Request):
+        c = req.cfg
+        return (c.mgalx, c.my, c.mgalz, req.precision, self.flags(req), c.Lx, c.Lz, c.Ly,
+                c.mesh_ratio, c.ncfl)
+
+    def idle_bytes(self) -> int:
+        return self.pool.idle_bytes() if self.pool else 0
+
+    def make_room(self, req: Request, nbytes: int, room: int) -> bool:
+        if self.pool is None:
+            return nbytes <= room
+        return self.pool.make_room(req.rid, self.key(req), nbytes, room)
+
+    def close(self):
+        """Free the kept runs."""
+        if self.pool:
+            self.pool.close_all()
 
     def flags(self, req: Request) -> int:
         return self.capi.LOMA_FLAG_SCALAR_Z if req.cfg.scalar_z else 0
@@ -85,9 +159,14 @@ class HipBackend:
 
     def start(self, req: Request, ic):
         c = req.cfg
-        run = self.capi.Run(c.mgalx, c.my, c.mgalz, ic["y"], c.Re, c.alp, c.bet, pr=c.pr,
-                            cfl=c.cfl, ncfl=c.ncfl, dt0=c.dt0, precision=req.precision,
-                            flags=self.flags(req))
+        run = self.pool.take(req.rid) if self.pool else None
+        if run is not None:
+            run.set_physics(c.Re, c.pr, c.cfl, c.dt0)
+        else:
+            run = self.capi.Run(c.mgalx, c.my, c.mgalz, ic["y"], c.Re, c.alp, c.bet, pr=c.pr,
+                                cfl=c.cfl, ncfl=c.ncfl, dt0=c.dt0, precision=req.precision,
+                                flags=self.flags(req))
+        run.req_key = self.key(req)
         try:
             run.upload(ic["vor"], ic["phi"], ic["psi"], ic["scal"], ic["u00"], ic["v00"],
                        ic["w00"], z=ic["z"] if c.scalar_z else None)
@@ -105,9 +184,15 @@ class HipBackend:
 
     def finish(self, run):
         try:
-            return run.download() if self.keep_state else None
-        finally:
+            out = run.download() if self.keep_state else None
+        except Exception:
             run.close()
+            raise
+        if self.pool is not None:
+            self.pool.put(run.req_key, run, run.bytes)
+        else:
+            run.close()
+        return out
 
     def free(self, run):
         run.close()
@@ -155,6 +240,18 @@ class Server:
         self.clock = clock
         self.idle_sleep = idle_sleep
 
+    def _idle(self) -> int:
+        f = getattr(self.b, "idle_bytes", None)
+        return f() if f else 0
+
+    def _room(self, req, nbytes: int, used: int) -> bool:
+        """Admission test: the backend's own (it may hold finished runs for
+        reuse and free them to make room), else plain budget arithmetic."""
+        mk = getattr(self.b, "make_room", None)
+        if mk is None:
+            return used + nbytes <= self.budget
+        return mk(req, nbytes, self.budget - used)
+
     def run(self, requests, window: float | None = None) -> dict:
         """Serve `requests` (arrival times relative to the start).  With
         `window`, admission stops at that time and runs still resident are cut
@@ -176,9 +273,15 @@ class Server:
         gp_steps = 0
         t0 = self.clock()
         closed = False
+        gaps = []          # loop passes that took longer than 20 ms (s)
+        t_prev = t0
         try:
             while pending or waiting or starting or active or retiring:
-                now = self.clock() - t0
+                t_now = self.clock()
+                if t_now - t_prev > 0.02:
+                    gaps.append(t_now - t_prev)
+                t_prev = t_now
+                now = t_now - t0
                 busy = False
                 if window is not None and now >= window and not closed:
                     closed = True
@@ -207,11 +310,11 @@ class Server:
                         busy = True
                         continue
                     nres = len(active) + len(starting)
-                    if nres >= self.max_slots or used + o.footprint > self.budget:
+                    if nres >= self.max_slots or not self._room(r, o.footprint, used):
                         break
                     waiting.popleft()
                     used += o.footprint
-                    peak_used = max(peak_used, used)
+                    peak_used = max(peak_used, used + self._idle())
                     o.admit_s = now
                     lv = _Live(r, o)
                     starting.append((lv, pool.submit(self.b.start, r, f.result())))
@@ -302,6 +405,8 @@ class Server:
             p99_ms=float(np.percentile(lat_ms, 99)) if lat_ms.size else math.nan,
             device_p50_ms=float(np.percentile(dev_ms, 50)) if dev_ms else math.nan,
             by_grid=by_grid,
+            slow_loop_passes=len(gaps),
+            slowest_loop_pass_ms=float(max(gaps) * 1e3) if gaps else 0.0,
             peak_bytes=int(peak_used),
             peak_active=int(peak_active),
             counts={s: sum(1 for o in outs.values() if o.status == s)

@@ -689,6 +689,140 @@ void orc_helmholtz_dirichlet(const orc_grid *g, const double *f, double *x,
   free(a);
 }
 
+/* Boundary functions of visc for the mixing layer (trunk/mixing/src/
+ * laps.f:893-1060): phi/v rows  C_phi*phi + C_vvv*v + C_dyv*dv/dy = BC  (bot1,
+ * bot2 at y(1); top1, top2 at y(my)); omega rows C_ome*ome + C_dyo*dome/dy. */
+typedef struct { double phi, vvv, dyv, bc; } visc_row;
+static visc_row visc_bot1(double rk2) { visc_row r = {0.0, sqrt(rk2), -1.0, 0.0}; return r; }
+static visc_row visc_bot2(double rk2) { visc_row r = {1.0, 0.0, 0.0, 0.0}; (void)rk2; return r; }
+static visc_row visc_top1(double rk2) { visc_row r = {0.0, sqrt(rk2), 1.0, 0.0}; return r; }
+static visc_row visc_top2(double rk2) { visc_row r = {1.0, 0.0, 0.0, 0.0}; (void)rk2; return r; }
+static const double VISC_OME = 1.0, VISC_DYO = 0.0, VISC_BCO = 0.0;  /* bot and top */
+
+/* calcrhsd1 + banbks7(prem1): d/deta without fmap (laps.f:61-111) */
+static void d1raw(const orc_grid *g, const double *u, double *du) {
+  band_matvec(g->dt12, g->my, 3, u, 1, du);
+  orc_banbks7(g->prem1, g->my, du);
+}
+
+/* visc, trunk/mixing/src/laps.f:535-857 */
+void orc_visc(const orc_grid *g, double *vorwk, double *vor, double *phi, double *v,
+              double *dvdy, double *phiwk, double rk1, double rk2, double dalbe,
+              double dtri, double dtxi) {
+  const int my = g->my, L = my - 1;
+  const double f0 = g->fmap[0], fL = g->fmap[L];
+  double *a1 = helm_lhs(g, rk1);  /* wk1 with rk1, laps.f:596-610 */
+  double *buf = calloc((size_t)16 * my, sizeof(double));
+  double *phipr = buf, *phipi = buf + my, *phi1 = buf + 2 * my, *phi2 = buf + 3 * my;
+  double *vpr = buf + 4 * my, *vpi = buf + 5 * my, *v1 = buf + 6 * my, *v2 = buf + 7 * my;
+  double *v3 = buf + 8 * my, *v4 = buf + 9 * my, *dvpr = buf + 10 * my, *dvpi = buf + 11 * my;
+  double *dv1 = buf + 12 * my, *dv2 = buf + 13 * my, *dv3 = buf + 14 * my, *dv4 = buf + 15 * my;
+  /* omega (laps.f:615-695) */
+  for (int j = 0; j < 2 * my; j++) vor[j] = vorwk[j];
+  rhslap(g, vorwk, 2, phipr);
+  rhslap(g, vorwk + 1, 2, phipi);
+  phi1[0] = 1.0;
+  phi2[L] = 1.0;
+  orc_banbks(a1, my, phipr);
+  orc_banbks(a1, my, phipi);
+  orc_banbks(a1, my, phi1);
+  orc_banbks(a1, my, phi2);
+  d1raw(g, phipr, dvpr);
+  d1raw(g, phipi, dvpi);
+  d1raw(g, phi1, dv1);
+  d1raw(g, phi2, dv2);
+  {
+    double AA[4], co[4];
+    AA[0] = VISC_OME + VISC_DYO * dv1[0] * f0;   /* AA(1,1) */
+    AA[1] = VISC_DYO * dv1[L] * fL;              /* AA(2,1) */
+    AA[2] = VISC_DYO * dv2[0] * f0;              /* AA(1,2) */
+    AA[3] = VISC_OME + VISC_DYO * dv2[L] * fL;   /* AA(2,2) */
+    co[0] = VISC_BCO - VISC_DYO * dvpr[0] * f0;
+    co[1] = VISC_BCO - VISC_DYO * dvpr[L] * fL;
+    co[2] = VISC_BCO - VISC_DYO * dvpi[0] * f0;
+    co[3] = VISC_BCO - VISC_DYO * dvpi[L] * fL;
+    orc_gaussj(AA, 2, co, 2);
+    for (int j = 0; j < my; j++) {
+      vorwk[2 * j] = phipr[j] + co[0] * phi1[j] + co[1] * phi2[j];
+      vorwk[2 * j + 1] = phipi[j] + co[2] * phi1[j] + co[3] * phi2[j];
+    }
+  }
+  for (int j = 0; j < 2 * my; j++)
+    vor[j] = dalbe != 0.0 ? dalbe * vorwk[j] + dtri * vor[j] + dtxi * v[j] : vorwk[j];
+  /* phi and v (laps.f:700-856) */
+  for (int j = 0; j < 2 * my; j++) phi[j] = phiwk[j];
+  rhslap(g, phiwk, 2, phipr);
+  rhslap(g, phiwk + 1, 2, phipi);
+  orc_banbks(a1, my, phipr);
+  orc_banbks(a1, my, phipi);
+  double *a2 = helm_lhs(g, rk2);  /* wk1 with rk2, laps.f:730-742 */
+  rhslap(g, phipr, 1, vpr);
+  rhslap(g, phipi, 1, vpi);
+  rhslap(g, phi1, 1, v1);
+  rhslap(g, phi2, 1, v2);
+  memset(v3, 0, my * sizeof(double));
+  memset(v4, 0, my * sizeof(double));
+  v3[0] = 1.0;
+  v4[L] = 1.0;
+  double *vs[6] = {vpr, vpi, v1, v2, v3, v4}, *ds[6] = {dvpr, dvpi, dv1, dv2, dv3, dv4};
+  for (int q = 0; q < 6; q++) orc_banbks(a2, my, vs[q]);
+  for (int q = 0; q < 6; q++) d1raw(g, vs[q], ds[q]);
+  {
+    /* rows bot1, bot2, top1, top2; columns phi1, phi2, v3, v4 (laps.f:793-816) */
+    const visc_row rb1 = visc_bot1(rk2), rb2 = visc_bot2(rk2);
+    const visc_row rt1 = visc_top1(rk2), rt2 = visc_top2(rk2);
+    double AA[16], co[8];
+#define AAM(r, c) AA[(r) + 4 * (c)]
+    AAM(0, 0) = rb1.phi + rb1.dyv * dv1[0] * f0;
+    AAM(0, 1) = rb1.dyv * dv2[0] * f0;
+    AAM(0, 2) = rb1.vvv + rb1.dyv * dv3[0] * f0;
+    AAM(0, 3) = rb1.dyv * dv4[0] * f0;
+    AAM(1, 0) = rb2.phi + rb2.dyv * dv1[0] * f0;
+    AAM(1, 1) = rb2.dyv * dv2[0] * f0;
+    AAM(1, 2) = rb2.vvv + rb2.dyv * dv3[0] * f0;
+    AAM(1, 3) = rb2.dyv * dv4[0] * f0;
+    AAM(2, 0) = rt1.dyv * dv1[L] * fL;
+    AAM(2, 1) = rt1.phi + rt1.dyv * dv2[L] * fL;
+    AAM(2, 2) = rt1.dyv * dv3[L] * fL;
+    AAM(2, 3) = rt1.vvv + rt1.dyv * dv4[L] * fL;
+    AAM(3, 0) = rt2.dyv * dv1[L] * fL;
+    AAM(3, 1) = rt2.phi + rt2.dyv * dv2[L] * fL;
+    AAM(3, 2) = rt2.dyv * dv3[L] * fL;
+    AAM(3, 3) = rt2.vvv + rt2.dyv * dv4[L] * fL;
+#undef AAM
+    co[0] = rb1.bc - rb1.dyv * dvpr[0] * f0;
+    co[1] = rb2.bc - rb2.dyv * dvpr[0] * f0;
+    co[2] = rt1.bc - rt1.dyv * dvpr[L] * fL;
+    co[3] = rt2.bc - rt2.dyv * dvpr[L] * fL;
+    co[4] = rb1.bc - rb1.dyv * dvpi[0] * f0;
+    co[5] = rb2.bc - rb2.dyv * dvpi[0] * f0;
+    co[6] = rt1.bc - rt1.dyv * dvpi[L] * fL;
+    co[7] = rt2.bc - rt2.dyv * dvpi[L] * fL;
+    orc_gaussj(AA, 4, co, 2);
+    for (int j = 0; j < my; j++) {
+      phiwk[2 * j] = phipr[j] + co[0] * phi1[j] + co[1] * phi2[j];
+      phiwk[2 * j + 1] = phipi[j] + co[4] * phi1[j] + co[5] * phi2[j];
+    }
+    for (int j = 0; j < 2 * my; j++)
+      phi[j] = dalbe != 0.0 ? dalbe * phiwk[j] + dtri * phi[j] + dtxi * dvdy[j] : phiwk[j];
+    for (int j = 0; j < my; j++) {
+      if (rk2 != 0.0) {
+        dvdy[2 * j] = (dvpr[j] + co[0] * dv1[j] + co[1] * dv2[j] + co[2] * dv3[j] +
+                       co[3] * dv4[j]) * g->fmap[j];
+        dvdy[2 * j + 1] = (dvpi[j] + co[4] * dv1[j] + co[5] * dv2[j] + co[6] * dv3[j] +
+                           co[7] * dv4[j]) * g->fmap[j];
+        v[2 * j] = vpr[j] + co[0] * v1[j] + co[1] * v2[j] + co[2] * v3[j] + co[3] * v4[j];
+        v[2 * j + 1] = vpi[j] + co[4] * v1[j] + co[5] * v2[j] + co[6] * v3[j] + co[7] * v4[j];
+      } else {
+        dvdy[2 * j] = dvdy[2 * j + 1] = v[2 * j] = v[2 * j + 1] = 0.0;
+      }
+    }
+  }
+  free(a2);
+  free(buf);
+  free(a1);
+}
+
 /* ------------------------------------------------------------------------ */
 /* FFT: recursive mixed-radix decimation in time with an explicit DFT for   */
 /* each prime factor.  sign = -1 forward (e^{-i}), +1 backward.  Unscaled.  */

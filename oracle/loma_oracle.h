@@ -66,6 +66,14 @@ void orc_inty8_ref(const orc_grid *g, const double *du, double *u);
  * solves (D2 - (rK + shift)) x = f with homogeneous Dirichlet ends */
 void orc_helmholtz_dirichlet(const orc_grid *g, const double *f, double *x,
                              double rK, double shift);
+/* visc (trunk/mixing/src/laps.f:535-857) with the mixing-layer boundary
+ * functions (laps.f:893-1060): implicit viscous step of Omega_y (Helmholtz
+ * with shift rk1, Dirichlet) and of phi/v (Helmholtz rk1 for phi, then
+ * (D2 - rk2) v = phi with the 4x4 boundary system).  Complex columns (2,my);
+ * arguments and their in/out roles as the reference's. */
+void orc_visc(const orc_grid *g, double *vorwk, double *vor, double *phi, double *v,
+              double *dvdy, double *phiwk, double rk1, double rk2, double dalbe,
+              double dtri, double dtxi);
 
 /* FFTPACK-convention transforms (cftsingle.f:2, rftsingle.f:25).
  * cft: m complex vectors of length n, stride between vectors = n complex

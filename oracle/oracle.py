@@ -58,6 +58,7 @@ def lib():
         for n in ("orc_lapvdvhom", "orc_solvepsi"):
             getattr(L, n).argtypes = [vp, _dp, _dp, _dp, ctypes.c_double]
         L.orc_helmholtz_dirichlet.argtypes = [vp, _dp, _dp, ctypes.c_double, ctypes.c_double]
+        L.orc_visc.argtypes = [vp] + [_dp] * 6 + [ctypes.c_double] * 5
         for n in ("orc_cft", "orc_rft"):
             getattr(L, n).argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_state_create.restype = vp
@@ -169,6 +170,20 @@ class Grid:
         lib().orc_helmholtz_dirichlet(self._h, _p(f.view(np.float64)), _p(x.view(np.float64)),
                                       float(rK), float(shift))
         return x
+
+    def visc(self, vorwk, v, dvdy, phiwk, rk1, rk2, dalbe, dtri, dtxi):
+        """visc (trunk/mixing/src/laps.f:535) on complex columns (my,): returns
+        the six outputs (vorwk = omega, vor, phi, v, dvdy, phiwk = phi)."""
+        a = {k: np.array(x, dtype=np.complex128) for k, x in
+             (("vorwk", vorwk), ("v", v), ("dvdy", dvdy), ("phiwk", phiwk))}
+        vor = np.zeros_like(a["vorwk"])
+        phi = np.zeros_like(a["vorwk"])
+        lib().orc_visc(self._h, _p(a["vorwk"].view(np.float64)), _p(vor.view(np.float64)),
+                       _p(phi.view(np.float64)), _p(a["v"].view(np.float64)),
+                       _p(a["dvdy"].view(np.float64)), _p(a["phiwk"].view(np.float64)),
+                       float(rk1), float(rk2), float(dalbe), float(dtri), float(dtxi))
+        return dict(vorwk=a["vorwk"], vor=vor, phi=phi, v=a["v"], dvdy=a["dvdy"],
+                    phiwk=a["phiwk"])
 
     def rK(self):
         """rK[i,k] = kx^2 + kz^2 in the reference mode order."""

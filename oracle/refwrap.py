@@ -113,3 +113,49 @@ def solve(name, phi, rK):
     getattr(lib(), name)(_f(phi.view(np.float32)), _f(v.view(np.float32)),
                          _f(dv.view(np.float32)), float(rK))
     return v, dv
+
+
+# ---- the reference's visc (trunk/mixing/src/laps.f:535), its own library ----
+MIXLIB = os.path.join(HERE, "_ref", "libloma_ref_mix.so")
+_mix = None
+
+
+def mix_available() -> bool:
+    return os.path.exists(MIXLIB)
+
+
+def mix_lib():
+    global _mix
+    if _mix is None:
+        L = ctypes.CDLL(MIXLIB)
+        L.ref_mix_dims.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        L.ref_mix_set.argtypes = [_dp] * 5
+        L.ref_mix_visc.argtypes = [_fp] * 6 + [ctypes.c_double] * 5
+        _mix = L
+    return _mix
+
+
+def mix_my() -> int:
+    d = (ctypes.c_int * 1)()
+    mix_lib().ref_mix_dims(d)
+    return d[0]
+
+
+def mix_set(prem1, dt12, dt21, dt22, fmap):
+    """Matrices in (my, nb) row-major form (the reference's (nb, my))."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (prem1, dt12, dt21, dt22, fmap)]
+    mix_lib().ref_mix_set(*[_d(a) for a in arrs])
+
+
+def mix_visc(vorwk, v, dvdy, phiwk, rk1, rk2, dalbe, dtri, dtxi):
+    """visc on complex64 columns (my,); returns dict of the six outputs."""
+    a = {k: np.ascontiguousarray(x, dtype=np.complex64).copy()
+         for k, x in (("vorwk", vorwk), ("v", v), ("dvdy", dvdy), ("phiwk", phiwk))}
+    vor = np.zeros_like(a["vorwk"])
+    phi = np.zeros_like(a["vorwk"])
+    mix_lib().ref_mix_visc(_f(a["vorwk"].view(np.float32)), _f(vor.view(np.float32)),
+                           _f(phi.view(np.float32)), _f(a["v"].view(np.float32)),
+                           _f(a["dvdy"].view(np.float32)), _f(a["phiwk"].view(np.float32)),
+                           float(rk1), float(rk2), float(dalbe), float(dtri), float(dtxi))
+    return dict(vorwk=a["vorwk"], vor=vor, phi=phi, v=a["v"], dvdy=a["dvdy"],
+                phiwk=a["phiwk"])

@@ -96,6 +96,39 @@ def test_helmholtz_and_inty8(gpu, oracle_mod):
     assert rel_l2(ud.cpu().numpy(), g.inty8(du)) < 1e-11
 
 
+@pytest.mark.parametrize("my,dalbe", [(129, 0.4), (257, 0.0), (513, 0.3)])
+def test_visc(gpu, oracle_mod, my, dalbe):
+    """visc (trunk/mixing/src/laps.f:535) on a batch of mode columns against
+    the oracle's restatement of the reference routine (which follows its 4x4
+    boundary system), rk1 and rk2 per column, one column with rk2 = 0."""
+    import torch
+    run, y = _mk(my)
+    g = oracle_mod.Grid(48, my, 48, y, 300.0, 0.25, 0.25)
+    rng = np.random.default_rng(11 + my)
+    n = 12
+    c = lambda: rng.standard_normal((n, my)) + 1j * rng.standard_normal((n, my))
+    vorwk, v, dvdy, phiwk = c(), c(), c(), c()
+    rk2 = np.concatenate([[0.0], 10 ** rng.uniform(-2, 1.5, n - 1)])
+    rk1 = rk2 + 10 ** rng.uniform(1, 4, n)
+    dtri, dtxi = 0.25, -0.1
+    d = {k: _dev(torch, a) for k, a in (("vorwk", vorwk), ("vor", np.zeros_like(vorwk)),
+                                        ("phi", np.zeros_like(vorwk)), ("v", v),
+                                        ("dvdy", dvdy), ("phiwk", phiwk), ("rk1", rk1),
+                                        ("rk2", rk2))}
+    print(f"visc my={my} dalbe={dalbe}", flush=True)
+    run.visc_dev(*[d[k].data_ptr() for k in ("vorwk", "vor", "phi", "v", "dvdy", "phiwk",
+                                             "rk1", "rk2")], dalbe, dtri, dtxi, n)
+    got = {k: d[k].cpu().numpy() for k in ("vorwk", "vor", "phi", "v", "dvdy", "phiwk")}
+    for q in range(n):
+        ref = g.visc(vorwk[q], v[q], dvdy[q], phiwk[q], rk1[q], rk2[q], dalbe, dtri, dtxi)
+        for k in got:
+            if rk2[q] == 0.0 and k in ("v", "dvdy"):
+                assert np.all(got[k][q] == 0), (q, k)
+                continue
+            e = rel_l2(got[k][q], ref[k])
+            assert e < 1e-10, (q, k, e)
+
+
 @pytest.mark.parametrize("n", [64, 96, 192, 256, 384])
 def test_fourz(gpu, oracle_mod, n):
     import torch

@@ -77,10 +77,13 @@ def test_alone_equals_mix(gpu, precision):
         arrival = [0.0, 0.0, 0.05, 0.1]
         return [Request(q, arrival[q], c[q], steps[q], precision) for q in range(4)]
 
-    be = HipBackend(keep_state=True)
+    be = HipBackend(keep_state=True, reuse=False)
     alone = Server(be, mem_budget=64 << 30).run(reqs()[:1])
     assert alone["outcomes"][0].status == "done"
+    # the mix reuses finished runs of a grid for later requests of that grid
+    be = HipBackend(keep_state=True, reuse=True)
     mix = Server(be, mem_budget=64 << 30, max_slots=8).run(reqs())
+    be.close()
     print("mix report", {k: v for k, v in mix.items() if k != "outcomes"}, flush=True)
     for q in range(4):
         assert mix["outcomes"][q].status == "done", (q, mix["outcomes"][q])
@@ -100,8 +103,43 @@ def test_budget_holds_back_admission(gpu):
     c = MixingLayerConfig(mgalx=64, my=129, mgalz=64, s=2.0, Re=160.0, seed=21)
     fp = capi.footprint(64, 129, 64, 1, capi.LOMA_FLAG_SCALAR_Z)
     reqs = [Request(0, 0.0, c, 6, 1), Request(1, 0.0, c, 6, 1)]
-    rep = Server(HipBackend(), mem_budget=fp + fp // 2).run(reqs)
+    be = HipBackend()
+    rep = Server(be, mem_budget=fp + fp // 2).run(reqs)
+    be.close()
     o = rep["outcomes"]
     assert o[0].status == o[1].status == "done"
     assert rep["peak_active"] == 1 and rep["peak_bytes"] == fp
     assert o[1].admit_s >= o[0].end_s
+
+
+def test_reused_run_equals_fresh(gpu):
+    """A finished run re-targeted at a new request of its grid (other Re,
+    density ratio and seed; loma_set_physics + upload) ends in the same state
+    as a fresh run of that request; a budget for one run of either grid makes
+    the pool free the kept run of the other grid."""
+    from loma_phd_amd import capi
+    from loma_phd_amd.grid import MixingLayerConfig
+    from loma_phd_amd.serve import HipBackend, Request, Server
+    print("reuse", flush=True)
+    a = MixingLayerConfig(mgalx=64, my=129, mgalz=64, s=2.0, Re=160.0, seed=31)
+    b = MixingLayerConfig(mgalx=64, my=129, mgalz=64, s=6.0, Re=700.0, seed=32)
+    fresh = HipBackend(keep_state=True, reuse=False)
+    ref = Server(fresh, mem_budget=64 << 30).run([Request(0, 0.0, b, 7, 1)])
+    be = HipBackend(keep_state=True, reuse=True)
+    rep = Server(be, mem_budget=64 << 30, max_slots=1).run(
+        [Request(0, 0.0, a, 5, 1), Request(1, 0.0, b, 7, 1)])
+    assert be.idle_bytes() > 0
+    be.close()
+    x, y = rep["outcomes"][1].state, ref["outcomes"][0].state
+    for f in ("vor", "phi", "psi", "scal", "u00", "v00", "w00", "z"):
+        assert np.array_equal(x[f], y[f]), f
+    # room for one run: the second grid evicts the first grid's kept run
+    c = MixingLayerConfig(mgalx=96, my=129, mgalz=64, s=3.0, Re=300.0, seed=33)
+    fa = capi.footprint(64, 129, 64, 1, capi.LOMA_FLAG_SCALAR_Z)
+    fc = capi.footprint(96, 129, 64, 1, capi.LOMA_FLAG_SCALAR_Z)
+    be = HipBackend()
+    rep = Server(be, mem_budget=max(fa, fc) + min(fa, fc) // 2).run(
+        [Request(0, 0.0, a, 3, 1), Request(1, 0.0, c, 3, 1)])
+    assert all(o.status == "done" for o in rep["outcomes"].values())
+    assert rep["peak_bytes"] <= max(fa, fc) + min(fa, fc) // 2
+    be.close()

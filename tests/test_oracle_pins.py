@@ -174,6 +174,82 @@ def test_operators_against_live_reference(oracle_mod):
     assert rel_l2(v, vr) < 2e-7 and rel_l2(dv, dvr) < 2e-7
 
 
+def _visc_inputs(my, seed):
+    rng = np.random.default_rng(seed)
+    c = lambda: (rng.standard_normal(my) + 1j * rng.standard_normal(my)).astype(np.complex64)
+    return c(), c(), c(), c()  # vorwk, v, dvdy, phiwk
+
+
+@pytest.mark.parametrize("rk1,rk2,dalbe", [(850.0, 0.35, 0.4), (40.0, 2.5, 0.0),
+                                           (120.0, 0.0, 0.3)])
+def test_visc_against_live_reference(oracle_mod, rk1, rk2, dalbe):
+    """orc_visc against the reference's own visc (trunk/mixing/src/laps.f:535,
+    built in place) on the same FD matrices, at the mixing code's my = 256.
+    The reference's columns are real(4): the oracle runs on the same
+    FP32-rounded inputs, and the bar is FP32 output rounding."""
+    from oracle import refwrap as R
+    from loma_phd_amd.grid import tanh_mesh
+    if not R.mix_available():
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    my = R.mix_my()
+    y = tanh_mesh(my, 20.0)
+    g = oracle_mod.Grid(48, my, 48, y, 160.0, 0.5, 0.5)
+    M = g.matrices()
+    R.mix_set(M["prem1"], M["dt12"], M["dt21"], M["dt22"], M["fmap"])
+    vorwk, v, dvdy, phiwk = _visc_inputs(my, 7)
+    args = (rk1, rk2, dalbe, 0.25, -0.1)
+    ref = R.mix_visc(vorwk, v, dvdy, phiwk, *args)
+    orc = g.visc(vorwk, v, dvdy, phiwk, *args)
+    for k in ("vorwk", "vor", "phi", "phiwk", "v", "dvdy"):
+        if rk2 == 0.0 and k in ("v", "dvdy"):
+            assert np.all(orc[k] == 0) and np.all(ref[k] == 0)
+            continue
+        e = rel_l2(orc[k], ref[k])
+        assert e < 2e-6, (k, e)
+
+
+def test_visc_satisfies_its_equations(oracle_mod):
+    """What the mathematics fixes for visc with the mixing-layer boundary
+    functions: Dirichlet Helmholtz rk1 for omega and phi, (D2 - rk2) v = phi
+    with the potential-flow Robin rows sqrt(rk2) v -+ dv/dy = 0 at the
+    bottom/top, dvdy = d/dy v, and the output combinations."""
+    from loma_phd_amd.grid import tanh_mesh
+    my = 129
+    y = tanh_mesh(my, 15.0)
+    g = oracle_mod.Grid(48, my, 48, y, 160.0, 0.5, 0.5)
+    M = g.matrices()
+    vorwk, v0, dvdy0, phiwk = (x.astype(np.complex128) for x in _visc_inputs(my, 3))
+    rk1, rk2, dalbe, dtri, dtxi = 300.0, 0.8, 0.5, 0.3, -0.2
+    o = g.visc(vorwk, v0, dvdy0, phiwk, rk1, rk2, dalbe, dtri, dtxi)
+
+    def band_apply(B, x):
+        p = B.shape[1] // 2
+        out = np.zeros_like(x)
+        for j in range(my):
+            for d in range(B.shape[1]):
+                c = j - p + d
+                if 0 <= c < my:
+                    out[j] += B[j, d] * x[c]
+        return out
+
+    def helm_res(x, f, rk):
+        r = band_apply(M["dt22"], x) - rk * band_apply(M["dt21"], x) - band_apply(M["dt21"], f)
+        return np.abs(r[1:-1]).max() / np.abs(band_apply(M["dt21"], f)).max()
+
+    om, ph = o["vorwk"], o["phiwk"]
+    assert helm_res(om, vorwk, rk1) < 1e-11 and om[0] == 0 and om[-1] == 0
+    assert helm_res(ph, phiwk, rk1) < 1e-11 and abs(ph[0]) < 1e-14 and abs(ph[-1]) < 1e-14
+    vv, dv = o["v"], o["dvdy"]
+    assert helm_res(vv, ph, rk2) < 1e-10
+    assert rel_l2(dv, g.deryr2(vv)) < 1e-12
+    k = np.sqrt(rk2)
+    scale = np.abs(dv).max()
+    assert abs(k * vv[0] - dv[0]) < 1e-10 * scale
+    assert abs(k * vv[-1] + dv[-1]) < 1e-10 * scale
+    assert rel_l2(o["vor"], dalbe * om + dtri * vorwk + dtxi * v0) < 1e-14
+    assert rel_l2(o["phi"], dalbe * ph + dtri * phiwk + dtxi * dvdy0) < 1e-14
+
+
 def test_poisson_solves_satisfy_discrete_system(oracle_mod):
     from loma_phd_amd.grid import tanh_mesh
     my = 97

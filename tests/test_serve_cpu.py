@@ -166,3 +166,25 @@ def test_footprint_matches_grid_scaling():
     assert m < a < z
     with pytest.raises(capi.LomaError):
         capi.footprint(64, 2000, 64, capi.LOMA_FP64)
+
+
+def test_run_pool_reuse_and_eviction():
+    from loma_phd_amd.serve import RunPool
+    closed = []
+    pool = RunPool(closed.append)
+    pool.put("g1", "run-a", 100)
+    pool.put("g2", "run-b", 200)
+    assert pool.idle_bytes() == 300
+    # a request of grid g1 takes the kept run whatever the room
+    assert pool.make_room(7, "g1", 100, room=0)
+    assert pool.take(7) == "run-a" and pool.idle_bytes() == 200
+    # grid g3 needs 250 of 300: the kept g2 run is freed
+    assert pool.make_room(8, "g3", 250, room=300)
+    assert closed == ["run-b"] and pool.idle_bytes() == 0 and pool.take(8) is None
+    # too large even with an empty pool
+    pool.put("g2", "run-c", 200)
+    assert not pool.make_room(9, "g4", 500, room=400)
+    assert closed == ["run-b", "run-c"]
+    pool.put("g5", "run-d", 10)
+    pool.close_all()
+    assert closed[-1] == "run-d" and pool.idle_bytes() == 0
