@@ -222,6 +222,7 @@ def bench_served(window, rate=2.0, seed=11, max_slots=32, workers=4):
                 peak_active=rep["peak_active"], peak_bytes=rep["peak_bytes"],
                 slow_loop_passes=rep["slow_loop_passes"],
                 slowest_loop_pass_ms=round(rep["slowest_loop_pass_ms"], 1),
+                slowest_call_ms={k: round(v, 1) for k, v in rep["slowest_call_ms"].items()},
                 reuse="finished runs kept per grid and re-targeted",
                 budget_bytes=budget, outcomes=rep["counts"], precision="mixed",
                 latency="host wall time from step enqueue to observed completion")

@@ -59,9 +59,12 @@ struct loma_run {
   FdOps fd;
   RunConst rc{};
   hipStream_t stream = nullptr;
-  // device allocations
-  std::vector<void*> allocs;
+  // device memory: one allocation (arena) carved by dalloc
+  void* arena = nullptr;
+  size_t arena_size = 0;
   size_t bytes = 0;
+  double* hstage = nullptr;  // pinned host staging of one complex field
+  size_t hstage_n = 0;       // doubles
   double* tab = nullptr;  // all per-row tables
   YTab T{};
   double* dt21 = nullptr;  // plain (my,5) row-major copies for the factor kernel
@@ -118,18 +121,21 @@ static int prof_end(loma_run* r, int cls, hipEvent_t a) {
   return 0;
 }
 
+// Every buffer of a run is carved from one device allocation, 256-byte
+// aligned: one hipMalloc (and one hipFree) per run instead of dozens, so
+// that setting runs up and tearing them down holds the runtime briefly while
+// other runs step.
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 template <typename P>
 static int dalloc(loma_run* r, P** p, size_t bytes) {
-  void* q = nullptr;
-  if (bytes == 0) bytes = 16;
-  hipError_t e = hipMalloc(&q, bytes);
-  if (e != hipSuccess) {
-    set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  const size_t a = align256(bytes == 0 ? 16 : bytes);
+  if (!r->arena || r->bytes + a > r->arena_size) {
+    set_error("loma_init: device arena exhausted (allocation plan out of step)");
     return LOMA_ERR_OOM;
   }
-  r->allocs.push_back(q);
-  r->bytes += bytes;
-  *p = (P*)q;
+  *p = (P*)((char*)r->arena + r->bytes);
+  r->bytes += a;
   return LOMA_OK;
 }
 
@@ -142,9 +148,9 @@ static int pick_M(int my) {
   return -1;
 }
 
-// Device bytes loma_init requests for a grid: the sum of its allocations, in
-// the order it makes them (kept in step with loma_init; a test compares this
-// with the byte count of a live run).
+// Device bytes loma_init requests for a grid: the sum of its allocations,
+// each rounded to 256 bytes (kept in step with loma_init; a test compares
+// this with the byte count of a live run).
 static size_t plan_bytes(int mgalx, int my, int mgalz, bool fp32, bool with_z) {
   const int M = pick_M(my);
   const size_t R = 64 * (size_t)M;
@@ -154,16 +160,31 @@ static size_t plan_bytes(int mgalx, int my, int mgalz, bool fp32, bool with_z) {
   const size_t S = np * mz * my, Sp = np * mz * myp, Wp = np * mgalz * myp;
   const size_t P = (size_t)mgalx * my * mgalz, es = fp32 ? 8 : 16;
   const size_t nsp = with_z ? 12 : 11, nst = with_z ? 7 : 6;
-  size_t b = 8 * (size_t)ytab_offset(YT_COUNT, M);  // per-row and scan tables
-  b += 2 * 8 * 5 * (size_t)my;                 // band copies
-  b += 8 * nclass + 8 * 5 * R * nclass + 32 * nclass;  // shifts, LU, Neumann constants
-  b += nst * S * 16;                           // spectral state and histories
-  b += nsp * (Sp + 2 * Wp) * es;               // transform buffers
-  b += (with_z ? 2 : 1) * P * 8;               // physical-space RK histories
-  b += (6 + 28) * 8 * (size_t)my;              // 00-mode work and mean profiles
-  b += sizeof(DevScalars);
-  b += (fp32 ? 8 : 16) * (size_t)(mgalz + mgalx) + 16 * (size_t)mgalz;  // twiddles
+  const size_t tw = fp32 ? 8 : 16;
+  size_t b = align256(8 * (size_t)ytab_offset(YT_COUNT, M));  // per-row and scan tables
+  b += 2 * align256(8 * 5 * (size_t)my);                       // band copies
+  b += align256(8 * nclass) + align256(8 * 5 * R * nclass) + align256(32 * nclass);
+  b += nst * align256(S * 16);                                 // state and histories
+  b += nsp * (align256(Sp * es) + 2 * align256(Wp * es));      // transform buffers
+  b += (with_z ? 2 : 1) * align256(P * 8);                     // physical RK histories
+  b += 6 * align256(8 * (size_t)my);                           // 00-mode work
+  b += 2 * align256(8 * 12 * (size_t)my) + 2 * align256(8 * 2 * (size_t)my);  // mean split
+  b += align256(sizeof(DevScalars));
+  b += align256(tw * mgalz) + align256(tw * mgalx) + align256(16 * (size_t)mgalz);  // twiddles
   return b;
+}
+
+// Pinned host staging for transfers of one complex field, grown on demand;
+// copies go on the run's own stream, so they hold the runtime only while
+// they are enqueued.
+static int stage_host(loma_run* r, size_t ndoubles) {
+  if (r->hstage_n >= ndoubles) return LOMA_OK;
+  if (r->hstage) hipHostFree(r->hstage);
+  r->hstage = nullptr;
+  r->hstage_n = 0;
+  HIPCHK(hipHostMalloc((void**)&r->hstage, ndoubles * sizeof(double), hipHostMallocDefault));
+  r->hstage_n = ndoubles;
+  return LOMA_OK;
 }
 
 extern "C" {
@@ -236,7 +257,8 @@ int loma_free(loma_run* r) {
     if (r->sev[q]) hipEventDestroy(r->sev[q]);
   }
   if (r->hsc) hipHostFree(r->hsc);
-  for (void* p : r->allocs) hipFree(p);
+  if (r->hstage) hipHostFree(r->hstage);
+  if (r->arena) hipFree(r->arena);
   if (r->owns_stream && r->stream) hipStreamDestroy(r->stream);
   delete r;
   return LOMA_OK;
@@ -285,6 +307,22 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   if (r->M < 0) { set_error("loma_init: my too large"); delete r; return LOMA_ERR_ARG; }
   int rc = fd_build(my, y, r->fd);
   if (rc) { set_error("loma_init: FD setup failed"); delete r; return rc; }
+  // the run's own non-blocking stream carries its set-up as well as its steps
+  if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("loma_init: stream creation failed"); delete r; return LOMA_ERR_DEVICE;
+  }
+  r->owns_stream = true;
+  r->arena_size = plan_bytes(mgalx, my, mgalz, r->fp32, r->with_z);
+  {
+    const hipError_t e = hipMalloc(&r->arena, r->arena_size);
+    if (e != hipSuccess) {
+      r->arena = nullptr;
+      set_error(std::string("loma_init: hipMalloc failed: ") + hipGetErrorString(e));
+      loma_free(r);
+      return LOMA_ERR_OOM;
+    }
+  }
+  hipStream_t sst = r->stream;
   // run constants
   RunConst& c = r->rc;
   c.mgalx = mgalx; c.my = my; c.mgalz = mgalz; c.mx = r->mx; c.mz = r->mz;
@@ -350,7 +388,8 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   lev[4] = scan_table(my, M, 3, false, f.loi.data(), nullptr, nullptr, H(YT_HLI));
   lev[5] = scan_table(my, M, 3, true, nullptr, f.upi.data(), f.invi.data(), H(YT_HUI));
   if ((rc = dalloc(r, &r->tab, tot * sizeof(double)))) { loma_free(r); return rc; }
-  if (hipMemcpy(r->tab, h.data(), tot * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpyAsync(r->tab, h.data(), tot * sizeof(double), hipMemcpyHostToDevice, sst) !=
+      hipSuccess) {
     set_error("loma_init: table upload failed"); loma_free(r); return LOMA_ERR_DEVICE;
   }
   YTab& T = r->T;
@@ -362,8 +401,8 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   // plain band copies for the Helmholtz factor kernel
   if ((rc = dalloc(r, &r->dt21, sizeof(double) * 5 * my))) { loma_free(r); return rc; }
   if ((rc = dalloc(r, &r->dt22, sizeof(double) * 5 * my))) { loma_free(r); return rc; }
-  hipMemcpy(r->dt21, f.a2.data(), sizeof(double) * 5 * my, hipMemcpyHostToDevice);
-  hipMemcpy(r->dt22, f.b2.data(), sizeof(double) * 5 * my, hipMemcpyHostToDevice);
+  hipMemcpyAsync(r->dt21, f.a2.data(), sizeof(double) * 5 * my, hipMemcpyHostToDevice, sst);
+  hipMemcpyAsync(r->dt22, f.b2.data(), sizeof(double) * 5 * my, hipMemcpyHostToDevice, sst);
   // ---- Helmholtz LU per (kx, |kz|) class: rK = kx^2 + kz^2 ----
   r->nclass = r->nplanes * (r->nz + 1);
   r->lu_stride = 5 * R;
@@ -376,16 +415,16 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
       }
     double* dsh = nullptr;
     if ((rc = dalloc(r, &dsh, sizeof(double) * r->nclass))) { loma_free(r); return rc; }
-    hipMemcpy(dsh, shifts.data(), sizeof(double) * r->nclass, hipMemcpyHostToDevice);
+    hipMemcpyAsync(dsh, shifts.data(), sizeof(double) * r->nclass, hipMemcpyHostToDevice, sst);
     if ((rc = dalloc(r, &r->helm_lu, sizeof(double) * (size_t)r->lu_stride * r->nclass))) {
       loma_free(r); return rc;
     }
     if ((rc = dalloc(r, &r->helm_bc, sizeof(double) * 4 * (size_t)r->nclass))) {
       loma_free(r); return rc;
     }
-    if (launch_helm_factor(r->dt21, r->dt22, my, M, dsh, r->nclass, r->helm_lu, 0) ||
-        launch_helm_bc(M, r->T, r->helm_lu, r->helm_bc, r->nclass, 0) ||
-        hipStreamSynchronize(nullptr) != hipSuccess) {
+    if (launch_helm_factor(r->dt21, r->dt22, my, M, dsh, r->nclass, r->helm_lu, sst) ||
+        launch_helm_bc(M, r->T, r->helm_lu, r->helm_bc, r->nclass, sst) ||
+        hipStreamSynchronize(sst) != hipSuccess) {
       set_error("loma_init: factor launch failed"); loma_free(r); return LOMA_ERR_DEVICE;
     }
   }
@@ -416,7 +455,7 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
     const size_t nmb[4] = {(size_t)12 * my, (size_t)12 * my, (size_t)2 * my, (size_t)2 * my};
     for (int q = 0; q < 4; ++q) {
       if ((rc = dalloc(r, mb[q], sizeof(double) * nmb[q]))) { loma_free(r); return rc; }
-      if (hipMemset(*mb[q], 0, sizeof(double) * nmb[q]) != hipSuccess) {
+      if (hipMemsetAsync(*mb[q], 0, sizeof(double) * nmb[q], sst) != hipSuccess) {
         set_error("loma_init: memset failed"); loma_free(r); return LOMA_ERR_DEVICE;
       }
     }
@@ -426,7 +465,8 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
     for (int j = 1; j < my - 1; ++j) hyv[j] = (y[j + 1] - y[j - 1]) / 2.0;  // main.f:689-696
     hyv[0] = (y[1] - y[0]) / 2.0;
     hyv[my - 1] = (y[my - 1] - y[my - 2]) / 2.0;
-    hipMemcpy(r->hy, hyv.data(), sizeof(double) * my, hipMemcpyHostToDevice);
+    hipMemcpyAsync(r->hy, hyv.data(), sizeof(double) * my, hipMemcpyHostToDevice, sst);
+    hipStreamSynchronize(sst);  // hyv is a local
   }
   if ((rc = dalloc(r, &r->sc, sizeof(DevScalars)))) { loma_free(r); return rc; }
   // twiddles
@@ -441,11 +481,13 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
       for (int q = 0; q < 2 * n; ++q) tf[q] = (float)tw[q];
       int e = dalloc(r, dst, sizeof(float) * 2 * n);
       if (e) return e;
-      hipMemcpy(*dst, tf.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice);
+      hipMemcpyAsync(*dst, tf.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, sst);
+      hipStreamSynchronize(sst);
     } else {
       int e = dalloc(r, dst, sizeof(double) * 2 * n);
       if (e) return e;
-      hipMemcpy(*dst, tw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice);
+      hipMemcpyAsync(*dst, tw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, sst);
+      hipStreamSynchronize(sst);
     }
     return 0;
   };
@@ -457,7 +499,8 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
       tw[2 * t + 1] = -std::sin(2.0 * M_PI * t / mgalz);
     }
     if ((rc = dalloc(r, &r->twz64, sizeof(double) * 2 * mgalz))) { loma_free(r); return rc; }
-    hipMemcpy(r->twz64, tw.data(), sizeof(double) * 2 * mgalz, hipMemcpyHostToDevice);
+    hipMemcpyAsync(r->twz64, tw.data(), sizeof(double) * 2 * mgalz, hipMemcpyHostToDevice, sst);
+    hipStreamSynchronize(sst);
   }
   // tile sizes: keep LDS per workgroup <= 64 KiB
   {
@@ -486,7 +529,7 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
     }
     r->L = 2 * pairs;
   }
-  if (hipStreamSynchronize(nullptr) != hipSuccess) {
+  if (hipStreamSynchronize(sst) != hipSuccess) {
     set_error("loma_init: device error during set-up"); loma_free(r); return LOMA_ERR_DEVICE;
   }
   if (flags & LOMA_FLAG_AUTOTUNE) {
@@ -520,22 +563,36 @@ int loma_set_stream(loma_run* r, void* stream) {
   return LOMA_OK;
 }
 
-static void host_to_c128(const void* src, int fp64, size_t n, std::vector<double>& dst) {
-  dst.resize(2 * n);
-  if (fp64)
-    std::memcpy(dst.data(), src, sizeof(double) * 2 * n);
-  else {
-    const float* s = (const float*)src;
-    for (size_t q = 0; q < 2 * n; ++q) dst[q] = s[q];
+// host complex field (interleaved float or double pairs) <-> doubles
+static void to_doubles(const void* src, int fp64, size_t n2, double* dst) {
+  if (fp64) {
+    std::memcpy(dst, src, sizeof(double) * n2);
+  } else {
+    const float* f = (const float*)src;
+    for (size_t q = 0; q < n2; ++q) dst[q] = f[q];
   }
 }
-static void c128_to_host(const std::vector<double>& src, int fp64, void* dst) {
-  if (fp64)
-    std::memcpy(dst, src.data(), sizeof(double) * src.size());
-  else {
-    float* d = (float*)dst;
-    for (size_t q = 0; q < src.size(); ++q) d[q] = (float)src[q];
+static void from_doubles(const double* src, int fp64, size_t n2, void* dst) {
+  if (fp64) {
+    std::memcpy(dst, src, sizeof(double) * n2);
+  } else {
+    float* f = (float*)dst;
+    for (size_t q = 0; q < n2; ++q) f[q] = (float)src[q];
   }
+}
+// One complex field host -> device (and back) through the pinned staging,
+// on the run's stream.  The staging is reused field after field.
+static int put_field(loma_run* r, const void* src, int fp64, size_t S, double2* dst) {
+  to_doubles(src, fp64, 2 * S, r->hstage);
+  HIPCHK(hipMemcpyAsync(dst, r->hstage, S * 16, hipMemcpyHostToDevice, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  return LOMA_OK;
+}
+static int get_field(loma_run* r, const double2* src, size_t S, int fp64, void* dst) {
+  HIPCHK(hipMemcpyAsync(r->hstage, src, S * 16, hipMemcpyDeviceToHost, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  from_doubles(r->hstage, fp64, 2 * S, dst);
+  return LOMA_OK;
 }
 
 int loma_upload(loma_run* r, const void* vor, const void* phi, const void* psi,
@@ -547,41 +604,49 @@ int loma_upload(loma_run* r, const void* vor, const void* phi, const void* psi,
   }
   std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
   const size_t S = (size_t)r->nplanes * r->mz * r->my;
-  std::vector<double> buf;
+  const int my = r->my;
+  int e = stage_host(r, std::max(2 * S, (size_t)3 * my));
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(r->stream));  // nothing of the run may still read the state
+  // boundary densities from the 00 mode of T (cross.f:208-213)
+  {
+    const double T0 = fp64 ? ((const double*)scal)[0] : ((const float*)scal)[0];
+    const double T1 = fp64 ? ((const double*)scal)[2 * (my - 1)]
+                           : ((const float*)scal)[2 * (my - 1)];
+    if (!(T0 > 0.0) || !(T1 > 0.0)) {
+      set_error("loma_upload: 00 mode of T must be positive at both ends");
+      return LOMA_ERR_ARG;
+    }
+    r->rc.rhobot = 1.0 / T0;
+    r->rc.rhotop = 1.0 / T1;
+  }
   const void* src[4] = {vor, phi, psi, scal};
   double2* dst[4] = {r->vor, r->phi, r->psi, r->scal};
-  double T0 = 0, T1 = 0;
-  for (int q = 0; q < 4; ++q) {
-    host_to_c128(src[q], fp64, S, buf);
-    if (q == 3) { T0 = buf[0]; T1 = buf[2 * (r->my - 1)]; }
-    HIPCHK(hipMemcpy(dst[q], buf.data(), S * 16, hipMemcpyHostToDevice));
-  }
+  for (int q = 0; q < 4; ++q)
+    if ((e = put_field(r, src[q], fp64, S, dst[q]))) return e;
   // history and work buffers start at zero (main.f:162-176)
-  HIPCHK(hipMemset(r->ten12, 0, S * 16));
-  HIPCHK(hipMemset(r->ten13, 0, S * 16));
-  HIPCHK(hipMemset(r->rhst, 0, sizeof(double) * (size_t)r->mgalx * r->my * r->mgalz));
-  if (r->rhsz) HIPCHK(hipMemset(r->rhsz, 0, sizeof(double) * (size_t)r->mgalx * r->my * r->mgalz));
-  HIPCHK(hipMemcpy(r->u00wk, u00, sizeof(double) * r->my, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(r->v00wk, v00, sizeof(double) * r->my, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(r->w00wk, w00, sizeof(double) * r->my, hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(r->rf0u, 0, sizeof(double) * r->my));
-  HIPCHK(hipMemset(r->rf0w, 0, sizeof(double) * r->my));
-  HIPCHK(hipMemset(r->rbar, 0, sizeof(double) * 2 * r->my));
-  HIPCHK(hipMemset(r->rzbar, 0, sizeof(double) * 2 * r->my));
-  // boundary densities from the 00 mode of T (cross.f:208-213)
-  if (!(T0 > 0.0) || !(T1 > 0.0)) {
-    set_error("loma_upload: 00 mode of T must be positive at both ends");
-    return LOMA_ERR_ARG;
-  }
-  r->rc.rhobot = 1.0 / T0;
-  r->rc.rhotop = 1.0 / T1;
+  hipStream_t st = r->stream;
+  const size_t P8 = sizeof(double) * (size_t)r->mgalx * my * r->mgalz;
+  HIPCHK(hipMemsetAsync(r->ten12, 0, S * 16, st));
+  HIPCHK(hipMemsetAsync(r->ten13, 0, S * 16, st));
+  HIPCHK(hipMemsetAsync(r->rhst, 0, P8, st));
+  if (r->rhsz) HIPCHK(hipMemsetAsync(r->rhsz, 0, P8, st));
+  HIPCHK(hipMemsetAsync(r->rf0u, 0, sizeof(double) * my, st));
+  HIPCHK(hipMemsetAsync(r->rf0w, 0, sizeof(double) * my, st));
+  HIPCHK(hipMemsetAsync(r->rbar, 0, sizeof(double) * 2 * my, st));
+  HIPCHK(hipMemsetAsync(r->rzbar, 0, sizeof(double) * 2 * my, st));
+  std::memcpy(r->hstage, u00, sizeof(double) * my);
+  std::memcpy(r->hstage + my, v00, sizeof(double) * my);
+  std::memcpy(r->hstage + 2 * my, w00, sizeof(double) * my);
+  HIPCHK(hipMemcpyAsync(r->u00wk, r->hstage, sizeof(double) * my, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(r->v00wk, r->hstage + my, sizeof(double) * my, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(r->w00wk, r->hstage + 2 * my, sizeof(double) * my, hipMemcpyHostToDevice,
+                        st));
   DevScalars s{};
   s.dt = (r->flags & LOMA_FLAG_DT_FP32) ? (double)(float)r->dt0 : r->dt0;
   s.time = 0.0;
-  HIPCHK(hipMemcpy(r->sc, &s, sizeof(s), hipMemcpyHostToDevice));
-  // the copies and fills above are on the null stream; the run's own stream
-  // does not wait for it
-  HIPCHK(hipStreamSynchronize(nullptr));
+  HIPCHK(hipMemcpyAsync(r->sc, &s, sizeof(s), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
   // events and pinned scalars of the one-step-in-flight interface, made here
   // (under the set-up lock) rather than in the stepping loop
   if (!r->sev[0]) {
@@ -601,12 +666,12 @@ int loma_upload_z(loma_run* r, const void* z, int fp64) {
   if (!r || !z || !r->with_z) { set_error("loma_upload_z: run has no scalar Z"); return LOMA_ERR_ARG; }
   std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
   const size_t S = (size_t)r->nplanes * r->mz * r->my;
-  std::vector<double> buf;
-  host_to_c128(z, fp64, S, buf);
-  r->rc.zbot = buf[0];
-  r->rc.ztop = buf[2 * (r->my - 1)];
-  HIPCHK(hipMemcpy(r->zf, buf.data(), S * 16, hipMemcpyHostToDevice));
-  HIPCHK(hipStreamSynchronize(nullptr));
+  int e = stage_host(r, 2 * S);
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(r->stream));
+  r->rc.zbot = fp64 ? ((const double*)z)[0] : ((const float*)z)[0];
+  r->rc.ztop = fp64 ? ((const double*)z)[2 * (r->my - 1)] : ((const float*)z)[2 * (r->my - 1)];
+  if ((e = put_field(r, z, fp64, S, r->zf))) return e;
   for (int q = 0; q < 2; ++q)
     if (r->gexec[q]) { hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
   return LOMA_OK;
@@ -616,31 +681,35 @@ int loma_download(loma_run* r, void* vor, void* phi, void* psi, void* scal, doub
                   double* v00, double* w00, int fp64) {
   if (!r || !r->uploaded) { set_error("loma_download: nothing uploaded"); return LOMA_ERR_STATE; }
   std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
-  HIPCHK(hipStreamSynchronize(r->stream));
   const size_t S = (size_t)r->nplanes * r->mz * r->my;
-  std::vector<double> buf(2 * S);
+  const int my = r->my;
+  int e = stage_host(r, std::max(2 * S, (size_t)3 * my));
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(r->stream));
   void* dst[4] = {vor, phi, psi, scal};
-  double2* src[4] = {r->vor, r->phi, r->psi, r->scal};
-  for (int q = 0; q < 4; ++q) {
-    if (!dst[q]) continue;
-    HIPCHK(hipMemcpy(buf.data(), src[q], S * 16, hipMemcpyDeviceToHost));
-    c128_to_host(buf, fp64, dst[q]);
-  }
-  if (u00) HIPCHK(hipMemcpy(u00, r->u00wk, sizeof(double) * r->my, hipMemcpyDeviceToHost));
-  if (v00) HIPCHK(hipMemcpy(v00, r->v00wk, sizeof(double) * r->my, hipMemcpyDeviceToHost));
-  if (w00) HIPCHK(hipMemcpy(w00, r->w00wk, sizeof(double) * r->my, hipMemcpyDeviceToHost));
+  const double2* src[4] = {r->vor, r->phi, r->psi, r->scal};
+  for (int q = 0; q < 4; ++q)
+    if (dst[q] && (e = get_field(r, src[q], S, fp64, dst[q]))) return e;
+  hipStream_t st = r->stream;
+  HIPCHK(hipMemcpyAsync(r->hstage, r->u00wk, sizeof(double) * my, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(r->hstage + my, r->v00wk, sizeof(double) * my, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(r->hstage + 2 * my, r->w00wk, sizeof(double) * my, hipMemcpyDeviceToHost,
+                        st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (u00) std::memcpy(u00, r->hstage, sizeof(double) * my);
+  if (v00) std::memcpy(v00, r->hstage + my, sizeof(double) * my);
+  if (w00) std::memcpy(w00, r->hstage + 2 * my, sizeof(double) * my);
   return LOMA_OK;
 }
 
 int loma_download_z(loma_run* r, void* z, int fp64) {
   if (!r || !r->with_z || !z) { set_error("loma_download_z: run has no scalar Z"); return LOMA_ERR_ARG; }
   std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
-  HIPCHK(hipStreamSynchronize(r->stream));
   const size_t S = (size_t)r->nplanes * r->mz * r->my;
-  std::vector<double> buf(2 * S);
-  HIPCHK(hipMemcpy(buf.data(), r->zf, S * 16, hipMemcpyDeviceToHost));
-  c128_to_host(buf, fp64, z);
-  return LOMA_OK;
+  int e = stage_host(r, 2 * S);
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(r->stream));
+  return get_field(r, r->zf, S, fp64, z);
 }
 
 static PhaseArgs phase_args(loma_run* r, int rkstep) {
@@ -889,6 +958,20 @@ static int run_steps(loma_run* r, int nsteps) {
   return LOMA_OK;
 }
 
+// device scalars -> host on the run's own stream (through the pinned copy
+// when the run has one), so the read waits for this run's work only
+static int read_scalars(loma_run* r, DevScalars* s) {
+  if (r->hsc) {
+    HIPCHK(hipMemcpyAsync(r->hsc, r->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    *s = *r->hsc;
+  } else {
+    HIPCHK(hipMemcpyAsync(s, r->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+  }
+  return LOMA_OK;
+}
+
 int loma_advance(loma_run* r, int nsteps) {
   if (!r || !r->uploaded) { set_error("loma_advance: nothing uploaded"); return LOMA_ERR_STATE; }
   if (nsteps < 0) { set_error("loma_advance: nsteps < 0"); return LOMA_ERR_ARG; }
@@ -898,9 +981,8 @@ int loma_advance(loma_run* r, int nsteps) {
   }
   int e = run_steps(r, nsteps);
   if (e) return e;
-  HIPCHK(hipStreamSynchronize(r->stream));
   DevScalars s;
-  HIPCHK(hipMemcpy(&s, r->sc, sizeof(s), hipMemcpyDeviceToHost));
+  if ((e = read_scalars(r, &s))) return e;
   if (s.nan) { set_error("loma_advance: non-finite state"); return LOMA_ERR_NAN; }
   return LOMA_OK;
 }
@@ -917,9 +999,9 @@ int loma_advance_async(loma_run* r, int nsteps) {
 
 int loma_sync(loma_run* r) {
   if (!r) return LOMA_ERR_ARG;
-  HIPCHK(hipStreamSynchronize(r->stream));
   DevScalars s;
-  HIPCHK(hipMemcpy(&s, r->sc, sizeof(s), hipMemcpyDeviceToHost));
+  int e = read_scalars(r, &s);
+  if (e) return e;
   if (s.nan) { set_error("loma_sync: non-finite state"); return LOMA_ERR_NAN; }
   return LOMA_OK;
 }
@@ -1077,9 +1159,9 @@ int loma_step_poll(loma_run* r, int* done, double* device_ms, int* nonfinite) {
 
 int loma_scalars(loma_run* r, double* out) {
   if (!r || !out) return LOMA_ERR_ARG;
-  HIPCHK(hipStreamSynchronize(r->stream));
   DevScalars s;
-  HIPCHK(hipMemcpy(&s, r->sc, sizeof(s), hipMemcpyDeviceToHost));
+  int e = read_scalars(r, &s);
+  if (e) return e;
   out[0] = s.dt; out[1] = s.time; out[2] = s.dtr; out[3] = r->istep; out[4] = s.nan;
   return LOMA_OK;
 }

@@ -70,6 +70,15 @@ __device__ __forceinline__ TabRef relaunder(const TabRef& b) {
   asm volatile("" : "+s"(z));
   return TabRef{b.r, b.ncol, b.off, b.lane8, z};
 }
+// The same, additionally ordered after `dep` is computed: a level of a scan
+// loads its table entries once the previous level's carry exists, all
+// together, instead of the compiler hoisting every level's loads to the
+// start (and spilling them) or serialising them one by one.
+__device__ __forceinline__ TabRef relaunder_after(const TabRef& b, double dep) {
+  int z = b.sbias;
+  asm volatile("" : "+s"(z) : "v"(dep));
+  return TabRef{b.r, b.ncol, b.off, b.lane8, z};
+}
 // index of entry `idx` of grid row `row` in a lane-major table
 template <int M>
 __host__ __device__ inline int lane_major_index(int row, int ncol, int idx) {
@@ -353,6 +362,14 @@ __device__ __forceinline__ void lsolve_tab(const TabRef lo, const TabRef H, int 
   for (int k = 0; k < 6; ++k) {
     if (k < levels) {
       const int d = 1 << k;
+      // the level's composed map is loaded before the shuffles and outside
+      // the lane test, so its loads are in flight together
+      const TabRef Hk = relaunder_after(H, c[0][0]);
+      double h[P][P];
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int b = 0; b < P; ++b) h[q][b] = tget(Hk, k * P + q, b);
       double cp[NC][P];
 #pragma unroll
       for (int n = 0; n < NC; ++n)
@@ -362,11 +379,9 @@ __device__ __forceinline__ void lsolve_tab(const TabRef lo, const TabRef H, int 
 #pragma unroll
         for (int q = 0; q < P; ++q)
 #pragma unroll
-          for (int b = 0; b < P; ++b) {
-            const double h = tget(H, k * P + q, b);
+          for (int b = 0; b < P; ++b)
 #pragma unroll
-            for (int n = 0; n < NC; ++n) c[n][q] += h * cp[n][b];
-          }
+            for (int n = 0; n < NC; ++n) c[n][q] += h[q][b] * cp[n][b];
       }
     }
   }
@@ -425,6 +440,12 @@ __device__ __forceinline__ void usolve_tab(const TabRef up, const TabRef inv, co
   for (int k = 0; k < 6; ++k) {
     if (k < levels) {
       const int d = 1 << k;
+      const TabRef Hk = relaunder_after(H, c[0][0]);
+      double h[P][P];
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int b = 0; b < P; ++b) h[q][b] = tget(Hk, k * P + q, b);
       double cp[NC][P];
 #pragma unroll
       for (int n = 0; n < NC; ++n)
@@ -434,11 +455,9 @@ __device__ __forceinline__ void usolve_tab(const TabRef up, const TabRef inv, co
 #pragma unroll
         for (int q = 0; q < P; ++q)
 #pragma unroll
-          for (int b = 0; b < P; ++b) {
-            const double h = tget(H, k * P + q, b);
+          for (int b = 0; b < P; ++b)
 #pragma unroll
-            for (int n = 0; n < NC; ++n) c[n][q] += h * cp[n][b];
-          }
+            for (int n = 0; n < NC; ++n) c[n][q] += h[q][b] * cp[n][b];
       }
     }
   }

@@ -137,11 +137,17 @@ __device__ __forceinline__ int lds_i(int i) { return PAD ? i + (i >> 3) : i; }
 __host__ __device__ inline int lds_entries(int n, int pad = LOMA_LDS_PAD) {
   return pad ? n + (n >> 3) + 1 : n;
 }
-// The FP64 z transforms use the padded layout: measured at 512x513x256 it
-// takes them from 1.43 / 1.89 ms to 1.28 / 1.70 ms (inverse / forward); the
-// FP32 ones and the x lines do not gain (FP32 x lines lose 6%).
+// Row-major LDS rows of a z-transform tile: entry i of row `row` at
+// row * rs + lds_i<PAD>(i).  The in-row pad spreads the stride-R butterfly
+// writes over the banks; the row stride is odd (in entries) so that the
+// tile's global loads and stores, which walk down a column (consecutive
+// rows, same i), hit distinct banks.
+__host__ __device__ inline int lds_row_stride(int n, int pad) { return n + (pad ? n / 8 : 0) + 1; }
+#ifndef LOMA_ZF_PAD32
+#define LOMA_ZF_PAD32 1
+#endif
 template <typename T>
-constexpr int zf_pad() { return sizeof(T) == 8 ? 1 : 0; }
+constexpr int zf_pad() { return sizeof(T) == 8 ? 1 : LOMA_ZF_PAD32; }
 
 template <int R, typename T, typename T2>
 __device__ __forceinline__ void stockham_item(const T2* __restrict__ src, T2* __restrict__ dst,
@@ -174,7 +180,7 @@ __device__ __forceinline__ void stockham_item(const T2* __restrict__ src, T2* __
 // buffer holding the result.  s = -1 forward (e^{-i}), +1 inverse; unscaled.
 // Ends with a barrier.
 template <typename T, typename T2>
-__device__ T2* fft_rows(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
+__device__ T2* fft_rows(T2* a, T2* b, int N, int RS, int nrows, const FftPlan& pl,
                         const T2* __restrict__ tw, int s) {
   int Ns = 1;
   for (int p = 0; p < pl.npass; ++p) {
@@ -184,7 +190,7 @@ __device__ T2* fft_rows(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
     const int total = nrows * nb;
     for (int it = threadIdx.x; it < total; it += blockDim.x) {
       const int row = it / nb, j = it - row * nb;
-      const int rb = row * N;
+      const int rb = row * RS;
       switch (R) {
         case 2: stockham_item<2, T, T2>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
         case 3: stockham_item<3, T, T2>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
@@ -206,7 +212,7 @@ __device__ T2* fft_rows(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
 // registers, synchronises, writes the butterflies back, synchronises.
 // nrows * N <= blockDim.x * E (E = complex entries per thread).
 template <int R, typename T, typename T2, int PAD>
-__device__ __forceinline__ void radix_pass_ip(T2* a, int N, int nrows, int Ns, int tstep,
+__device__ __forceinline__ void radix_pass_ip(T2* a, int N, int RS, int nrows, int Ns, int tstep,
                                               const T2* __restrict__ tw, int s, T2* v,
                                               int nitems) {
   const int nb = N / R;
@@ -217,7 +223,7 @@ __device__ __forceinline__ void radix_pass_ip(T2* a, int N, int nrows, int Ns, i
     if (it < total) {
       const int row = it / nb, j = it - row * nb;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[k * R + r] = a[lds_i<PAD>(row * N + j + r * nb)];
+      for (int r = 0; r < R; ++r) v[k * R + r] = a[row * RS + lds_i<PAD>(j + r * nb)];
     }
   }
   __syncthreads();
@@ -241,9 +247,9 @@ __device__ __forceinline__ void radix_pass_ip(T2* a, int N, int nrows, int Ns, i
       if (R == 4) dft4<T, T2>(w, s);
       if (R == 5) dft5<T, T2>(w, s);
       if (R == 8) dft8<T, T2>(w, s);
-      const int idxD = row * N + (j / Ns) * Ns * R + jm;
+      const int idxD = (j / Ns) * Ns * R + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) a[lds_i<PAD>(idxD + r * Ns)] = w[r];
+      for (int r = 0; r < R; ++r) a[row * RS + lds_i<PAD>(idxD + r * Ns)] = w[r];
     }
   }
   __syncthreads();
@@ -253,16 +259,17 @@ template <typename T, typename T2, int E, int PAD>
 __device__ void fft_rows_ip(T2* a, int N, int nrows, const FftPlan& pl,
                             const T2* __restrict__ tw, int s) {
   T2 v[E + 8];
+  const int RS = lds_row_stride(N, PAD);
   int Ns = 1;
   for (int p = 0; p < pl.npass; ++p) {
     const int R = pl.rad[p];
     const int tstep = N / (Ns * R);
     switch (R) {
-      case 2: radix_pass_ip<2, T, T2, PAD>(a, N, nrows, Ns, tstep, tw, s, v, (E + 1) / 2); break;
-      case 3: radix_pass_ip<3, T, T2, PAD>(a, N, nrows, Ns, tstep, tw, s, v, (E + 2) / 3); break;
-      case 4: radix_pass_ip<4, T, T2, PAD>(a, N, nrows, Ns, tstep, tw, s, v, (E + 3) / 4); break;
-      case 5: radix_pass_ip<5, T, T2, PAD>(a, N, nrows, Ns, tstep, tw, s, v, (E + 4) / 5); break;
-      default: radix_pass_ip<8, T, T2, PAD>(a, N, nrows, Ns, tstep, tw, s, v, (E + 7) / 8); break;
+      case 2: radix_pass_ip<2, T, T2, PAD>(a, N, RS, nrows, Ns, tstep, tw, s, v, (E + 1) / 2); break;
+      case 3: radix_pass_ip<3, T, T2, PAD>(a, N, RS, nrows, Ns, tstep, tw, s, v, (E + 2) / 3); break;
+      case 4: radix_pass_ip<4, T, T2, PAD>(a, N, RS, nrows, Ns, tstep, tw, s, v, (E + 3) / 4); break;
+      case 5: radix_pass_ip<5, T, T2, PAD>(a, N, RS, nrows, Ns, tstep, tw, s, v, (E + 4) / 5); break;
+      default: radix_pass_ip<8, T, T2, PAD>(a, N, RS, nrows, Ns, tstep, tw, s, v, (E + 7) / 8); break;
     }
     Ns *= R;
   }
@@ -276,7 +283,7 @@ constexpr int ct_radix(int m) {
   return m % 8 == 0 ? 8 : (m % 4 == 0 ? 4 : (m % 2 == 0 ? 2 : (m % 3 == 0 ? 3 : 5)));
 }
 
-template <typename T, typename T2, int N, int Ns, int NT>
+template <typename T, typename T2, int N, int Ns, int NT, int RS>
 __device__ __forceinline__ T2* fft_rows_ct(T2* a, T2* b, int nrows, const T2* __restrict__ tw,
                                            int s) {
   if constexpr (Ns >= N) {
@@ -292,10 +299,10 @@ __device__ __forceinline__ T2* fft_rows_ct(T2* a, T2* b, int nrows, const T2* __
     const int total = nrows * nb;
     for (int it = t0; it < total; it += NT) {
       const int row = it / nb, j = it - row * nb;
-      stockham_item<R, T, T2>(a, b, row * N, j, nb, Ns, tstep, tw, s);
+      stockham_item<R, T, T2>(a, b, row * RS, j, nb, Ns, tstep, tw, s);
     }
     __syncthreads();
-    return fft_rows_ct<T, T2, N, Ns * R, NT>(b, a, nrows, tw, s);
+    return fft_rows_ct<T, T2, N, Ns * R, NT, RS>(b, a, nrows, tw, s);
   }
 }
 
@@ -307,6 +314,7 @@ __device__ __forceinline__ void fft_rows_ip_ct(T2* a, int nrows, const T2* __res
     constexpr int nb = N / R;
     constexpr int tstep = N / (Ns * R);
     constexpr int nitems = (E + R - 1) / R;
+    constexpr int RS = N + (PAD ? N / 8 : 0) + 1;
     const int total = nrows * nb;
     int t0 = threadIdx.x;
     asm volatile("" : "+v"(t0));
@@ -316,7 +324,7 @@ __device__ __forceinline__ void fft_rows_ip_ct(T2* a, int nrows, const T2* __res
       if (it < total) {
         const int row = it / nb, j = it - row * nb;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[k * R + r] = a[lds_i<PAD>(row * N + j + r * nb)];
+        for (int r = 0; r < R; ++r) v[k * R + r] = a[row * RS + lds_i<PAD>(j + r * nb)];
       }
     }
     __syncthreads();
@@ -340,9 +348,9 @@ __device__ __forceinline__ void fft_rows_ip_ct(T2* a, int nrows, const T2* __res
         if (R == 4) dft4<T, T2>(w, s);
         if (R == 5) dft5<T, T2>(w, s);
         if (R == 8) dft8<T, T2>(w, s);
-        const int idxD = row * N + (j / Ns) * Ns * R + jm;
+        const int idxD = (j / Ns) * Ns * R + jm;
 #pragma unroll
-        for (int r = 0; r < R; ++r) a[lds_i<PAD>(idxD + r * Ns)] = w[r];
+        for (int r = 0; r < R; ++r) a[row * RS + lds_i<PAD>(idxD + r * Ns)] = w[r];
       }
     }
     __syncthreads();
@@ -351,13 +359,21 @@ __device__ __forceinline__ void fft_rows_ip_ct(T2* a, int nrows, const T2* __res
 }
 
 // NN > 0: compile-time length NN; NN = 0: the run-time plan
+// LOMA_XL_RS1 = 1 stores x-line rows with an odd stride (in entries); it
+// was measured slower than the even stride at 512x513x256, so it is off
+#ifndef LOMA_XL_RS1
+#define LOMA_XL_RS1 0
+#endif
+template <int NN>
+__device__ __forceinline__ int xl_row_stride(int N) { return (NN ? NN : N) + LOMA_XL_RS1; }
+
 template <typename T, typename T2, int NN, int NT>
 __device__ __forceinline__ T2* fft_any(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
                                        const T2* __restrict__ tw, int s) {
   if constexpr (NN > 0)
-    return fft_rows_ct<T, T2, NN, 1, NT>(a, b, nrows, tw, s);
+    return fft_rows_ct<T, T2, NN, 1, NT, NN + LOMA_XL_RS1>(a, b, nrows, tw, s);
   else
-    return fft_rows<T, T2>(a, b, N, nrows, pl, tw, s);
+    return fft_rows<T, T2>(a, b, N, N + LOMA_XL_RS1, nrows, pl, tw, s);
 }
 template <typename T, typename T2, int E, int NN, int NT, int PAD>
 __device__ __forceinline__ void fft_any_ip(T2* a, int N, int nrows, const FftPlan& pl,
@@ -383,6 +399,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
   const int N = NN ? NN : a.rc.mgalz, J = a.J, my = a.rc.my, myp = a.rc.myp, mz = a.rc.mz,
             nz = a.rc.nz;
   constexpr int PAD = zf_pad<T>();
+  const int RS = lds_row_stride(N, PAD);
   T2* A = (T2*)smem;
   const int nyb = (my + J - 1) / J;
   const int i = blockIdx.x / nyb, j0 = (blockIdx.x - i * nyb) * J;
@@ -393,20 +410,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
   const int nzero = N - mz;
   for (int it = threadIdx.x; it < nzero * J; it += blockDim.x) {
     const int q = it / J, jj = it - q * J;
-    A[lds_i<PAD>(jj * N + nz + 1 + q)] = T2{0, 0};
+    A[jj * RS + lds_i<PAD>(nz + 1 + q)] = T2{0, 0};
   }
   for (int it = threadIdx.x; it < mz * J; it += blockDim.x) {
     const int k = it / J, jj = it - k * J, j = j0 + jj;
     const int z = (k <= nz) ? k : k + N - mz;
     T2 v = T2{0, 0};
     if (j < my) v = src[((size_t)i * mz + k) * myp + j];
-    A[lds_i<PAD>(jj * N + z)] = v;
+    A[jj * RS + lds_i<PAD>(z)] = v;
   }
   __syncthreads();
   fft_any_ip<T, T2, E, NN, 256, PAD>(A, N, J, a.plz, (const T2*)a.twz, +1);
   for (int it = threadIdx.x; it < N * J; it += blockDim.x) {
     const int z = it / J, jj = it - z * J, j = j0 + jj;
-    if (j < my) dst[((size_t)i * N + z) * myp + j] = A[lds_i<PAD>(jj * N + z)];
+    if (j < my) dst[((size_t)i * N + z) * myp + j] = A[jj * RS + lds_i<PAD>(z)];
   }
 }
 
@@ -417,6 +434,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
   const int N = NN ? NN : a.rc.mgalz, J = a.J, my = a.rc.my, myp = a.rc.myp, mz = a.rc.mz,
             nz = a.rc.nz;
   constexpr int PAD = zf_pad<T>();
+  const int RS = lds_row_stride(N, PAD);
   T2* A = (T2*)smem;
   const int nyb = (my + J - 1) / J;
   const int i = blockIdx.x / nyb, j0 = (blockIdx.x - i * nyb) * J;
@@ -427,7 +445,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
     const int z = it / J, jj = it - z * J, j = j0 + jj;
     T2 v = T2{0, 0};
     if (j < my) v = src[((size_t)i * N + z) * myp + j];
-    A[lds_i<PAD>(jj * N + z)] = v;
+    A[jj * RS + lds_i<PAD>(z)] = v;
   }
   __syncthreads();
   fft_any_ip<T, T2, E, NN, 256, PAD>(A, N, J, a.plz, (const T2*)a.twz, -1);
@@ -436,7 +454,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
     const int k = it / J, jj = it - k * J, j = j0 + jj;
     if (j < my) {
       const int z = (k <= nz) ? k : k + N - mz;
-      T2 v = A[lds_i<PAD>(jj * N + z)];
+      T2 v = A[jj * RS + lds_i<PAD>(z)];
       v.x *= sc;
       v.y *= sc;
       dst[((size_t)i * mz + k) * myp + j] = v;
@@ -517,11 +535,12 @@ __device__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src, T2* A, T
   xl_fetch<T2, XR, NN>(a, src, z, j0, pairs, v);
   const int N = NN ? NN : a.rc.mgalx, np = NN ? NN / 3 : a.rc.nplanes;
   const int lp = __builtin_ctz(pairs);
+  const int RS = xl_row_stride<NN>(N);
   // the dealiased band np..N-np is zero
   const int nzero = N - 2 * np + 1;
   for (int it = threadIdx.x; it < pairs * nzero; it += XL_THREADS) {
     const int p = it / nzero, q = it - p * nzero;
-    A[lds_i(p * N + np + q)] = T2{0, 0};
+    A[lds_i(p * RS + np + q)] = T2{0, 0};
   }
 #pragma unroll
   for (int r = 0; r < XR; ++r) {
@@ -529,7 +548,7 @@ __device__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src, T2* A, T
     if (it < np * pairs) {
       const int ii = it >> lp, p = it & (pairs - 1);
       const T2 ca = v.a[r], cb = v.b[r];
-      const int rb = p * N;
+      const int rb = p * RS;
       if (ii == 0) {
         // the imaginary part of the kx=0 coefficient is ignored (rft, rftsingle.f:87-91)
         A[lds_i(rb)] = T2{ca.x, cb.x};
@@ -551,13 +570,14 @@ __device__ void xl_forward(const FftArgs& a, T2* __restrict__ dst, T2* A, T2* B,
   const int N = NN ? NN : a.rc.mgalx, np = NN ? NN / 3 : a.rc.nplanes, my = a.rc.my,
             mgz = a.rc.mgalz;
   const int lp = __builtin_ctz(pairs);
+  const int RS = xl_row_stride<NN>(N);
   __syncthreads();
   const T2* R = fft_any<T, T2, NN, XL_THREADS>(A, B, N, pairs, a.plx, (const T2*)a.twx, -1);
   const T hs = (T)(0.5 / (double)N);
   for (int it = threadIdx.x; it < np * pairs; it += XL_THREADS) {
     const int ii = it >> lp, p = it & (pairs - 1);
     const int ja = j0 + 2 * p, jb = ja + 1;
-    const int rb = p * N;
+    const int rb = p * RS;
     const T2 zk = R[lds_i(rb + ii)];
     T2 zc = R[lds_i(rb + (N - ii) % N)];
     zc.y = -zc.y;
@@ -622,8 +642,8 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
   if (tile >= rc.mgalz * nyb) return;
   const int z = tile / nyb, j0 = (tile - z * nyb) * a.L;
   T2* A = (T2*)smem;
-  T2* B = A + lds_entries(pairs * N);
-  const int tot = pairs * N;
+  const int RS = xl_row_stride<NN>(N);
+  T2* B = A + lds_entries(pairs * RS);
   constexpr int NIN = WZ ? 12 : 8;
   const double ipe = 1.0 / rc.peclet;
   const XlRk k = xl_rk(a.sc->dt, a.rkstep);
@@ -675,7 +695,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
     for (int m = 0; m < EPT; ++m) {
       const int x = ent(m);
       T2 q = T2{0, 0};
-      if (x < N) q = R[lds_i(p * N + x)];
+      if (x < N) q = R[lds_i(p * RS + x)];
       v[m][0] = q.x;
       v[m][1] = q.y;
     }
@@ -685,7 +705,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
       const int x = ent(m);
-      if (x < N) A[lds_i(p * N + x)] = T2{v[m][0], v[m][1]};
+      if (x < N) A[lds_i(p * RS + x)] = T2{v[m][0], v[m][1]};
     }
   };
   T2* const* wb = (T2* const*)a.wbuf;
@@ -770,7 +790,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
         }
         val[h] = v;
       }
-      A[lds_i(p * N + x)] = T2{val[0], val[1]};
+      A[lds_i(p * RS + x)] = T2{val[0], val[1]};
     }
     xl_forward<T, T2, NN>(a, wo[q], A, B, z, j0, pairs);
   }
@@ -879,7 +899,7 @@ __global__ __launch_bounds__(256) void k_fourz(FftPlan pl, const double2* tw, do
   if (row >= nrows) return;
   for (int it = threadIdx.x; it < N; it += blockDim.x) A[lds_i(it)] = c[(size_t)row * N + it];
   __syncthreads();
-  double2* R = fft_rows<double, double2>(A, B, N, 1, pl, tw, iopt < 0 ? -1 : 1);
+  double2* R = fft_rows<double, double2>(A, B, N, N, 1, pl, tw, iopt < 0 ? -1 : 1);
   const double sc = iopt < 0 ? 1.0 / N : 1.0;
   for (int it = threadIdx.x; it < N; it += blockDim.x) {
     double2 v = R[lds_i(it)];
@@ -893,10 +913,10 @@ __global__ __launch_bounds__(256) void k_fourz(FftPlan pl, const double2* tw, do
 // launchers
 // ------------------------------------------------------------------------
 static size_t zfft_lds(bool fp32, const FftArgs& a) {
-  return (size_t)lds_entries(a.J * a.rc.mgalz, fp32 ? 0 : 1) * (fp32 ? 8 : 16);
+  return (size_t)a.J * lds_row_stride(a.rc.mgalz, fp32 ? LOMA_ZF_PAD32 : 1) * (fp32 ? 8 : 16);
 }
 static size_t xline_lds(bool fp32, const FftArgs& a) {
-  return (size_t)2 * lds_entries((a.L / 2) * a.rc.mgalx) * (fp32 ? 8 : 16);
+  return (size_t)2 * lds_entries((a.L / 2) * (a.rc.mgalx + LOMA_XL_RS1)) * (fp32 ? 8 : 16);
 }
 
 // kernel instance for a transform length: compile-time plans for the

@@ -183,16 +183,20 @@ class HipBackend:
         return run.step_poll()
 
     def finish(self, run):
+        """Final state (with keep_state); the run is closed here, or kept by
+        `retired` once the loop has returned its bytes to the budget."""
         try:
             out = run.download() if self.keep_state else None
         except Exception:
             run.close()
             raise
-        if self.pool is not None:
-            self.pool.put(run.req_key, run, run.bytes)
-        else:
+        if self.pool is None:
             run.close()
         return out
+
+    def retired(self, run):
+        if self.pool is not None:
+            self.pool.put(run.req_key, run, run.bytes)
 
     def free(self, run):
         run.close()
@@ -240,6 +244,14 @@ class Server:
         self.clock = clock
         self.idle_sleep = idle_sleep
 
+    def _timed(self, name, fn, *args):
+        t = time.perf_counter()
+        out = fn(*args)
+        dt = time.perf_counter() - t
+        if dt > self._slow.get(name, 0.0):
+            self._slow[name] = dt
+        return out
+
     def _idle(self) -> int:
         f = getattr(self.b, "idle_bytes", None)
         return f() if f else 0
@@ -274,6 +286,7 @@ class Server:
         t0 = self.clock()
         closed = False
         gaps = []          # loop passes that took longer than 20 ms (s)
+        self._slow = {}    # slowest main-thread call per kind (s)
         t_prev = t0
         try:
             while pending or waiting or starting or active or retiring:
@@ -310,7 +323,8 @@ class Server:
                         busy = True
                         continue
                     nres = len(active) + len(starting)
-                    if nres >= self.max_slots or not self._room(r, o.footprint, used):
+                    if nres >= self.max_slots or not self._timed("admit", self._room, r,
+                                                                  o.footprint, used):
                         break
                     waiting.popleft()
                     used += o.footprint
@@ -336,13 +350,13 @@ class Server:
                         lv.out.status = "cut" if closed else "done"
                         retiring.append((lv, pool.submit(self.b.finish, lv.run)))
                         continue
-                    self.b.begin(lv.run)
+                    self._timed("begin", self.b.begin, lv.run)
                     lv.t_begin = self.clock()
                     active.append(lv)
                 peak_active = max(peak_active, len(active))
                 # poll the steps in flight
                 for lv in list(active):
-                    done, ms, nonfinite = self.b.poll(lv.run)
+                    done, ms, nonfinite = self._timed("poll", self.b.poll, lv.run)
                     if not done:
                         continue
                     t = self.clock()
@@ -362,7 +376,7 @@ class Server:
                         active.remove(lv)
                         retiring.append((lv, pool.submit(self.b.finish, lv.run)))
                     else:
-                        self.b.begin(lv.run)
+                        self._timed("begin", self.b.begin, lv.run)
                         lv.t_begin = self.clock()
                 # retirement finished: memory returns to the budget
                 for item in list(retiring):
@@ -377,6 +391,9 @@ class Server:
                         lv.out.status, lv.out.error = "failed", repr(f.exception())
                     elif lv.out.status in ("done", "cut"):
                         lv.out.state = f.result()
+                        keep = getattr(self.b, "retired", None)
+                        if keep:  # the backend may keep the run for reuse
+                            keep(lv.run)
                 if not busy:
                     time.sleep(self.idle_sleep)
         finally:
@@ -407,6 +424,7 @@ class Server:
             by_grid=by_grid,
             slow_loop_passes=len(gaps),
             slowest_loop_pass_ms=float(max(gaps) * 1e3) if gaps else 0.0,
+            slowest_call_ms={k: float(v * 1e3) for k, v in self._slow.items()},
             peak_bytes=int(peak_used),
             peak_active=int(peak_active),
             counts={s: sum(1 for o in outs.values() if o.status == s)
