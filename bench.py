@@ -28,14 +28,15 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # HBM bytes per launch measured with rocprofv3 on the 512x513x256 step
-# without Z: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors read off a
-# plain 1 GiB device copy in the same runs (python3 -m
-# loma_phd_amd.profile_step; summary in profiles/rk3_step_512x513x256_hbm_traffic.md).
+# without Z: read = 32/64/128 x the TCC read requests of each size, write =
+# WRITE_SIZE x 1024; a plain 1 GiB device copy in the same runs reads and
+# writes 1 GiB by these sums (python3 -m loma_phd_amd.profile_step; summary
+# in profiles/rk3_step_512x513x256_hbm_traffic.md).
 MEASURED_TRAFFIC = {
-    "fp64": {"phaseA": 4.481e9, "zfft_inv": 4.776e9, "xline": 8.032e9,
-             "zfft_fwd": 6.582e9, "phaseC": 4.724e9, "phaseC2": 1.715e9},
-    "mixed": {"phaseA": 3.600e9, "zfft_inv": 2.408e9, "xline": 7.056e9,
-              "zfft_fwd": 3.327e9, "phaseC": 3.506e9, "phaseC2": 1.592e9},
+    "fp64": {"phaseA": 4.266e9, "zfft_inv": 4.776e9, "xline": 8.016e9,
+             "zfft_fwd": 6.582e9, "phaseC": 4.769e9, "phaseC2": 1.029e9},
+    "mixed": {"phaseA": 3.311e9, "zfft_inv": 2.408e9, "xline": 4.569e9,
+              "zfft_fwd": 3.327e9, "phaseC": 3.583e9, "phaseC2": 1.005e9},
 }
 
 

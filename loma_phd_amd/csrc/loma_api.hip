@@ -502,7 +502,7 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
     hipMemcpyAsync(r->twz64, tw.data(), sizeof(double) * 2 * mgalz, hipMemcpyHostToDevice, sst);
     hipStreamSynchronize(sst);
   }
-  // tile sizes: keep LDS per workgroup <= 64 KiB
+  // tile sizes: keep dynamic LDS per workgroup <= 64 KiB
   {
     const int e2 = r->fp32 ? 8 : 16;
     // z transforms: J y entries per workgroup, J * mgalz <= 256 * E
@@ -518,7 +518,8 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
     r->J = J;
     // x lines: 2*pairs lines per 256-thread workgroup, pairs*mgalx entries
     // at most 256*EPT (EPT = 4 in FP64, 8 in FP32): two LDS buffers of
-    // 32 KiB together
+    // 32 KiB together, 64 KiB in FP64 where each transform carries two
+    // fields (MI355X allows 160 KiB per workgroup)
     const int cap = r->fp32 ? 2048 : 1024;
     int pairs = 16;  // a power of two: each thread serves one line pair
     while (pairs > 1 && pairs * mgalx > cap) pairs /= 2;

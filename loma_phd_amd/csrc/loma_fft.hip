@@ -149,6 +149,42 @@ __host__ __device__ inline int lds_row_stride(int n, int pad) { return n + (pad 
 template <typename T>
 constexpr int zf_pad() { return sizeof(T) == 8 ? 1 : LOMA_ZF_PAD32; }
 
+// Twiddles of one butterfly: w[r] *= tw[jm * r * tstep] (conjugated for
+// s > 0), r = 1..R-1.  With LOMA_TW_REC = 1 (default) only tw[jm * tstep]
+// is loaded and its powers are formed by products (at most three
+// multiplications deep), which trades six table loads per butterfly for
+// complex multiplications: 5.75 against 5.97 ms per FP64 x-line launch and
+// 1.64 against 1.74 ms per FP64 forward z launch at 512x513x256.
+#ifndef LOMA_TW_REC
+#define LOMA_TW_REC 1
+#endif
+template <int R, typename T2>
+__device__ __forceinline__ void twiddle(T2* w, const T2* __restrict__ tw, int jm, int tstep,
+                                        int s) {
+  if constexpr (LOMA_TW_REC) {
+    T2 p[R];
+    p[1] = tw[jm * tstep];
+    if (s > 0) p[1].y = -p[1].y;
+    if constexpr (R >= 3) p[2] = cmul(p[1], p[1]);
+    if constexpr (R >= 4) p[3] = cmul(p[2], p[1]);
+    if constexpr (R >= 5) p[4] = cmul(p[2], p[2]);
+    if constexpr (R >= 8) {
+      p[5] = cmul(p[4], p[1]);
+      p[6] = cmul(p[3], p[3]);
+      p[7] = cmul(p[4], p[3]);
+    }
+#pragma unroll
+    for (int r = 1; r < R; ++r) w[r] = cmul(w[r], p[r]);
+  } else {
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      T2 t = tw[jm * r * tstep];
+      if (s > 0) t.y = -t.y;
+      w[r] = cmul(w[r], t);
+    }
+  }
+}
+
 template <int R, typename T, typename T2>
 __device__ __forceinline__ void stockham_item(const T2* __restrict__ src, T2* __restrict__ dst,
                                               int rowbase, int j, int nb, int Ns, int tstep,
@@ -157,14 +193,7 @@ __device__ __forceinline__ void stockham_item(const T2* __restrict__ src, T2* __
   const int jm = j % Ns;
 #pragma unroll
   for (int r = 0; r < R; ++r) v[r] = src[lds_i(rowbase + j + r * nb)];
-  if (Ns > 1) {
-#pragma unroll
-    for (int r = 1; r < R; ++r) {
-      T2 w = tw[jm * r * tstep];
-      if (s > 0) w.y = -w.y;
-      v[r] = cmul(v[r], w);
-    }
-  }
+  if (Ns > 1) twiddle<R>(v, tw, jm, tstep, s);
   if (R == 2) dft2<T, T2>(v);
   if (R == 3) dft3<T, T2>(v, s);
   if (R == 4) dft4<T, T2>(v, s);
@@ -234,14 +263,7 @@ __device__ __forceinline__ void radix_pass_ip(T2* a, int N, int RS, int nrows, i
       const int row = it / nb, j = it - row * nb;
       const int jm = j % Ns;
       T2* w = v + k * R;
-      if (Ns > 1) {
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          T2 t = tw[jm * r * tstep];
-          if (s > 0) t.y = -t.y;
-          w[r] = cmul(w[r], t);
-        }
-      }
+      if (Ns > 1) twiddle<R>(w, tw, jm, tstep, s);
       if (R == 2) dft2<T, T2>(w);
       if (R == 3) dft3<T, T2>(w, s);
       if (R == 4) dft4<T, T2>(w, s);
@@ -335,14 +357,7 @@ __device__ __forceinline__ void fft_rows_ip_ct(T2* a, int nrows, const T2* __res
         const int row = it / nb, j = it - row * nb;
         const int jm = j % Ns;
         T2* w = v + k * R;
-        if (Ns > 1) {
-#pragma unroll
-          for (int r = 1; r < R; ++r) {
-            T2 t = tw[jm * r * tstep];
-            if (s > 0) t.y = -t.y;
-            w[r] = cmul(w[r], t);
-          }
-        }
+        if (Ns > 1) twiddle<R>(w, tw, jm, tstep, s);
         if (R == 2) dft2<T, T2>(w);
         if (R == 3) dft3<T, T2>(w, s);
         if (R == 4) dft4<T, T2>(w, s);
@@ -464,8 +479,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
 
 // ------------------------------------------------------------------------
 // x lines: inverse x transform of the 8 (12 with Z) transform inputs, phys2
-// algebra, forward x transform of the 11 (12) outputs, for 2*PAIRS lines
-// (fixed z, consecutive y) per 512-thread workgroup.  Two real lines share
+// algebra, forward x transform of the 11 (12) outputs, for 2*pairs lines
+// (fixed z, consecutive y) per 256-thread workgroup.  Two real lines share
 // one complex FFT.  The algebra runs in two passes so that only rho*u_i, T
 // and one accumulator per point stay in registers:
 //   pass 1: T, rho u, rho v, rho w -> u_i = rho u_i T and the six rho u_i u_j
@@ -525,65 +540,91 @@ __device__ __forceinline__ void xl_fetch(const FftArgs& a, const T2* __restrict_
   }
 }
 
-// lines of field `src` (W layout [i][z][j]) -> Hermitian-packed complex
-// rows in A (row p = lines j0+2p, j0+2p+1), then the inverse FFT.  Returns
-// the buffer holding the physical pairs.
-template <typename T, typename T2, int XR, int NN>
-__device__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src, T2* A, T2* B, int z,
-                          int j0, int pairs) {
-  XlIn<T2, XR> v;
-  xl_fetch<T2, XR, NN>(a, src, z, j0, pairs, v);
+// Fields per x transform.  The rows of field f are f*pairs ..
+// f*pairs+pairs-1, so one transform of 2*pairs rows keeps every thread of
+// a radix pass busy where pairs alone is too few rows: FP64 at 512 points
+// has 2 pairs, 128 radix-8 butterflies for 256 threads (two fields per
+// transform measured 5.13 against 5.58 ms per x-line launch at
+// 512x513x256).  FP32 has 4 pairs and gains nothing from two fields.
+#ifndef LOMA_XL_DUAL64
+#define LOMA_XL_DUAL64 1
+#endif
+#ifndef LOMA_XL_DUAL32
+#define LOMA_XL_DUAL32 0
+#endif
+template <typename T>
+constexpr int xl_nf() {
+  return (sizeof(T) == 8 ? LOMA_XL_DUAL64 : LOMA_XL_DUAL32) ? 2 : 1;
+}
+
+// lines of NF fields (W layout [i][z][j]; src1 unused for NF = 1) ->
+// Hermitian-packed complex rows in A (row f*pairs + p = lines j0+2p,
+// j0+2p+1 of field f), then the inverse FFT.  Returns the buffer holding the
+// physical pairs.
+template <typename T, typename T2, int XR, int NN, int NF>
+__device__ __forceinline__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src0,
+                          const T2* __restrict__ src1, T2* A, T2* B, int z, int j0, int pairs) {
+  XlIn<T2, XR> v[NF];
+  xl_fetch<T2, XR, NN>(a, src0, z, j0, pairs, v[0]);
+  if constexpr (NF > 1) xl_fetch<T2, XR, NN>(a, src1, z, j0, pairs, v[1]);
   const int N = NN ? NN : a.rc.mgalx, np = NN ? NN / 3 : a.rc.nplanes;
   const int lp = __builtin_ctz(pairs);
   const int RS = xl_row_stride<NN>(N);
   // the dealiased band np..N-np is zero
   const int nzero = N - 2 * np + 1;
-  for (int it = threadIdx.x; it < pairs * nzero; it += XL_THREADS) {
+  for (int it = threadIdx.x; it < NF * pairs * nzero; it += XL_THREADS) {
     const int p = it / nzero, q = it - p * nzero;
     A[lds_i(p * RS + np + q)] = T2{0, 0};
   }
 #pragma unroll
-  for (int r = 0; r < XR; ++r) {
-    const int it = threadIdx.x + r * XL_THREADS;
-    if (it < np * pairs) {
-      const int ii = it >> lp, p = it & (pairs - 1);
-      const T2 ca = v.a[r], cb = v.b[r];
-      const int rb = p * RS;
-      if (ii == 0) {
-        // the imaginary part of the kx=0 coefficient is ignored (rft, rftsingle.f:87-91)
-        A[lds_i(rb)] = T2{ca.x, cb.x};
-      } else {
-        A[lds_i(rb + ii)] = T2{ca.x - cb.y, ca.y + cb.x};           // X_a + i X_b
-        A[lds_i(rb + N - ii)] = T2{ca.x + cb.y, -ca.y + cb.x};      // conj(X_a) + i conj(X_b)
+  for (int f = 0; f < NF; ++f) {
+#pragma unroll
+    for (int r = 0; r < XR; ++r) {
+      const int it = threadIdx.x + r * XL_THREADS;
+      if (it < np * pairs) {
+        const int ii = it >> lp, p = it & (pairs - 1);
+        const T2 ca = v[f].a[r], cb = v[f].b[r];
+        const int rb = (f * pairs + p) * RS;
+        if (ii == 0) {
+          // the imaginary part of the kx=0 coefficient is ignored (rft, rftsingle.f:87-91)
+          A[lds_i(rb)] = T2{ca.x, cb.x};
+        } else {
+          A[lds_i(rb + ii)] = T2{ca.x - cb.y, ca.y + cb.x};           // X_a + i X_b
+          A[lds_i(rb + N - ii)] = T2{ca.x + cb.y, -ca.y + cb.x};      // conj(X_a) + i conj(X_b)
+        }
       }
     }
   }
   __syncthreads();
-  return fft_any<T, T2, NN, XL_THREADS>(A, B, N, pairs, a.plx, (const T2*)a.twx, +1);
+  return fft_any<T, T2, NN, XL_THREADS>(A, B, N, NF * pairs, a.plx, (const T2*)a.twx, +1);
 }
 
-// forward FFT of A (filled by the caller, then synchronised here) and
-// Hermitian unpacking of the pairs into field `dst` (scaled by 1/N)
-template <typename T, typename T2, int NN>
-__device__ void xl_forward(const FftArgs& a, T2* __restrict__ dst, T2* A, T2* B, int z,
-                           int j0, int pairs) {
+// forward FFT of A (rows of NF outputs, filled by the caller, then
+// synchronised here) and Hermitian unpacking of the pairs into fields dst0
+// (rows 0..pairs-1) and dst1 (rows pairs..2*pairs-1), scaled by 1/N
+template <typename T, typename T2, int NN, int NF>
+__device__ __forceinline__ void xl_forward(const FftArgs& a, T2* __restrict__ dst0, T2* __restrict__ dst1,
+                           T2* A, T2* B, int z, int j0, int pairs) {
   const int N = NN ? NN : a.rc.mgalx, np = NN ? NN / 3 : a.rc.nplanes, my = a.rc.my,
             mgz = a.rc.mgalz;
   const int lp = __builtin_ctz(pairs);
   const int RS = xl_row_stride<NN>(N);
   __syncthreads();
-  const T2* R = fft_any<T, T2, NN, XL_THREADS>(A, B, N, pairs, a.plx, (const T2*)a.twx, -1);
+  const T2* R = fft_any<T, T2, NN, XL_THREADS>(A, B, N, NF * pairs, a.plx, (const T2*)a.twx, -1);
   const T hs = (T)(0.5 / (double)N);
-  for (int it = threadIdx.x; it < np * pairs; it += XL_THREADS) {
-    const int ii = it >> lp, p = it & (pairs - 1);
+  const int per = np * pairs;
+  for (int it = threadIdx.x; it < NF * per; it += XL_THREADS) {
+    const int f = NF > 1 && it >= per ? 1 : 0, jt = it - f * per;
+    const int ii = jt >> lp, p = jt & (pairs - 1);
     const int ja = j0 + 2 * p, jb = ja + 1;
-    const int rb = p * RS;
+    const int rb = (f * pairs + p) * RS;
     const T2 zk = R[lds_i(rb + ii)];
     T2 zc = R[lds_i(rb + (N - ii) % N)];
     zc.y = -zc.y;
     T2 ya{(zk.x + zc.x) * hs, (zk.y + zc.y) * hs};
     T2 yb{(zk.y - zc.y) * hs, -(zk.x - zc.x) * hs};  // (zk - zc) / (2i)
     if (ii == 0) { ya.y = 0; yb.y = 0; }
+    T2* dst = f ? dst1 : dst0;
     const size_t base = ((size_t)ii * mgz + z) * a.rc.myp;
     if (ja < my) dst[base + ja] = ya;
     if (jb < my) dst[base + jb] = yb;
@@ -643,7 +684,8 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
   const int z = tile / nyb, j0 = (tile - z * nyb) * a.L;
   T2* A = (T2*)smem;
   const int RS = xl_row_stride<NN>(N);
-  T2* B = A + lds_entries(pairs * RS);
+  constexpr int XL_NF = xl_nf<T>();
+  T2* B = A + lds_entries(XL_NF * pairs * RS);
   constexpr int NIN = WZ ? 12 : 8;
   const double ipe = 1.0 / rc.peclet;
   const XlRk k = xl_rk(a.sc->dt, a.rkstep);
@@ -689,36 +731,55 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
     }
   }
   __syncthreads();
+  // XR items of the np * pairs input coefficients per thread
+  constexpr int XR = (EPT + 2) / 3;
   auto ent = [&](int m) { return x0 + TP * m; };  // x of entry m (valid if < N)
-  auto grab = [&](const T2* R, T (&v)[EPT][2]) {
+  // this thread's entries of field f's rows (no barrier)
+  auto grab_at = [&](const T2* R, int f, T (&v)[EPT][2]) {
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
       const int x = ent(m);
       T2 q = T2{0, 0};
-      if (x < N) q = R[lds_i(p * RS + x)];
+      if (x < N) q = R[lds_i((f * pairs + p) * RS + x)];
       v[m][0] = q.x;
       v[m][1] = q.y;
     }
-    __syncthreads();
   };
-  auto put = [&](const T (&v)[EPT][2]) {
+  auto put_at = [&](int f, const T (&v)[EPT][2]) {
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
       const int x = ent(m);
-      if (x < N) A[lds_i(p * RS + x)] = T2{v[m][0], v[m][1]};
+      if (x < N) A[lds_i((f * pairs + p) * RS + x)] = T2{v[m][0], v[m][1]};
     }
+  };
+  // physical lines of fields s0 and s1 into v0 and v1
+  auto inv2 = [&](const T2* s0, const T2* s1, T (&v0)[EPT][2], T (&v1)[EPT][2]) {
+    if constexpr (XL_NF == 2) {
+      const T2* R = xl_inverse<T, T2, XR, NN, 2>(a, s0, s1, A, B, z, j0, pairs);
+      grab_at(R, 0, v0);
+      grab_at(R, 1, v1);
+      __syncthreads();
+    } else {
+      const T2* R = xl_inverse<T, T2, XR, NN, 1>(a, s0, nullptr, A, B, z, j0, pairs);
+      grab_at(R, 0, v0);
+      __syncthreads();
+      R = xl_inverse<T, T2, XR, NN, 1>(a, s1, nullptr, A, B, z, j0, pairs);
+      grab_at(R, 0, v1);
+      __syncthreads();
+    }
+  };
+  auto inv1 = [&](const T2* s0, T (&v0)[EPT][2]) {
+    const T2* R = xl_inverse<T, T2, XR, NN, 1>(a, s0, nullptr, A, B, z, j0, pairs);
+    grab_at(R, 0, v0);
+    __syncthreads();
   };
   T2* const* wb = (T2* const*)a.wbuf;
   T2* const* wo = (T2* const*)a.wout;
 
   // ---- pass 1: T, rho u_i -> u_i, rho u_i u_j; CFL maxima ----
   T tt[EPT][2], ru[EPT][2], rv[EPT][2], rw[EPT][2];  // perturbations
-  // XR items of the np * pairs input coefficients per thread
-  constexpr int XR = (EPT + 2) / 3;
-  grab(xl_inverse<T, T2, XR, NN>(a, wb[7], A, B, z, j0, pairs), tt);
-  grab(xl_inverse<T, T2, XR, NN>(a, wb[0], A, B, z, j0, pairs), ru);
-  grab(xl_inverse<T, T2, XR, NN>(a, wb[1], A, B, z, j0, pairs), rv);
-  grab(xl_inverse<T, T2, XR, NN>(a, wb[2], A, B, z, j0, pairs), rw);
+  inv2(wb[7], wb[0], tt, ru);
+  inv2(wb[1], wb[2], rv, rw);
   // profiles of rho u, rho v, rho w, T for the two lines
   double pu[2], pv[2], pw[2], pT[2];
 #pragma unroll
@@ -760,12 +821,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
   // minus the product of the profiles, expanded exactly so that every term
   // carries a perturbation factor:  (P+p)(Q+q)(R+r) - PQR
   //   = p (Q+q)(R+r) + P (q (R+r) + Q r)
-#pragma unroll 1
-  for (int q = 0; q < 9; ++q) {
-    opaque_regs(tt);
-    opaque_regs(ru);
-    opaque_regs(rv);
-    opaque_regs(rw);
+  auto fill = [&](int q, int f) {  // output q into the rows of field f
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
       const int x = ent(m);
@@ -790,56 +846,102 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
         }
         val[h] = v;
       }
-      A[lds_i(p * RS + x)] = T2{val[0], val[1]};
+      A[lds_i((f * pairs + p) * RS + x)] = T2{val[0], val[1]};
     }
-    xl_forward<T, T2, NN>(a, wo[q], A, B, z, j0, pairs);
+  };
+#pragma unroll 1
+  for (int q = 0; q < 8; q += 2) {
+    opaque_regs(tt);
+    opaque_regs(ru);
+    opaque_regs(rv);
+    opaque_regs(rw);
+    if constexpr (XL_NF == 2) {
+      fill(q, 0);
+      fill(q + 1, 1);
+      xl_forward<T, T2, NN, 2>(a, wo[q], wo[q + 1], A, B, z, j0, pairs);
+    } else {
+      fill(q, 0);
+      xl_forward<T, T2, NN, 1>(a, wo[q], nullptr, A, B, z, j0, pairs);
+      fill(q + 1, 0);
+      xl_forward<T, T2, NN, 1>(a, wo[q + 1], nullptr, A, B, z, j0, pairs);
+    }
   }
+  fill(8, 0);
+  xl_forward<T, T2, NN, 1>(a, wo[8], nullptr, A, B, z, j0, pairs);
 
   // ---- pass 2: r = T (lap T / Pe - rho u . grad T)  (cross.f:1652-1673) ----
   // acc holds the perturbation part; the profile part is per line
   T acc[EPT][2], accz[EPT][2];
-  {
-    T f[EPT][2];
-    grab(xl_inverse<T, T2, XR, NN>(a, wb[3], A, B, z, j0, pairs), f);  // lap T
+  // gradient terms: T_x (4), T_y (5), T_z (6); Z_x (8), Z_y (9), Z_z (10):
+  // (R + r)(G + g) - R G = (R + r) g + r G
+  auto grad_term = [&](int c, int gq, const T (&fl)[EPT][2], T (&ac)[EPT][2]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const T R_ = (T)(c == 0 ? pu[h] : (c == 1 ? pv[h] : pw[h])), G_ = (T)prof(h, gq);
+#pragma unroll
+      for (int m = 0; m < EPT; ++m) {
+        const T rc_ = c == 0 ? ru[m][h] : (c == 1 ? rv[m][h] : rw[m][h]);
+        ac[m][h] -= (rc_ + R_) * fl[m][h] + rc_ * G_;
+      }
+    }
+  };
+  if constexpr (XL_NF == 2 && !WZ) {
+    T f[EPT][2], g[EPT][2];
+    inv2(wb[3], wb[4], f, g);  // lap T, T_x
 #pragma unroll
     for (int m = 0; m < EPT; ++m)
 #pragma unroll
       for (int h = 0; h < 2; ++h) acc[m][h] = f[m][h] * (T)ipe;
-    if constexpr (WZ) {
-      grab(xl_inverse<T, T2, XR, NN>(a, wb[11], A, B, z, j0, pairs), f);  // lap Z
+    grad_term(0, 4, g, acc);
+    opaque_regs(ru);
+    opaque_regs(rv);
+    opaque_regs(rw);
+    inv2(wb[5], wb[6], f, g);  // T_y, T_z
+    grad_term(1, 5, f, acc);
+    grad_term(2, 6, g, acc);
+  } else if constexpr (XL_NF == 2) {
+    T f[EPT][2], g[EPT][2];
+    inv2(wb[3], wb[11], f, g);  // lap T, lap Z
 #pragma unroll
-      for (int m = 0; m < EPT; ++m)
+    for (int m = 0; m < EPT; ++m)
 #pragma unroll
-        for (int h = 0; h < 2; ++h) accz[m][h] = f[m][h] * (T)ipe;
-    }
-    // gradient terms: T_x (4), T_y (5), T_z (6); Z_x (8), Z_y (9), Z_z (10):
-    // (R + r)(G + g) - R G = (R + r) g + r G
+      for (int h = 0; h < 2; ++h) {
+        acc[m][h] = f[m][h] * (T)ipe;
+        accz[m][h] = g[m][h] * (T)ipe;
+      }
 #pragma unroll 1
     for (int c = 0; c < 3; ++c) {
       opaque_regs(ru);
       opaque_regs(rv);
       opaque_regs(rw);
-      grab(xl_inverse<T, T2, XR, NN>(a, wb[4 + c], A, B, z, j0, pairs), f);  // T_x, T_y, T_z
+      inv2(wb[4 + c], wb[8 + c], f, g);  // T_c, Z_c
+      grad_term(c, 4 + c, f, acc);
+      grad_term(c, 8 + c, g, accz);
+    }
+  } else {
+    T f[EPT][2];
+    inv1(wb[3], f);  // lap T
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const T R_ = (T)(c == 0 ? pu[h] : (c == 1 ? pv[h] : pw[h])), G_ = (T)prof(h, 4 + c);
+    for (int m = 0; m < EPT; ++m)
 #pragma unroll
-        for (int m = 0; m < EPT; ++m) {
-          const T rc_ = c == 0 ? ru[m][h] : (c == 1 ? rv[m][h] : rw[m][h]);
-          acc[m][h] -= (rc_ + R_) * f[m][h] + rc_ * G_;
-        }
-      }
+      for (int h = 0; h < 2; ++h) acc[m][h] = f[m][h] * (T)ipe;
+    if constexpr (WZ) {
+      inv1(wb[11], f);  // lap Z
+#pragma unroll
+      for (int m = 0; m < EPT; ++m)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) accz[m][h] = f[m][h] * (T)ipe;
+    }
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+      opaque_regs(ru);
+      opaque_regs(rv);
+      opaque_regs(rw);
+      inv1(wb[4 + c], f);  // T_x, T_y, T_z
+      grad_term(c, 4 + c, f, acc);
       if constexpr (WZ) {
-        grab(xl_inverse<T, T2, XR, NN>(a, wb[8 + c], A, B, z, j0, pairs), f);  // Z_x, Z_y, Z_z
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const T R_ = (T)(c == 0 ? pu[h] : (c == 1 ? pv[h] : pw[h])), G_ = (T)prof(h, 8 + c);
-#pragma unroll
-          for (int m = 0; m < EPT; ++m) {
-            const T rc_ = c == 0 ? ru[m][h] : (c == 1 ? rv[m][h] : rw[m][h]);
-            accz[m][h] -= (rc_ + R_) * f[m][h] + rc_ * G_;
-          }
-        }
+        inv1(wb[8 + c], f);  // Z_x, Z_y, Z_z
+        grad_term(c, 8 + c, f, accz);
       }
     }
   }
@@ -878,13 +980,19 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
     }
   }
   // outputs 9 drho, 10 DT, 11 DZ
-  put(rv);
-  xl_forward<T, T2, NN>(a, wo[9], A, B, z, j0, pairs);
-  put(ru);
-  xl_forward<T, T2, NN>(a, wo[10], A, B, z, j0, pairs);
+  if constexpr (XL_NF == 2) {
+    put_at(0, rv);
+    put_at(1, ru);
+    xl_forward<T, T2, NN, 2>(a, wo[9], wo[10], A, B, z, j0, pairs);
+  } else {
+    put_at(0, rv);
+    xl_forward<T, T2, NN, 1>(a, wo[9], nullptr, A, B, z, j0, pairs);
+    put_at(0, ru);
+    xl_forward<T, T2, NN, 1>(a, wo[10], nullptr, A, B, z, j0, pairs);
+  }
   if constexpr (WZ) {
-    put(rw);
-    xl_forward<T, T2, NN>(a, wo[11], A, B, z, j0, pairs);
+    put_at(0, rw);
+    xl_forward<T, T2, NN, 1>(a, wo[11], nullptr, A, B, z, j0, pairs);
   }
 }
 
@@ -916,7 +1024,8 @@ static size_t zfft_lds(bool fp32, const FftArgs& a) {
   return (size_t)a.J * lds_row_stride(a.rc.mgalz, fp32 ? LOMA_ZF_PAD32 : 1) * (fp32 ? 8 : 16);
 }
 static size_t xline_lds(bool fp32, const FftArgs& a) {
-  return (size_t)2 * lds_entries((a.L / 2) * (a.rc.mgalx + LOMA_XL_RS1)) * (fp32 ? 8 : 16);
+  const int nf = fp32 ? xl_nf<float>() : xl_nf<double>();
+  return (size_t)2 * lds_entries(nf * (a.L / 2) * (a.rc.mgalx + LOMA_XL_RS1)) * (fp32 ? 8 : 16);
 }
 
 // kernel instance for a transform length: compile-time plans for the

@@ -56,12 +56,13 @@ def test_substeps_fp64(gpu, oracle_mod):
         _compare(run, S, 1e-11, f"substep {rk}")
 
 
-@pytest.mark.parametrize("shape", [(64, 129, 64), (48, 97, 36)])
-def test_steps_fp64(gpu, oracle_mod, shape):
+@pytest.mark.parametrize("shape,zflag", [((64, 129, 64), True), ((48, 97, 36), True),
+                                         ((48, 97, 36), False)])
+def test_steps_fp64(gpu, oracle_mod, shape, zflag):
     from loma_phd_amd import capi
     cfg = _cfg(*shape, Re=160.0, s=2.0, seed=5)
-    print(f"10 steps fp64 {shape}", flush=True)
-    run, S, g = _setup(cfg, capi.LOMA_FP64)
+    print(f"10 steps fp64 {shape} z={zflag}", flush=True)
+    run, S, g = _setup(cfg, capi.LOMA_FP64, zflag)
     for n in range(10):
         run.advance(1)
         assert S.step(1) == 0
