@@ -198,7 +198,7 @@ def bench_ensemble(nruns, steps, warmup, seed=7, workers=8):
     return out
 
 
-def bench_served(window, rate=2.0, seed=11, max_slots=32, workers=4):
+def bench_served(window, rate=2.0, seed=11, max_slots=32, workers=8):
     """Served ensemble: Poisson arrivals (rate per s) over `window` seconds,
     grids from 64x129x64 to 384x385x192, density ratios U[1,8], step budgets
     U[200,2000], admission by device footprint; runs still resident at the
@@ -214,7 +214,10 @@ def bench_served(window, rate=2.0, seed=11, max_slots=32, workers=4):
                      workers=workers).run(reqs, window=window)
     finally:
         be.close()
+    prep = sorted(b - a for n, a, b in be.spans if n in ("prepare", "seed"))
     return dict(window_s=window, arrival_rate_per_s=rate, requests=len(reqs),
+                prep_s=dict(n=len(prep), p50=round(prep[len(prep) // 2], 2) if prep else None,
+                            max=round(prep[-1], 2) if prep else None),
                 gpts_per_s=round(rep["gp_steps_per_s"], 1), steps=rep["steps"],
                 p50_ms=round(rep["p50_ms"], 3), p99_ms=round(rep["p99_ms"], 3),
                 device_p50_ms=round(rep["device_p50_ms"], 3),
@@ -224,7 +227,10 @@ def bench_served(window, rate=2.0, seed=11, max_slots=32, workers=4):
                 slow_loop_passes=rep["slow_loop_passes"],
                 slowest_loop_pass_ms=round(rep["slowest_loop_pass_ms"], 1),
                 slowest_call_ms={k: round(v, 1) for k, v in rep["slowest_call_ms"].items()},
+                slow_calls=rep["slow_calls"], slow_call_overlaps=rep["slow_call_overlaps"],
+                slow_pass_overlaps=rep["slow_pass_overlaps"],
                 reuse="finished runs kept per grid and re-targeted",
+                preparation="seeded draws on a host thread, state built on the device (loma_seed)",
                 budget_bytes=budget, outcomes=rep["counts"], precision="mixed",
                 latency="host wall time from step enqueue to observed completion")
 

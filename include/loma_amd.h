@@ -88,6 +88,23 @@ int loma_download(loma_run *run, void *vor, void *phi, void *psi, void *scal,
 int loma_upload_z(loma_run *run, const void *z, int fp64);
 int loma_download_z(loma_run *run, void *z, int fp64);
 
+/* Seeded initial condition built on the device, in place of an upload:
+ * the temporal mixing layer of trunk/initloma3/src/main.f:95-609
+ * (perturbation :123-291, rho00/U/Z profiles :360-373, psi and the
+ * spectral state :452-463).  coef holds the seeded draws, complex
+ * (interleaved doubles) [3][nplanes][mz][nymax]: for u, v, w the
+ * coefficient amp*exp(2 pi i phs) of exp(i (n+1) (2 kh / nymax) y); s is the
+ * density ratio, amp the target rms of each velocity component on the
+ * centre plane, env_width the Gaussian y-window width, k0 the peak
+ * wavenumber of the spectrum shape (kh/k0)^2 exp(-(kh/k0)^2).  Then as
+ * loma_upload: history reset, boundary densities, 00 modes (and Z if the
+ * run has it).  The draws are staged in a transform work buffer, 48*nymax
+ * bytes per mode; my >= 48 always fits, smaller grids may be refused with
+ * LOMA_ERR_ARG.  The reference seeds from the wall clock
+ * (main.f:36-45); the caller's seeded generator replaces it. */
+int loma_seed(loma_run *run, const double *coef, int nymax, double s,
+              double amp, double env_width, double k0);
+
 /* Time advance: nsteps full RK3 steps (cross.f:225-1223 loop body),
  * device resident.  Returns LOMA_ERR_NAN if the state became non-finite. */
 int loma_advance(loma_run *run, int nsteps);

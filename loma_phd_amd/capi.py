@@ -29,7 +29,7 @@ EXPORTS = [
     "loma_inty8_dev", "loma_deryr2", "loma_deryyr2", "loma_lapvdvhom", "loma_solvepsi",
     "loma_fourz_dev", "loma_band5_solve_dev", "loma_advance_many", "loma_advance_async",
     "loma_tiles", "loma_footprint", "loma_step_begin", "loma_step_poll",
-    "loma_visc_dev", "loma_visc", "loma_set_physics",
+    "loma_visc_dev", "loma_visc", "loma_set_physics", "loma_seed",
     "loma_sync", "loma_stream", "loma_profile", "loma_profile_read",
 ]
 KERNEL_CLASSES = ["phaseA", "zfft_inv", "xline", "zfft_fwd", "phaseC", "finish", "phaseC2"]
@@ -66,6 +66,7 @@ def lib():
             "loma_upload": ([_vp, _vp, _vp, _vp, _vp, _dp, _dp, _dp, i], i),
             "loma_download": ([_vp, _vp, _vp, _vp, _vp, _dp, _dp, _dp, i], i),
             "loma_upload_z": ([_vp, _vp, i], i),
+            "loma_seed": ([_vp, _vp, i, d, d, d, d], i),
             "loma_download_z": ([_vp, _vp, i], i),
             "loma_advance": ([_vp, i], i),
             "loma_substep": ([_vp, i], i),
@@ -196,6 +197,15 @@ class Run:
         if z is not None:
             zz = np.ascontiguousarray(z, dtype=np.complex128).reshape(self.shape())
             check(lib().loma_upload_z(self._h, zz.ctypes.data_as(_vp), 1))
+
+    def seed(self, coef, s, amp, env_width, k0):
+        """Seeded initial condition built on the device (loma_seed) in place
+        of an upload; coef: complex (3, nplanes, mz, nymax) draws."""
+        c = np.ascontiguousarray(coef, dtype=np.complex128)
+        if c.ndim != 4 or c.shape[:3] != (3,) + self.shape()[:2]:
+            raise ValueError(f"coef shape {c.shape} is not (3, nplanes, mz, nymax)")
+        check(lib().loma_seed(self._h, c.ctypes.data_as(_vp), c.shape[3], float(s),
+                              float(amp), float(env_width), float(k0)))
 
     def download(self):
         f = [np.zeros(self.shape(), np.complex128) for _ in range(4)]

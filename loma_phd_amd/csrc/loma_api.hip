@@ -57,6 +57,7 @@ struct loma_run {
   double Re = 0, alp = 0, bet = 0, pr = 0.7, cfl = 0.5, dt0 = 1e-4;
   int ncfl = 1;
   FdOps fd;
+  std::vector<double> y;  // the y grid (host copy)
   RunConst rc{};
   hipStream_t stream = nullptr;
   // device memory: one allocation (arena) carved by dalloc
@@ -174,15 +175,25 @@ static size_t plan_bytes(int mgalx, int my, int mgalz, bool fp32, bool with_z) {
   return b;
 }
 
+static void* pinned_get(size_t bytes) {
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+  return p;
+}
+static void pinned_put(void* p, size_t) {
+  if (p) hipHostFree(p);
+}
+
 // Pinned host staging for transfers of one complex field, grown on demand;
 // copies go on the run's own stream, so they hold the runtime only while
 // they are enqueued.
 static int stage_host(loma_run* r, size_t ndoubles) {
   if (r->hstage_n >= ndoubles) return LOMA_OK;
-  if (r->hstage) hipHostFree(r->hstage);
+  pinned_put(r->hstage, r->hstage_n * sizeof(double));
   r->hstage = nullptr;
   r->hstage_n = 0;
-  HIPCHK(hipHostMalloc((void**)&r->hstage, ndoubles * sizeof(double), hipHostMallocDefault));
+  r->hstage = (double*)pinned_get(ndoubles * sizeof(double));
+  if (!r->hstage) { set_error("pinned host allocation failed"); return LOMA_ERR_OOM; }
   r->hstage_n = ndoubles;
   return LOMA_OK;
 }
@@ -256,8 +267,8 @@ int loma_free(loma_run* r) {
     if (r->gexec[q]) hipGraphExecDestroy(r->gexec[q]);
     if (r->sev[q]) hipEventDestroy(r->sev[q]);
   }
-  if (r->hsc) hipHostFree(r->hsc);
-  if (r->hstage) hipHostFree(r->hstage);
+  pinned_put(r->hsc, sizeof(DevScalars));
+  pinned_put(r->hstage, r->hstage_n * sizeof(double));
   if (r->arena) hipFree(r->arena);
   if (r->owns_stream && r->stream) hipStreamDestroy(r->stream);
   delete r;
@@ -307,6 +318,7 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   if (r->M < 0) { set_error("loma_init: my too large"); delete r; return LOMA_ERR_ARG; }
   int rc = fd_build(my, y, r->fd);
   if (rc) { set_error("loma_init: FD setup failed"); delete r; return rc; }
+  r->y.assign(y, y + my);
   // the run's own non-blocking stream carries its set-up as well as its steps
   if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("loma_init: stream creation failed"); delete r; return LOMA_ERR_DEVICE;
@@ -596,35 +608,12 @@ static int get_field(loma_run* r, const double2* src, size_t S, int fp64, void* 
   return LOMA_OK;
 }
 
-int loma_upload(loma_run* r, const void* vor, const void* phi, const void* psi,
-                const void* scal, const double* u00, const double* v00, const double* w00,
-                int fp64) {
-  if (!r || !vor || !phi || !psi || !scal || !u00 || !v00 || !w00) {
-    set_error("loma_upload: null argument");
-    return LOMA_ERR_ARG;
-  }
-  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
+// Shared end of loma_upload and loma_seed, once the spectral fields are on
+// the device: history and work buffers zeroed, 00 modes and scalars set,
+// the step interface's events made, captured step graphs dropped.
+static int upload_tail(loma_run* r, const double* u00, const double* v00, const double* w00) {
   const size_t S = (size_t)r->nplanes * r->mz * r->my;
   const int my = r->my;
-  int e = stage_host(r, std::max(2 * S, (size_t)3 * my));
-  if (e) return e;
-  HIPCHK(hipStreamSynchronize(r->stream));  // nothing of the run may still read the state
-  // boundary densities from the 00 mode of T (cross.f:208-213)
-  {
-    const double T0 = fp64 ? ((const double*)scal)[0] : ((const float*)scal)[0];
-    const double T1 = fp64 ? ((const double*)scal)[2 * (my - 1)]
-                           : ((const float*)scal)[2 * (my - 1)];
-    if (!(T0 > 0.0) || !(T1 > 0.0)) {
-      set_error("loma_upload: 00 mode of T must be positive at both ends");
-      return LOMA_ERR_ARG;
-    }
-    r->rc.rhobot = 1.0 / T0;
-    r->rc.rhotop = 1.0 / T1;
-  }
-  const void* src[4] = {vor, phi, psi, scal};
-  double2* dst[4] = {r->vor, r->phi, r->psi, r->scal};
-  for (int q = 0; q < 4; ++q)
-    if ((e = put_field(r, src[q], fp64, S, dst[q]))) return e;
   // history and work buffers start at zero (main.f:162-176)
   hipStream_t st = r->stream;
   const size_t P8 = sizeof(double) * (size_t)r->mgalx * my * r->mgalz;
@@ -654,13 +643,125 @@ int loma_upload(loma_run* r, const void* vor, const void* phi, const void* psi,
     HIPCHK(hipEventCreate(&r->sev[0]));
     HIPCHK(hipEventCreate(&r->sev[1]));
   }
-  if (!r->hsc) HIPCHK(hipHostMalloc((void**)&r->hsc, sizeof(DevScalars), hipHostMallocDefault));
+  if (!r->hsc) {
+    r->hsc = (DevScalars*)pinned_get(sizeof(DevScalars));
+    if (!r->hsc) { set_error("loma_upload: pinned host allocation failed"); return LOMA_ERR_OOM; }
+  }
   r->inflight = false;
   r->istep = 0;
   r->uploaded = true;
   for (int q = 0; q < 2; ++q)
     if (r->gexec[q]) { hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
   return LOMA_OK;
+}
+
+int loma_upload(loma_run* r, const void* vor, const void* phi, const void* psi,
+                const void* scal, const double* u00, const double* v00, const double* w00,
+                int fp64) {
+  if (!r || !vor || !phi || !psi || !scal || !u00 || !v00 || !w00) {
+    set_error("loma_upload: null argument");
+    return LOMA_ERR_ARG;
+  }
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
+  const size_t S = (size_t)r->nplanes * r->mz * r->my;
+  const int my = r->my;
+  int e = stage_host(r, std::max(2 * S, (size_t)3 * my));
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(r->stream));  // nothing of the run may still read the state
+  // boundary densities from the 00 mode of T (cross.f:208-213)
+  {
+    const double T0 = fp64 ? ((const double*)scal)[0] : ((const float*)scal)[0];
+    const double T1 = fp64 ? ((const double*)scal)[2 * (my - 1)]
+                           : ((const float*)scal)[2 * (my - 1)];
+    if (!(T0 > 0.0) || !(T1 > 0.0)) {
+      set_error("loma_upload: 00 mode of T must be positive at both ends");
+      return LOMA_ERR_ARG;
+    }
+    r->rc.rhobot = 1.0 / T0;
+    r->rc.rhotop = 1.0 / T1;
+  }
+  const void* src[4] = {vor, phi, psi, scal};
+  double2* dst[4] = {r->vor, r->phi, r->psi, r->scal};
+  for (int q = 0; q < 4; ++q)
+    if ((e = put_field(r, src[q], fp64, S, dst[q]))) return e;
+  return upload_tail(r, u00, v00, w00);
+}
+
+int loma_seed(loma_run* r, const double* coef, int nymax, double s, double amp,
+              double env_width, double k0) {
+  if (!r || !coef) { set_error("loma_seed: null argument"); return LOMA_ERR_ARG; }
+  if (nymax < 1 || nymax > 64 || !(s > 0.0) || !(env_width > 0.0) || !(k0 > 0.0) ||
+      !(amp >= 0.0)) {
+    set_error("loma_seed: nymax in [1,64], s, env_width, k0 > 0 and amp >= 0 required");
+    return LOMA_ERR_ARG;
+  }
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
+  const int my = r->my;
+  const size_t nmodes = (size_t)r->nplanes * r->mz;
+  const size_t es = r->fp32 ? 8 : 16;
+  const size_t wbytes = (size_t)r->nplanes * r->mgalz * r->rc.myp * es;
+  const size_t cbytes = 3 * nmodes * nymax * 16;
+  // the coefficients, the per-mode energies and y go to transform work
+  // buffers, which the next step overwrites anyway
+  if (cbytes > wbytes || nmodes * 3 * 8 > wbytes || (size_t)my * 8 > wbytes) {
+    set_error("loma_seed: transform work buffer too small for the draws (my >= 48 always fits)");
+    return LOMA_ERR_ARG;
+  }
+  int e = stage_host(r, std::max({cbytes / 8, nmodes * 3, (size_t)3 * my}));
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(r->stream));  // nothing of the run may still read the state
+  hipStream_t st = r->stream;
+  double2* dcoef = (double2*)r->wbuf[0];
+  double* dpart = (double*)r->wbuf[1];
+  double* dy = (double*)r->wbuf[2];
+  std::memcpy(r->hstage, coef, cbytes);
+  HIPCHK(hipMemcpyAsync(dcoef, r->hstage, cbytes, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::memcpy(r->hstage, r->y.data(), sizeof(double) * my);
+  HIPCHK(hipMemcpyAsync(dy, r->hstage, sizeof(double) * my, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  IcArgs a{};
+  a.rc = r->rc;
+  a.T = r->T;
+  a.helm_lu = r->helm_lu;
+  a.lu_stride = r->lu_stride;
+  a.coef = dcoef;
+  a.y = dy;
+  a.partial = dpart;
+  a.vor = r->vor; a.phi = r->phi; a.psi = r->psi; a.scal = r->scal; a.zf = r->zf;
+  a.s = s; a.env_width = env_width; a.k0 = k0;
+  a.scale[0] = a.scale[1] = a.scale[2] = 1.0;
+  a.nymax = nymax;
+  a.with_z = r->with_z ? 1 : 0;
+  // pass 1: centre-plane energy of each mode, summed here in mode order
+  a.pass = 1;
+  LCHK(launch_ic(r->M, a, st));
+  HIPCHK(hipMemcpyAsync(r->hstage, dpart, nmodes * 3 * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int c = 0; c < 3; ++c) {
+    double sum = 0.0;
+    for (size_t w = 0; w < nmodes; ++w) sum += r->hstage[w * 3 + c];
+    const double rms = std::sqrt(sum);
+    a.scale[c] = rms > 0.0 ? amp / rms : 1.0;
+  }
+  // pass 2: the state
+  a.pass = 2;
+  LCHK(launch_ic(r->M, a, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // mean profiles: rho00 = 1 + (s-1)/(s+1) tanh(-2y), U = tanh(2y)/2,
+  // Z = (1 + tanh(2y))/2 (initloma3 main.f:360-373); boundary densities from
+  // the 00 mode of T = 1/rho00 as loma_upload takes them (cross.f:208-213)
+  std::vector<double> u00(my), zero(my, 0.0);
+  const double* yy = r->y.data();
+  auto rho = [&](double y) { return 1.0 + (s - 1.0) / (s + 1.0) * std::tanh(-2.0 * y); };
+  for (int j = 0; j < my; ++j) u00[j] = rho(yy[j]) * (0.5 * std::tanh(2.0 * yy[j]));
+  r->rc.rhobot = 1.0 / (1.0 / rho(yy[0]));
+  r->rc.rhotop = 1.0 / (1.0 / rho(yy[my - 1]));
+  if (r->with_z) {
+    r->rc.zbot = 0.5 * (1.0 + std::tanh(2.0 * yy[0]));
+    r->rc.ztop = 0.5 * (1.0 + std::tanh(2.0 * yy[my - 1]));
+  }
+  return upload_tail(r, u00.data(), zero.data(), zero.data());
 }
 
 int loma_upload_z(loma_run* r, const void* z, int fp64) {

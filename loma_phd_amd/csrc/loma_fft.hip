@@ -185,14 +185,14 @@ __device__ __forceinline__ void twiddle(T2* w, const T2* __restrict__ tw, int jm
   }
 }
 
-template <int R, typename T, typename T2>
+template <int R, typename T, typename T2, int PAD = LOMA_LDS_PAD>
 __device__ __forceinline__ void stockham_item(const T2* __restrict__ src, T2* __restrict__ dst,
                                               int rowbase, int j, int nb, int Ns, int tstep,
                                               const T2* __restrict__ tw, int s) {
   T2 v[R];
   const int jm = j % Ns;
 #pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = src[lds_i(rowbase + j + r * nb)];
+  for (int r = 0; r < R; ++r) v[r] = src[lds_i<PAD>(rowbase + j + r * nb)];
   if (Ns > 1) twiddle<R>(v, tw, jm, tstep, s);
   if (R == 2) dft2<T, T2>(v);
   if (R == 3) dft3<T, T2>(v, s);
@@ -201,14 +201,14 @@ __device__ __forceinline__ void stockham_item(const T2* __restrict__ src, T2* __
   if (R == 8) dft8<T, T2>(v, s);
   const int idxD = rowbase + (j / Ns) * Ns * R + jm;
 #pragma unroll
-  for (int r = 0; r < R; ++r) dst[lds_i(idxD + r * Ns)] = v[r];
+  for (int r = 0; r < R; ++r) dst[lds_i<PAD>(idxD + r * Ns)] = v[r];
 }
 
 // nrows independent length-N transforms of rows stored with stride N in a
 // (LDS, padded layout lds_i); b is scratch of the same size.  Returns the
 // buffer holding the result.  s = -1 forward (e^{-i}), +1 inverse; unscaled.
 // Ends with a barrier.
-template <typename T, typename T2>
+template <typename T, typename T2, int PAD = LOMA_LDS_PAD>
 __device__ T2* fft_rows(T2* a, T2* b, int N, int RS, int nrows, const FftPlan& pl,
                         const T2* __restrict__ tw, int s) {
   int Ns = 1;
@@ -221,11 +221,11 @@ __device__ T2* fft_rows(T2* a, T2* b, int N, int RS, int nrows, const FftPlan& p
       const int row = it / nb, j = it - row * nb;
       const int rb = row * RS;
       switch (R) {
-        case 2: stockham_item<2, T, T2>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
-        case 3: stockham_item<3, T, T2>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
-        case 4: stockham_item<4, T, T2>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
-        case 5: stockham_item<5, T, T2>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
-        default: stockham_item<8, T, T2>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
+        case 2: stockham_item<2, T, T2, PAD>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
+        case 3: stockham_item<3, T, T2, PAD>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
+        case 4: stockham_item<4, T, T2, PAD>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
+        case 5: stockham_item<5, T, T2, PAD>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
+        default: stockham_item<8, T, T2, PAD>(a, b, rb, j, nb, Ns, tstep, tw, s); break;
       }
     }
     __syncthreads();
@@ -305,7 +305,7 @@ constexpr int ct_radix(int m) {
   return m % 8 == 0 ? 8 : (m % 4 == 0 ? 4 : (m % 2 == 0 ? 2 : (m % 3 == 0 ? 3 : 5)));
 }
 
-template <typename T, typename T2, int N, int Ns, int NT, int RS>
+template <typename T, typename T2, int N, int Ns, int NT, int RS, int PAD = LOMA_LDS_PAD>
 __device__ __forceinline__ T2* fft_rows_ct(T2* a, T2* b, int nrows, const T2* __restrict__ tw,
                                            int s) {
   if constexpr (Ns >= N) {
@@ -321,10 +321,10 @@ __device__ __forceinline__ T2* fft_rows_ct(T2* a, T2* b, int nrows, const T2* __
     const int total = nrows * nb;
     for (int it = t0; it < total; it += NT) {
       const int row = it / nb, j = it - row * nb;
-      stockham_item<R, T, T2>(a, b, row * RS, j, nb, Ns, tstep, tw, s);
+      stockham_item<R, T, T2, PAD>(a, b, row * RS, j, nb, Ns, tstep, tw, s);
     }
     __syncthreads();
-    return fft_rows_ct<T, T2, N, Ns * R, NT, RS>(b, a, nrows, tw, s);
+    return fft_rows_ct<T, T2, N, Ns * R, NT, RS, PAD>(b, a, nrows, tw, s);
   }
 }
 
@@ -382,13 +382,13 @@ __device__ __forceinline__ void fft_rows_ip_ct(T2* a, int nrows, const T2* __res
 template <int NN>
 __device__ __forceinline__ int xl_row_stride(int N) { return (NN ? NN : N) + LOMA_XL_RS1; }
 
-template <typename T, typename T2, int NN, int NT>
+template <typename T, typename T2, int NN, int NT, int PAD = LOMA_LDS_PAD>
 __device__ __forceinline__ T2* fft_any(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
                                        const T2* __restrict__ tw, int s) {
   if constexpr (NN > 0)
-    return fft_rows_ct<T, T2, NN, 1, NT, NN + LOMA_XL_RS1>(a, b, nrows, tw, s);
+    return fft_rows_ct<T, T2, NN, 1, NT, NN + LOMA_XL_RS1, PAD>(a, b, nrows, tw, s);
   else
-    return fft_rows<T, T2>(a, b, N, N + LOMA_XL_RS1, nrows, pl, tw, s);
+    return fft_rows<T, T2, PAD>(a, b, N, N + LOMA_XL_RS1, nrows, pl, tw, s);
 }
 template <typename T, typename T2, int E, int NN, int NT, int PAD>
 __device__ __forceinline__ void fft_any_ip(T2* a, int N, int nrows, const FftPlan& pl,
@@ -556,6 +556,20 @@ template <typename T>
 constexpr int xl_nf() {
   return (sizeof(T) == 8 ? LOMA_XL_DUAL64 : LOMA_XL_DUAL32) ? 2 : 1;
 }
+// x-line LDS rows: one pad entry after every eight (lds_i<1>) spreads the
+// strided butterfly writes of the first two radix-8 passes over the banks.
+// FP64 (16-byte entries): 4.91 against 5.20 ms per launch at 512x513x256;
+// FP32 measured slower padded (3.03 against 2.90 ms) and stays unpadded.
+#ifndef LOMA_XL_PAD64
+#define LOMA_XL_PAD64 1
+#endif
+#ifndef LOMA_XL_PAD32
+#define LOMA_XL_PAD32 0
+#endif
+template <typename T>
+constexpr int xl_pad() {
+  return sizeof(T) == 8 ? LOMA_XL_PAD64 : LOMA_XL_PAD32;
+}
 
 // lines of NF fields (W layout [i][z][j]; src1 unused for NF = 1) ->
 // Hermitian-packed complex rows in A (row f*pairs + p = lines j0+2p,
@@ -567,6 +581,7 @@ __device__ __forceinline__ T2* xl_inverse(const FftArgs& a, const T2* __restrict
   XlIn<T2, XR> v[NF];
   xl_fetch<T2, XR, NN>(a, src0, z, j0, pairs, v[0]);
   if constexpr (NF > 1) xl_fetch<T2, XR, NN>(a, src1, z, j0, pairs, v[1]);
+  constexpr int XP = xl_pad<T>();
   const int N = NN ? NN : a.rc.mgalx, np = NN ? NN / 3 : a.rc.nplanes;
   const int lp = __builtin_ctz(pairs);
   const int RS = xl_row_stride<NN>(N);
@@ -574,7 +589,7 @@ __device__ __forceinline__ T2* xl_inverse(const FftArgs& a, const T2* __restrict
   const int nzero = N - 2 * np + 1;
   for (int it = threadIdx.x; it < NF * pairs * nzero; it += XL_THREADS) {
     const int p = it / nzero, q = it - p * nzero;
-    A[lds_i(p * RS + np + q)] = T2{0, 0};
+    A[lds_i<XP>(p * RS + np + q)] = T2{0, 0};
   }
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
@@ -587,16 +602,16 @@ __device__ __forceinline__ T2* xl_inverse(const FftArgs& a, const T2* __restrict
         const int rb = (f * pairs + p) * RS;
         if (ii == 0) {
           // the imaginary part of the kx=0 coefficient is ignored (rft, rftsingle.f:87-91)
-          A[lds_i(rb)] = T2{ca.x, cb.x};
+          A[lds_i<XP>(rb)] = T2{ca.x, cb.x};
         } else {
-          A[lds_i(rb + ii)] = T2{ca.x - cb.y, ca.y + cb.x};           // X_a + i X_b
-          A[lds_i(rb + N - ii)] = T2{ca.x + cb.y, -ca.y + cb.x};      // conj(X_a) + i conj(X_b)
+          A[lds_i<XP>(rb + ii)] = T2{ca.x - cb.y, ca.y + cb.x};           // X_a + i X_b
+          A[lds_i<XP>(rb + N - ii)] = T2{ca.x + cb.y, -ca.y + cb.x};      // conj(X_a) + i conj(X_b)
         }
       }
     }
   }
   __syncthreads();
-  return fft_any<T, T2, NN, XL_THREADS>(A, B, N, NF * pairs, a.plx, (const T2*)a.twx, +1);
+  return fft_any<T, T2, NN, XL_THREADS, XP>(A, B, N, NF * pairs, a.plx, (const T2*)a.twx, +1);
 }
 
 // forward FFT of A (rows of NF outputs, filled by the caller, then
@@ -605,12 +620,13 @@ __device__ __forceinline__ T2* xl_inverse(const FftArgs& a, const T2* __restrict
 template <typename T, typename T2, int NN, int NF>
 __device__ __forceinline__ void xl_forward(const FftArgs& a, T2* __restrict__ dst0, T2* __restrict__ dst1,
                            T2* A, T2* B, int z, int j0, int pairs) {
+  constexpr int XP = xl_pad<T>();
   const int N = NN ? NN : a.rc.mgalx, np = NN ? NN / 3 : a.rc.nplanes, my = a.rc.my,
             mgz = a.rc.mgalz;
   const int lp = __builtin_ctz(pairs);
   const int RS = xl_row_stride<NN>(N);
   __syncthreads();
-  const T2* R = fft_any<T, T2, NN, XL_THREADS>(A, B, N, NF * pairs, a.plx, (const T2*)a.twx, -1);
+  const T2* R = fft_any<T, T2, NN, XL_THREADS, XP>(A, B, N, NF * pairs, a.plx, (const T2*)a.twx, -1);
   const T hs = (T)(0.5 / (double)N);
   const int per = np * pairs;
   for (int it = threadIdx.x; it < NF * per; it += XL_THREADS) {
@@ -618,8 +634,8 @@ __device__ __forceinline__ void xl_forward(const FftArgs& a, T2* __restrict__ ds
     const int ii = jt >> lp, p = jt & (pairs - 1);
     const int ja = j0 + 2 * p, jb = ja + 1;
     const int rb = (f * pairs + p) * RS;
-    const T2 zk = R[lds_i(rb + ii)];
-    T2 zc = R[lds_i(rb + (N - ii) % N)];
+    const T2 zk = R[lds_i<XP>(rb + ii)];
+    T2 zc = R[lds_i<XP>(rb + (N - ii) % N)];
     zc.y = -zc.y;
     T2 ya{(zk.x + zc.x) * hs, (zk.y + zc.y) * hs};
     T2 yb{(zk.y - zc.y) * hs, -(zk.x - zc.x) * hs};  // (zk - zc) / (2i)
@@ -685,7 +701,8 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
   T2* A = (T2*)smem;
   const int RS = xl_row_stride<NN>(N);
   constexpr int XL_NF = xl_nf<T>();
-  T2* B = A + lds_entries(XL_NF * pairs * RS);
+  constexpr int XP = xl_pad<T>();
+  T2* B = A + lds_entries(XL_NF * pairs * RS, XP);
   constexpr int NIN = WZ ? 12 : 8;
   const double ipe = 1.0 / rc.peclet;
   const XlRk k = xl_rk(a.sc->dt, a.rkstep);
@@ -740,7 +757,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
     for (int m = 0; m < EPT; ++m) {
       const int x = ent(m);
       T2 q = T2{0, 0};
-      if (x < N) q = R[lds_i((f * pairs + p) * RS + x)];
+      if (x < N) q = R[lds_i<XP>((f * pairs + p) * RS + x)];
       v[m][0] = q.x;
       v[m][1] = q.y;
     }
@@ -749,7 +766,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
       const int x = ent(m);
-      if (x < N) A[lds_i((f * pairs + p) * RS + x)] = T2{v[m][0], v[m][1]};
+      if (x < N) A[lds_i<XP>((f * pairs + p) * RS + x)] = T2{v[m][0], v[m][1]};
     }
   };
   // physical lines of fields s0 and s1 into v0 and v1
@@ -846,7 +863,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
         }
         val[h] = v;
       }
-      A[lds_i((f * pairs + p) * RS + x)] = T2{val[0], val[1]};
+      A[lds_i<XP>((f * pairs + p) * RS + x)] = T2{val[0], val[1]};
     }
   };
 #pragma unroll 1
@@ -1025,7 +1042,9 @@ static size_t zfft_lds(bool fp32, const FftArgs& a) {
 }
 static size_t xline_lds(bool fp32, const FftArgs& a) {
   const int nf = fp32 ? xl_nf<float>() : xl_nf<double>();
-  return (size_t)2 * lds_entries(nf * (a.L / 2) * (a.rc.mgalx + LOMA_XL_RS1)) * (fp32 ? 8 : 16);
+  const int pad = fp32 ? xl_pad<float>() : xl_pad<double>();
+  return (size_t)2 * lds_entries(nf * (a.L / 2) * (a.rc.mgalx + LOMA_XL_RS1), pad) *
+         (fp32 ? 8 : 16);
 }
 
 // kernel instance for a transform length: compile-time plans for the

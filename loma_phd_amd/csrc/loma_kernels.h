@@ -65,6 +65,23 @@ struct PhaseArgs {
   int nmodes;                // modes handled by this launch
 };
 
+// device initial condition (loma_seed): seeded mixing-layer perturbation
+struct IcArgs {
+  RunConst rc;
+  YTab T;
+  const double* helm_lu;     // per (i,|kz|) class LU tables (lane-major)
+  int lu_stride;
+  const double2* coef;       // [3][nplanes*mz][nymax]: coefficient of e1^(n+1)
+  const double* y;           // [my]
+  double* partial;           // [nplanes*mz][3]: 2|a|^2 at the centre row (pass 1)
+  double2 *vor, *phi, *psi, *scal, *zf;
+  double s, env_width, k0;
+  double scale[3];           // target rms / centre-plane rms of u, v, w (pass 2)
+  int nymax;
+  int pass;                  // 1: centre-plane energies, 2: the state
+  int with_z;
+};
+
 struct FftArgs {
   RunConst rc;
   FftPlan plz, plx;
@@ -106,6 +123,7 @@ int launch_visc(int M, const YTab& T, const double* lu1, const double* lu2, cons
 int launch_phaseA(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
 int launch_phaseC(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
 int launch_phaseC2(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
+int launch_ic(int M, const IcArgs& a, hipStream_t st);
 int launch_finish_step(DevScalars* sc, RunConst rc, int icfl, hipStream_t st);
 
 int launch_zfft_inv(bool fp32, const FftArgs& a, hipStream_t st);

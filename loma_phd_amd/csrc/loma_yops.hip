@@ -22,6 +22,8 @@ int launch_visc_m(const YTab&, const double*, const double*, int, const double*,
                   double, int, hipStream_t);
 template <int M>
 int launch_phase_m(int, bool, const PhaseArgs&, hipStream_t);
+template <int M>
+int launch_ic_m(const IcArgs&, hipStream_t);
 
 RkCoef rk_coefficients() {
   RkCoef r;
@@ -93,6 +95,9 @@ int launch_phaseC(int M, bool fp32, const PhaseArgs& a, hipStream_t st) {
 int launch_phaseC2(int M, bool fp32, const PhaseArgs& a, hipStream_t st) {
   if (a.nmodes <= 0) return 0;
   LOMA_DISPATCH_M(M, launch_phase_m<MM>(2, fp32, a, st));
+}
+int launch_ic(int M, const IcArgs& a, hipStream_t st) {
+  LOMA_DISPATCH_M(M, launch_ic_m<MM>(a, st));
 }
 
 // LU of (dt22 - s dt21) with identity end rows (Lapvdvhom laps.f:349-366),

@@ -825,6 +825,183 @@ __global__ __launch_bounds__(256) void k_phaseC2(PhaseArgs a) {
   store_col<M>(a.psi + o, my, lane, st, ps);
 }
 
+// Seeded mixing-layer initial condition on the device, one wavefront per
+// (kx, kz) mode (the recipe of trunk/initloma3/src/main.f:95-463 with the
+// seeded draws of the host: coef[c][mode][n] = amp_n exp(2 pi i phs_n)).
+//   u_c(y) = shape(kh) ywin(y) sum_n coef_n e1^(n+1),  e1 = exp(i 2 kh/nymax y)
+//   (cu, cv, cw) = curl(u, v, w), zero on the kx = 0 plane and kz = 0 modes;
+//   pass 1: 2 |c_.|^2 at the centre row per mode (the host sums them in
+//   mode order into the centre-plane rms of each component);
+//   pass 2: rho u_i = rho scale_c c_i; psi from lap psi = div(rho u)
+//   (Lapvdvhom, main.f:452-463); Omega_y = ikz rho u - ikx rho w;
+//   phi = (D2 - k^2)(rho v - d psi/dy); T and Z in the 00 mode only.
+template <int M>
+__global__ __launch_bounds__(256) void k_ic(IcArgs a) {
+  LOMA_STAGE(M);
+  const RunConst& rc = a.rc;
+  const int nmodes = rc.nplanes * rc.mz;
+  const int w = wave_uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  if (w >= nmodes) return;
+  int i, k;
+  double kx, kz;
+  mode_of(rc, w, i, k, kx, kz);
+  const int my = rc.my, r0 = lane * M;
+  const size_t o = (size_t)w * my;
+  double yv[M], rho[M];
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    const int r = r0 + t;
+    yv[t] = r < my ? a.y[r] : 0.0;
+    rho[t] = 1.0 + (a.s - 1.0) / (a.s + 1.0) * tanh(-2.0 * yv[t]);
+  }
+  if (i == 0 || k == 0) {
+    // no perturbation on the kx = 0 plane and the kz = 0 modes
+    if (a.pass == 1) {
+      if (lane == 0) a.partial[(size_t)w * 3] = a.partial[(size_t)w * 3 + 1] =
+          a.partial[(size_t)w * 3 + 2] = 0.0;
+      return;
+    }
+    double x[2][M];
+#pragma unroll
+    for (int t = 0; t < M; ++t) x[0][t] = x[1][t] = 0.0;
+    store_col<M>(a.vor + o, my, lane, st, x);
+    store_col<M>(a.phi + o, my, lane, st, x);
+    store_col<M>(a.psi + o, my, lane, st, x);
+    const bool m00 = i == 0 && k == 0;
+#pragma unroll
+    for (int t = 0; t < M; ++t) x[0][t] = m00 ? 1.0 / rho[t] : 0.0;  // T = 1/rho00
+    store_col<M>(a.scal + o, my, lane, st, x);
+    if (a.with_z) {
+#pragma unroll
+      for (int t = 0; t < M; ++t) x[0][t] = m00 ? 0.5 * (1.0 + tanh(2.0 * yv[t])) : 0.0;
+      store_col<M>(a.zf + o, my, lane, st, x);
+    }
+    return;
+  }
+  const double kh = sqrt(kx * kx + kz * kz);
+  const double q = kh / a.k0;
+  const double shape = q * q * exp(-(q * q));
+  const double ky1 = 2.0 * kh / a.nymax;
+  double er[M], ei[M], win[M];
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    sincos(ky1 * yv[t], &ei[t], &er[t]);
+    const double yw = yv[t] / a.env_width;
+    win[t] = shape * exp(-(yw * yw));
+  }
+  // component c of the velocity into f[n0], f[n0+1] (Horner in e1)
+  auto build = [&](int c, double (&f)[4][M], int n0) {
+    const double2* cf = a.coef + ((size_t)c * nmodes + w) * a.nymax;
+    const double2 top = cf[a.nymax - 1];
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+      f[n0][t] = top.x * er[t] - top.y * ei[t];
+      f[n0 + 1][t] = top.x * ei[t] + top.y * er[t];
+    }
+    for (int n = a.nymax - 2; n >= 0; --n) {
+      const double2 cn = cf[n];
+#pragma unroll
+      for (int t = 0; t < M; ++t) {
+        const double pr = f[n0][t] + cn.x, pi = f[n0 + 1][t] + cn.y;
+        f[n0][t] = pr * er[t] - pi * ei[t];
+        f[n0 + 1][t] = pr * ei[t] + pi * er[t];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+      f[n0][t] *= win[t];
+      f[n0 + 1][t] *= win[t];
+    }
+  };
+  // u in rows 0,1 and w in rows 2,3; one derivative pass for both
+  double uw[4][M], duw[4][M];
+  build(0, uw, 0);
+  build(2, uw, 2);
+  op_dy1<M, 4>(a.T, lane, uw, duw);
+  double cu[2][M], cv[2][M], cw[2][M];
+  {
+    double v[4][M];
+    build(1, v, 0);
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+      // cu = dw - ikz v, cv = ikz u - ikx w, cw = ikx v - du   (main.f:240-257)
+      cu[0][t] = duw[2][t] + kz * v[1][t];
+      cu[1][t] = duw[3][t] - kz * v[0][t];
+      cv[0][t] = -kz * uw[1][t] + kx * uw[3][t];
+      cv[1][t] = kz * uw[0][t] - kx * uw[2][t];
+      cw[0][t] = -kx * v[1][t] - duw[0][t];
+      cw[1][t] = kx * v[0][t] - duw[1][t];
+    }
+  }
+  const int jc = my / 2;
+  if (a.pass == 1) {
+    // kx > 0 planes count twice (the kx < 0 half is their conjugate)
+    auto energy = [&](const double (&x)[2][M]) {
+      const double re = col_value<M>(x[0], jc), im = col_value<M>(x[1], jc);
+      return 2.0 * (re * re + im * im);
+    };
+    const double e[3] = {energy(cu), energy(cv), energy(cw)};
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a.partial[(size_t)w * 3 + c] = e[c];
+    }
+    return;
+  }
+  // rho u_i with the normalisation of each component
+#pragma unroll
+  for (int t = 0; t < M; ++t)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      cu[h][t] = rho[t] * (cu[h][t] * a.scale[0]);
+      cv[h][t] = rho[t] * (cv[h][t] * a.scale[1]);
+      cw[h][t] = rho[t] * (cw[h][t] * a.scale[2]);
+    }
+  {
+    double vo[2][M];
+#pragma unroll
+    for (int t = 0; t < M; ++t) {  // Omega_y = ikz rho u - ikx rho w
+      vo[0][t] = -kz * cu[1][t] - (-kx * cw[1][t]);
+      vo[1][t] = kz * cu[0][t] - kx * cw[0][t];
+    }
+    store_col<M>(a.vor + o, my, lane, st, vo);
+  }
+  double x[2][M];
+  {
+    double dv[2][M], dvv[2][M];
+    op_dy1<M, 2>(a.T, lane, cv, dv);
+#pragma unroll
+    for (int t = 0; t < M; ++t) {  // div = ikx rho u + d(rho v)/dy + ikz rho w
+      dvv[0][t] = (-kx * cu[1][t] + dv[0][t]) + (-kz * cw[1][t]);
+      dvv[1][t] = (kx * cu[0][t] + dv[1][t]) + kz * cw[0][t];
+    }
+    const double* lu = a.helm_lu + (size_t)helm_class(rc, i, k) * a.lu_stride;
+    op_helm<M, 2>(a.T, lu, lane, dvv, x);  // psi
+  }
+  store_col<M>(a.psi + o, my, lane, st, x);
+  {
+    double d[2][M];
+    op_dy1<M, 2>(a.T, lane, x, d);
+#pragma unroll
+    for (int t = 0; t < M; ++t) {  // m_y = rho v - d psi/dy
+      x[0][t] = cv[0][t] - d[0][t];
+      x[1][t] = cv[1][t] - d[1][t];
+    }
+    op_dy2<M, 2>(a.T, lane, x, d);
+    const double rK = kx * kx + kz * kz;
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+      d[0][t] = d[0][t] - rK * x[0][t];
+      d[1][t] = d[1][t] - rK * x[1][t];
+    }
+    store_col<M>(a.phi + o, my, lane, st, d);
+  }
+#pragma unroll
+  for (int t = 0; t < M; ++t) x[0][t] = x[1][t] = 0.0;
+  store_col<M>(a.scal + o, my, lane, st, x);
+  if (a.with_z) store_col<M>(a.zf + o, my, lane, st, x);
+}
+
 // per-M launchers (instantiated in loma_yops_m*.hip, one chunk size each)
 template <int M>
 int launch_dy_m(int order, const YTab& T, const double* u, double* du, int ncols,
@@ -884,6 +1061,13 @@ int launch_phase_m(int which, bool fp32, const PhaseArgs& a, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
+template <int M>
+int launch_ic_m(const IcArgs& a, hipStream_t st) {
+  const int nmodes = a.rc.nplanes * a.rc.mz;
+  hipLaunchKernelGGL((k_ic<M>), dim3((nmodes + 3) / 4), dim3(256), 0, st, a);
+  return (int)hipGetLastError();
+}
+
 #define LOMA_INSTANTIATE_M(MV)                                                        \
   template int launch_dy_m<MV>(int, const YTab&, const double*, double*, int, hipStream_t); \
   template int launch_lapvdvhom_m<MV>(const YTab&, const double*, int, const double*,     \
@@ -896,6 +1080,7 @@ int launch_phase_m(int which, bool fp32, const PhaseArgs& a, hipStream_t st) {
                                  const double*, double*, double*, double*, double*,       \
                                  double*, double*, const double*, double, double, double, \
                                  int, hipStream_t);                                       \
-  template int launch_phase_m<MV>(int, bool, const PhaseArgs&, hipStream_t);
+  template int launch_phase_m<MV>(int, bool, const PhaseArgs&, hipStream_t);              \
+  template int launch_ic_m<MV>(const IcArgs&, hipStream_t);
 
 }  // namespace loma

@@ -130,6 +130,36 @@ def mean_profiles(cfg: MixingLayerConfig, y):
     return rho, U, Z
 
 
+NYMAX = 8  # y harmonics of each perturbation mode (main.f:123-291)
+
+
+def peak_wavenumber(cfg: MixingLayerConfig) -> float:
+    """k0 of the perturbation spectrum (kh/k0)^2 exp(-(kh/k0)^2)."""
+    return max(cfg.alp, cfg.bet) * 4.0
+
+
+def ic_coefficients(cfg: MixingLayerConfig) -> np.ndarray:
+    """The seeded draws of the perturbation: for u, v, w and each mode,
+    c_n = amp_n exp(2 pi i phs_n) with amp_n ~ U(-1, 1), phs_n ~ U(0, 1)
+    (main.f:123-160), the coefficient of exp(i (n+1) (2 kh / NYMAX) y).
+    Shape (3, nplanes, mz, NYMAX) complex128; the same draws feed the host
+    initial condition and the device one (loma_seed)."""
+    np_, mz = cfg.nplanes, cfg.mz
+    rng = np.random.Generator(np.random.PCG64(cfg.seed))
+    out = np.empty((3, np_, mz, NYMAX), np.complex128)
+    for c in range(3):
+        amp = 2.0 * (0.5 - rng.random((np_, mz, NYMAX)))
+        phs = rng.random((np_, mz, NYMAX))
+        out[c] = amp * np.exp(2j * np.pi * phs)
+    return out
+
+
+def seed_run(run, cfg: MixingLayerConfig):
+    """The initial condition of `cfg` built on the device of `run`
+    (capi.Run.seed): only the seeded draws are made here."""
+    run.seed(ic_coefficients(cfg), cfg.s, cfg.amp, cfg.env_width, peak_wavenumber(cfg))
+
+
 def initial_condition(cfg: MixingLayerConfig, ops: HostOps | None = None):
     """Spectral state dict (vor, phi, psi, scal, u00, v00, w00, z) in the
     reference layout [kx plane][kz][y], complex128."""
@@ -137,24 +167,26 @@ def initial_condition(cfg: MixingLayerConfig, ops: HostOps | None = None):
     ops = ops or HostOps(y)
     np_, mz, my = cfg.nplanes, cfg.mz, cfg.my
     nz = (mz - 1) // 2
-    rng = np.random.Generator(np.random.PCG64(cfg.seed))
+    coefs = ic_coefficients(cfg)
     kx = cfg.alp * np.arange(np_)
     kk = np.arange(mz)
     kz = np.where(kk <= nz, cfg.bet * kk, -cfg.bet * (mz - kk))
     kh = np.sqrt(kx[:, None] ** 2 + kz[None, :] ** 2)
-    nymax = 8
-    k0 = max(cfg.alp, cfg.bet) * 4.0
+    nymax = NYMAX
+    k0 = peak_wavenumber(cfg)
     ywin = np.exp(-(y / cfg.env_width) ** 2)
     vel = []
     shape = (kh / k0) ** 2 * np.exp(-(kh / k0) ** 2)
-    for _ in range(3):
-        amp = 2.0 * (0.5 - rng.random((np_, mz, nymax)))
-        phs = rng.random((np_, mz, nymax))
-        f = np.zeros((np_, mz, my), np.complex128)
-        for n in range(nymax):
-            ky = 2.0 * kh / nymax * (n + 1)
-            f += amp[:, :, n, None] * np.exp(1j * (ky[:, :, None] * y[None, None, :] +
-                                                   2 * np.pi * phs[:, :, n, None]))
+    for comp in range(3):
+        # sum_n amp_n exp(i (ky_n y + 2 pi phs_n)) with ky_n = (n+1) ky_1:
+        # exp(i ky_n y) = e1^(n+1), so the sum is a polynomial in e1, summed
+        # by Horner's rule (one complex exponential per point, not nymax)
+        c = coefs[comp]
+        e1 = np.exp(1j * (2.0 * kh / nymax)[:, :, None] * y[None, None, :])
+        f = c[:, :, nymax - 1, None] * e1
+        for n in range(nymax - 2, -1, -1):
+            f += c[:, :, n, None]
+            f *= e1
         vel.append(f * (shape[:, :, None] * ywin[None, None, :]))
     u, v, w = vel
     ikx = 1j * kx[:, None, None]

@@ -3,11 +3,14 @@
 Run requests arrive over time.  Each names a grid, a density ratio, a
 Reynolds number, a seed and a step budget.  The loop
 
-  * prepares each request's initial condition on the host as it arrives
-    (a worker thread, so the loop keeps polling meanwhile);
-  * admits requests in arrival order when a slot is free and the run's device
-    footprint (loma_footprint: the exact bytes loma_init will allocate) fits
-    in the memory budget beside the runs already resident;
+  * prepares each request's initial condition as it arrives, on a worker
+    thread so the loop keeps polling meanwhile (HipBackend draws the seeded
+    coefficients there and builds the state on the device, loma_seed);
+  * admits prepared requests in arrival order when a slot is free and the
+    run's device footprint (loma_footprint: the exact bytes loma_init will
+    allocate) fits in the memory budget beside the runs already resident; a
+    request still being prepared does not hold back those behind it, a
+    prepared one that does not fit does;
   * keeps every admitted run's spectral state on the GPU and keeps one RK3
     step in flight per run, each run on its own non-blocking stream
     (loma_step_begin / loma_step_poll), so the steps of all resident runs
@@ -121,12 +124,26 @@ class HipBackend:
     finished runs are kept per grid (RunPool) and re-targeted at the next
     request of that grid (loma_set_physics + upload)."""
 
-    def __init__(self, keep_state: bool = False, use_graph: bool = True, reuse: bool = True):
+    def __init__(self, keep_state: bool = False, use_graph: bool = True, reuse: bool = True,
+                 device_ic: bool = True):
         from . import capi  # loads libloma_hip.so; fails loudly if it is missing
         self.capi = capi
         self.keep_state = keep_state
         self.use_graph = use_graph
-        self.pool = RunPool(lambda run: run.close()) if reuse else None
+        # device_ic: only the seeded draws are made on the host and the state
+        # is built on the device (loma_seed).  The host initial condition is
+        # numpy work that holds the GIL for seconds per large grid and would
+        # stall the loop thread.
+        self.device_ic = device_ic
+        self.pool = RunPool(lambda run: self._span("close", run.close)) if reuse else None
+        self.spans: list = []  # (name, t_start, t_end) of every set-up / retirement call
+
+    def _span(self, name, fn, *args):
+        t = time.perf_counter()
+        try:
+            return fn(*args)
+        finally:
+            self.spans.append((name, t, time.perf_counter()))
 
     def key(self, req: Request):
         c = req.cfg
@@ -154,25 +171,36 @@ class HipBackend:
         return self.capi.footprint(c.mgalx, c.my, c.mgalz, req.precision, self.flags(req))
 
     def prepare(self, req: Request):
-        from .ic import initial_condition
-        return initial_condition(req.cfg)
+        """What set-up needs of `req`'s initial condition: the seeded draws
+        (device_ic) or the whole host-built state."""
+        from . import ic
+        if self.device_ic:
+            return self._span("prepare", lambda: {"coef": ic.ic_coefficients(req.cfg),
+                                                  "y": req.cfg.y()})
+        return self._span("prepare", ic.initial_condition, req.cfg)
 
     def start(self, req: Request, ic):
         c = req.cfg
         run = self.pool.take(req.rid) if self.pool else None
         if run is not None:
-            run.set_physics(c.Re, c.pr, c.cfl, c.dt0)
+            self._span("set_physics", run.set_physics, c.Re, c.pr, c.cfl, c.dt0)
         else:
-            run = self.capi.Run(c.mgalx, c.my, c.mgalz, ic["y"], c.Re, c.alp, c.bet, pr=c.pr,
-                                cfl=c.cfl, ncfl=c.ncfl, dt0=c.dt0, precision=req.precision,
-                                flags=self.flags(req))
+            run = self._span("init", lambda: self.capi.Run(
+                c.mgalx, c.my, c.mgalz, ic["y"], c.Re, c.alp, c.bet, pr=c.pr, cfl=c.cfl,
+                ncfl=c.ncfl, dt0=c.dt0, precision=req.precision, flags=self.flags(req)))
         run.req_key = self.key(req)
         try:
-            run.upload(ic["vor"], ic["phi"], ic["psi"], ic["scal"], ic["u00"], ic["v00"],
-                       ic["w00"], z=ic["z"] if c.scalar_z else None)
-            run.use_graph(self.use_graph)
+            if "coef" in ic:
+                from .ic import peak_wavenumber
+                self._span("seed", run.seed, ic["coef"], c.s, c.amp, c.env_width,
+                           peak_wavenumber(c))
+            else:
+                self._span("upload", lambda: run.upload(
+                    ic["vor"], ic["phi"], ic["psi"], ic["scal"], ic["u00"], ic["v00"],
+                    ic["w00"], z=ic["z"] if c.scalar_z else None))
+            self._span("use_graph", run.use_graph, self.use_graph)
         except Exception:
-            run.close()
+            self._span("close", run.close)
             raise
         return run
 
@@ -186,12 +214,12 @@ class HipBackend:
         """Final state (with keep_state); the run is closed here, or kept by
         `retired` once the loop has returned its bytes to the budget."""
         try:
-            out = run.download() if self.keep_state else None
+            out = self._span("download", run.download) if self.keep_state else None
         except Exception:
-            run.close()
+            self._span("close", run.close)
             raise
         if self.pool is None:
-            run.close()
+            self._span("close", run.close)
         return out
 
     def retired(self, run):
@@ -199,7 +227,7 @@ class HipBackend:
             self.pool.put(run.req_key, run, run.bytes)
 
     def free(self, run):
-        run.close()
+        self._span("close", run.close)
 
 
 class _Inline:
@@ -247,9 +275,25 @@ class Server:
     def _timed(self, name, fn, *args):
         t = time.perf_counter()
         out = fn(*args)
-        dt = time.perf_counter() - t
+        t1 = time.perf_counter()
+        dt = t1 - t
         if dt > self._slow.get(name, 0.0):
             self._slow[name] = dt
+        if dt > 0.02:
+            self._stalls.append((name, t, t1))
+        return out
+
+    def _stall_overlaps(self, stalls) -> dict:
+        """For loop-thread intervals over 20 ms: how many of them overlapped
+        each kind of backend call (initial condition, set-up, retirement)."""
+        spans = getattr(self.b, "spans", None)
+        if not spans:
+            return {}
+        out: dict = {}
+        for _, a0, a1 in stalls:
+            kinds = {n for n, b0, b1 in spans if b0 < a1 and b1 > a0}
+            for k in kinds or {"none"}:
+                out[k] = out.get(k, 0) + 1
         return out
 
     def _idle(self) -> int:
@@ -287,12 +331,15 @@ class Server:
         closed = False
         gaps = []          # loop passes that took longer than 20 ms (s)
         self._slow = {}    # slowest main-thread call per kind (s)
+        self._stalls = []  # main-thread calls over 20 ms: (kind, start, end)
+        self._gap_spans = []  # loop passes over 20 ms: ("pass", start, end)
         t_prev = t0
         try:
             while pending or waiting or starting or active or retiring:
                 t_now = self.clock()
                 if t_now - t_prev > 0.02:
                     gaps.append(t_now - t_prev)
+                    self._gap_spans.append(("pass", t_prev, t_now))
                 t_prev = t_now
                 now = t_now - t0
                 busy = False
@@ -311,12 +358,18 @@ class Server:
                     outs[r.rid].footprint = self.b.footprint(r)
                     waiting.append((r, pool.submit(self.b.prepare, r)))
                     busy = True
-                # admission in arrival order
-                while waiting and waiting[0][1].done():
-                    r, f = waiting[0]
+                # admission in arrival order among the prepared requests: one
+                # still being prepared is passed over, a prepared one that does
+                # not fit stops admission (so large runs are not starved)
+                qi = 0
+                while qi < len(waiting):
+                    r, f = waiting[qi]
+                    if not f.done():
+                        qi += 1
+                        continue
                     o = outs[r.rid]
                     if o.footprint > self.budget or f.exception() is not None:
-                        waiting.popleft()
+                        del waiting[qi]
                         o.status = "rejected" if o.footprint > self.budget else "failed"
                         o.error = "" if f.exception() is None else repr(f.exception())
                         o.end_s = now
@@ -326,7 +379,7 @@ class Server:
                     if nres >= self.max_slots or not self._timed("admit", self._room, r,
                                                                   o.footprint, used):
                         break
-                    waiting.popleft()
+                    del waiting[qi]
                     used += o.footprint
                     peak_used = max(peak_used, used + self._idle())
                     o.admit_s = now
@@ -425,6 +478,9 @@ class Server:
             slow_loop_passes=len(gaps),
             slowest_loop_pass_ms=float(max(gaps) * 1e3) if gaps else 0.0,
             slowest_call_ms={k: float(v * 1e3) for k, v in self._slow.items()},
+            slow_calls=len(self._stalls),
+            slow_call_overlaps=self._stall_overlaps(self._stalls),
+            slow_pass_overlaps=self._stall_overlaps(self._gap_spans),
             peak_bytes=int(peak_used),
             peak_active=int(peak_active),
             counts={s: sum(1 for o in outs.values() if o.status == s)

@@ -1,0 +1,66 @@
+"""The initial condition built on the device (loma_seed) against the host
+one (ic.initial_condition) made from the same seeded draws: the same
+operators with different rounding, so every field agrees within 1e-12
+relative L2.  Two steps from either start agree within 1e-11 in FP64; in the
+mixed form the 1e-13 start difference flips FP32 roundings in the
+transforms, so the bar there is 1e-6 (the mixed form itself is held to 1e-5
+against the FP64 oracle).  A grid whose transform work buffer cannot hold
+the draws is refused."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("vor", "phi", "psi", "scal", "u00", "v00", "w00")
+
+
+def _run(cfg, y, precision, zflag):
+    from loma_phd_amd import capi
+    flags = capi.LOMA_FLAG_SCALAR_Z if zflag else 0
+    return capi.Run(cfg.mgalx, cfg.my, cfg.mgalz, y, cfg.Re, cfg.alp, cfg.bet, cfg.pr,
+                    cfg.cfl, cfg.ncfl, cfg.dt0, precision, flags)
+
+
+@pytest.mark.parametrize("grid,precision,zflag,s", [
+    ((64, 129, 64), 1, True, 4.0), ((96, 97, 48), 0, False, 2.0), ((48, 97, 36), 0, True, 7.0)])
+def test_seed_equals_host_initial_condition(gpu, grid, precision, zflag, s):
+    from loma_phd_amd import ic
+    from loma_phd_amd.grid import MixingLayerConfig
+    print("seed", grid, precision, zflag, s, flush=True)
+    cfg = MixingLayerConfig(mgalx=grid[0], my=grid[1], mgalz=grid[2], s=s, Re=300.0, seed=17,
+                            scalar_z=zflag)
+    st = ic.initial_condition(cfg)
+    a = _run(cfg, st["y"], precision, zflag)
+    ic.seed_run(a, cfg)
+    b = _run(cfg, st["y"], precision, zflag)
+    b.upload(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"],
+             z=st["z"] if zflag else None)
+    da = a.download()
+    names = FIELDS + (("z",) if zflag else ())
+    for f in names:
+        e = rel_l2(da[f], st[f])
+        assert e < 1e-12, (f, e)
+    a.advance(2)
+    b.advance(2)
+    da, db = a.download(), b.download()
+    bar = 1e-11 if precision == 0 else 1e-6
+    for f in names:
+        e = rel_l2(da[f], db[f])
+        assert e < bar, ("after 2 steps", f, e)
+    sa, sb = a.scalars(), b.scalars()
+    assert abs(sa["dt"] - sb["dt"]) <= 1e-12 * sb["dt"]
+    a.close()
+    b.close()
+
+
+def test_seed_refuses_grid_without_room(gpu):
+    from loma_phd_amd import capi, ic
+    from loma_phd_amd.grid import MixingLayerConfig
+    print("seed refused", flush=True)
+    cfg = MixingLayerConfig(mgalx=32, my=16, mgalz=32, s=2.0, Re=160.0, seed=3, scalar_z=False)
+    r = _run(cfg, cfg.y(), 1, False)
+    with pytest.raises(capi.LomaError):
+        ic.seed_run(r, cfg)
+    r.close()

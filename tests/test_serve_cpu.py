@@ -188,3 +188,27 @@ def test_run_pool_reuse_and_eviction():
     pool.put("g5", "run-d", 10)
     pool.close_all()
     assert closed[-1] == "run-d" and pool.idle_bytes() == 0
+
+
+def test_slow_preparation_does_not_block_later_requests():
+    """A request whose initial condition is still being prepared is passed
+    over; the prepared requests behind it are admitted in arrival order."""
+    import threading
+    gate = threading.Event()
+
+    class SlowFirst(FakeBackend):
+        def prepare(self, req):
+            if req.rid == 0:
+                gate.wait(timeout=30)
+            return ("ic", req.rid)
+
+        def finish(self, run):
+            if run.req.rid == 2:
+                gate.set()  # the first request's preparation ends late
+            return super().finish(run)
+
+    be = SlowFirst()
+    reqs = [req(0, 0.0, steps=2), req(1, 0.0, steps=2), req(2, 0.0, steps=2)]
+    rep = Server(be, mem_budget=small_budget(8), workers=4, idle_sleep=0).run(reqs)
+    assert rep["counts"]["done"] == 3
+    assert be.started[:2] == [1, 2] and be.started[2] == 0
