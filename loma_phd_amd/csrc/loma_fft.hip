@@ -573,11 +573,11 @@ constexpr int xl_pad() {
 
 // lines of NF fields (W layout [i][z][j]; src1 unused for NF = 1) ->
 // Hermitian-packed complex rows in A (row f*pairs + p = lines j0+2p,
-// j0+2p+1 of field f), then the inverse FFT.  Returns the buffer holding the
-// physical pairs.
+// j0+2p+1 of field f); no barrier.
 template <typename T, typename T2, int XR, int NN, int NF>
-__device__ __forceinline__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src0,
-                          const T2* __restrict__ src1, T2* A, T2* B, int z, int j0, int pairs) {
+__device__ __forceinline__ void xl_pack(const FftArgs& a, const T2* __restrict__ src0,
+                                        const T2* __restrict__ src1, T2* A, int z, int j0,
+                                        int pairs) {
   XlIn<T2, XR> v[NF];
   xl_fetch<T2, XR, NN>(a, src0, z, j0, pairs, v[0]);
   if constexpr (NF > 1) xl_fetch<T2, XR, NN>(a, src1, z, j0, pairs, v[1]);
@@ -610,23 +610,18 @@ __device__ __forceinline__ T2* xl_inverse(const FftArgs& a, const T2* __restrict
       }
     }
   }
-  __syncthreads();
-  return fft_any<T, T2, NN, XL_THREADS, XP>(A, B, N, NF * pairs, a.plx, (const T2*)a.twx, +1);
 }
 
-// forward FFT of A (rows of NF outputs, filled by the caller, then
-// synchronised here) and Hermitian unpacking of the pairs into fields dst0
-// (rows 0..pairs-1) and dst1 (rows pairs..2*pairs-1), scaled by 1/N
+// Hermitian unpacking of the transformed pairs in R into fields dst0 (rows
+// 0..pairs-1) and dst1 (rows pairs..2*pairs-1), scaled by 1/N; no barrier.
 template <typename T, typename T2, int NN, int NF>
-__device__ __forceinline__ void xl_forward(const FftArgs& a, T2* __restrict__ dst0, T2* __restrict__ dst1,
-                           T2* A, T2* B, int z, int j0, int pairs) {
+__device__ __forceinline__ void xl_unpack(const FftArgs& a, const T2* R, T2* __restrict__ dst0,
+                                          T2* __restrict__ dst1, int z, int j0, int pairs) {
   constexpr int XP = xl_pad<T>();
   const int N = NN ? NN : a.rc.mgalx, np = NN ? NN / 3 : a.rc.nplanes, my = a.rc.my,
             mgz = a.rc.mgalz;
   const int lp = __builtin_ctz(pairs);
   const int RS = xl_row_stride<NN>(N);
-  __syncthreads();
-  const T2* R = fft_any<T, T2, NN, XL_THREADS, XP>(A, B, N, NF * pairs, a.plx, (const T2*)a.twx, -1);
   const T hs = (T)(0.5 / (double)N);
   const int per = np * pairs;
   for (int it = threadIdx.x; it < NF * per; it += XL_THREADS) {
@@ -645,6 +640,85 @@ __device__ __forceinline__ void xl_forward(const FftArgs& a, T2* __restrict__ ds
     if (ja < my) dst[base + ja] = ya;
     if (jb < my) dst[base + jb] = yb;
   }
+}
+
+// pack, then the inverse FFT.  Returns the buffer holding the physical pairs.
+template <typename T, typename T2, int XR, int NN, int NF>
+__device__ __forceinline__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src0,
+                          const T2* __restrict__ src1, T2* A, T2* B, int z, int j0, int pairs) {
+  xl_pack<T, T2, XR, NN, NF>(a, src0, src1, A, z, j0, pairs);
+  __syncthreads();
+  const int N = NN ? NN : a.rc.mgalx;
+  return fft_any<T, T2, NN, XL_THREADS, xl_pad<T>()>(A, B, N, NF * pairs, a.plx,
+                                                      (const T2*)a.twx, +1);
+}
+
+// forward FFT of A (rows of NF outputs, filled by the caller, then
+// synchronised here) and the unpacking
+template <typename T, typename T2, int NN, int NF>
+__device__ __forceinline__ void xl_forward(const FftArgs& a, T2* __restrict__ dst0, T2* __restrict__ dst1,
+                           T2* A, T2* B, int z, int j0, int pairs) {
+  const int N = NN ? NN : a.rc.mgalx;
+  __syncthreads();
+  const T2* R = fft_any<T, T2, NN, XL_THREADS, xl_pad<T>()>(A, B, N, NF * pairs, a.plx,
+                                                             (const T2*)a.twx, -1);
+  xl_unpack<T, T2, NN, NF>(a, R, dst0, dst1, z, j0, pairs);
+  __syncthreads();
+}
+
+// Register-resident ends of the 512-point transform (radix 8, 8, 8) for 4
+// pairs (4 rows, 256 items per pass: one per thread, row tid/64, item
+// j = tid%64).  The last inverse pass reads and writes entries j + 64r of its
+// row, which are exactly the entries x0 + 64m the physics assigns to the
+// thread; so the inverse keeps its last pass in registers (no store, no
+// barrier, no reload) and the forward takes its first pass from registers.
+template <typename T, typename T2, int XR>
+__device__ __forceinline__ void xl_inverse_reg(const FftArgs& a, const T2* __restrict__ src,
+                                               T2* A, T2* B, int z, int j0, int pairs,
+                                               T (&v)[8][2]) {
+  static_assert(xl_pad<T>() == 0, "register ends assume unpadded rows");
+  constexpr int RS = 512 + LOMA_XL_RS1;
+  const T2* tw = (const T2*)a.twx;
+  const int row = threadIdx.x >> 6, j = threadIdx.x & 63;
+  xl_pack<T, T2, XR, 512, 1>(a, src, nullptr, A, z, j0, pairs);
+  __syncthreads();
+  stockham_item<8, T, T2, 0>(A, B, row * RS, j, 64, 1, 64, tw, +1);
+  __syncthreads();
+  stockham_item<8, T, T2, 0>(B, A, row * RS, j, 64, 8, 8, tw, +1);
+  __syncthreads();
+  T2 w[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) w[r] = A[row * RS + j + r * 64];
+  twiddle<8>(w, tw, j, 1, +1);
+  dft8<T, T2>(w, +1);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    v[r][0] = w[r].x;
+    v[r][1] = w[r].y;
+  }
+  __syncthreads();  // A is packed again by the next field
+}
+
+template <typename T, typename T2>
+__device__ __forceinline__ void xl_forward_reg(const FftArgs& a, T2* __restrict__ dst, T2* A,
+                                               T2* B, int z, int j0, int pairs,
+                                               const T (&v)[8][2]) {
+  static_assert(xl_pad<T>() == 0, "register ends assume unpadded rows");
+  constexpr int RS = 512 + LOMA_XL_RS1;
+  const T2* tw = (const T2*)a.twx;
+  const int row = threadIdx.x >> 6, j = threadIdx.x & 63;
+  T2 w[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) w[r] = T2{v[r][0], v[r][1]};
+  dft8<T, T2>(w, -1);  // first pass (no twiddles): entries j + 64r -> 8j + r
+#pragma unroll
+  for (int r = 0; r < 8; ++r) A[row * RS + 8 * j + r] = w[r];
+  __syncthreads();
+  stockham_item<8, T, T2, 0>(A, B, row * RS, j, 64, 8, 8, tw, -1);
+  __syncthreads();
+  stockham_item<8, T, T2, 0>(B, A, row * RS, j, 64, 64, 1, tw, -1);
+  __syncthreads();
+  xl_unpack<T, T2, 512, 1>(a, A, dst, nullptr, z, j0, pairs);
   __syncthreads();
 }
 
@@ -685,7 +759,8 @@ __device__ __forceinline__ XlRk xl_rk(double dt, int rkstep) {
 // Compiled for two waves per SIMD (LOMA_XL_WPE; FP64 always): at three,
 // FP64 spills heavily (and its Z-transport build gave wrong results), and
 // FP32 spills enough to measure slower (3.33 vs 3.22 ms at 512x513x256).
-template <typename T, bool WZ, int EPT, int NN>
+// XREG: the register-resident transform ends (FP32, 512 points, 4 pairs)
+template <typename T, bool WZ, int EPT, int NN, bool XREG>
 __global__ __launch_bounds__(XL_THREADS)
 __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 : LOMA_XL_WPE)))) void k_xline(FftArgs a) {
   using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
@@ -702,6 +777,8 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
   const int RS = xl_row_stride<NN>(N);
   constexpr int XL_NF = xl_nf<T>();
   constexpr int XP = xl_pad<T>();
+  static_assert(!XREG || (NN == 512 && EPT == 8 && XL_NF == 1 && XP == 0),
+                "register-resident ends: FP32 rows of 512 points, one field per transform");
   T2* B = A + lds_entries(XL_NF * pairs * RS, XP);
   constexpr int NIN = WZ ? 12 : 8;
   const double ipe = 1.0 / rc.peclet;
@@ -711,7 +788,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
   const int TP = XL_THREADS / pairs;
   const int p = threadIdx.x / TP, x0 = threadIdx.x - p * TP;
   const int jl[2] = {j0 + 2 * p, j0 + 2 * p + 1};
-  auto prof = [&](int h, int q) -> double {  // y-profile q of line h (0 past my)
+  auto prof = [&](int h, int q) __attribute__((always_inline)) -> double {  // y-profile q of line h (0 past my)
     return (jl[h] < my && q < NIN) ? a.mean_in[(size_t)q * my + jl[h]] : 0.0;
   };
   // output estimates per line (one thread per line)
@@ -750,9 +827,9 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
   __syncthreads();
   // XR items of the np * pairs input coefficients per thread
   constexpr int XR = (EPT + 2) / 3;
-  auto ent = [&](int m) { return x0 + TP * m; };  // x of entry m (valid if < N)
+  auto ent = [&](int m) __attribute__((always_inline)) { return x0 + TP * m; };  // x of entry m (valid if < N)
   // this thread's entries of field f's rows (no barrier)
-  auto grab_at = [&](const T2* R, int f, T (&v)[EPT][2]) {
+  auto grab_at = [&](const T2* R, int f, T (&v)[EPT][2]) __attribute__((always_inline)) {
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
       const int x = ent(m);
@@ -762,7 +839,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
       v[m][1] = q.y;
     }
   };
-  auto put_at = [&](int f, const T (&v)[EPT][2]) {
+  auto put_at = [&](int f, const T (&v)[EPT][2]) __attribute__((always_inline)) {
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
       const int x = ent(m);
@@ -770,12 +847,15 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
     }
   };
   // physical lines of fields s0 and s1 into v0 and v1
-  auto inv2 = [&](const T2* s0, const T2* s1, T (&v0)[EPT][2], T (&v1)[EPT][2]) {
+  auto inv2 = [&](const T2* s0, const T2* s1, T (&v0)[EPT][2], T (&v1)[EPT][2]) __attribute__((always_inline)) {
     if constexpr (XL_NF == 2) {
       const T2* R = xl_inverse<T, T2, XR, NN, 2>(a, s0, s1, A, B, z, j0, pairs);
       grab_at(R, 0, v0);
       grab_at(R, 1, v1);
       __syncthreads();
+    } else if constexpr (XREG) {
+      xl_inverse_reg<T, T2, XR>(a, s0, A, B, z, j0, pairs, v0);
+      xl_inverse_reg<T, T2, XR>(a, s1, A, B, z, j0, pairs, v1);
     } else {
       const T2* R = xl_inverse<T, T2, XR, NN, 1>(a, s0, nullptr, A, B, z, j0, pairs);
       grab_at(R, 0, v0);
@@ -785,10 +865,23 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
       __syncthreads();
     }
   };
-  auto inv1 = [&](const T2* s0, T (&v0)[EPT][2]) {
-    const T2* R = xl_inverse<T, T2, XR, NN, 1>(a, s0, nullptr, A, B, z, j0, pairs);
-    grab_at(R, 0, v0);
-    __syncthreads();
+  auto inv1 = [&](const T2* s0, T (&v0)[EPT][2]) __attribute__((always_inline)) {
+    if constexpr (XREG) {
+      xl_inverse_reg<T, T2, XR>(a, s0, A, B, z, j0, pairs, v0);
+    } else {
+      const T2* R = xl_inverse<T, T2, XR, NN, 1>(a, s0, nullptr, A, B, z, j0, pairs);
+      grab_at(R, 0, v0);
+      __syncthreads();
+    }
+  };
+  // forward transform of one output held by this thread at its entries
+  auto fwd1 = [&](T2* dst, const T (&v)[EPT][2]) __attribute__((always_inline)) {
+    if constexpr (XREG) {
+      xl_forward_reg<T, T2>(a, dst, A, B, z, j0, pairs, v);
+    } else {
+      put_at(0, v);
+      xl_forward<T, T2, NN, 1>(a, dst, nullptr, A, B, z, j0, pairs);
+    }
   };
   T2* const* wb = (T2* const*)a.wbuf;
   T2* const* wo = (T2* const*)a.wout;
@@ -838,10 +931,11 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
   // minus the product of the profiles, expanded exactly so that every term
   // carries a perturbation factor:  (P+p)(Q+q)(R+r) - PQR
   //   = p (Q+q)(R+r) + P (q (R+r) + Q r)
-  auto fill = [&](int q, int f) {  // output q into the rows of field f
+  auto values = [&](int q, T (&out)[EPT][2]) __attribute__((always_inline)) {  // output q at this thread's entries
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
       const int x = ent(m);
+      out[m][0] = out[m][1] = (T)0;
       if (x >= N) continue;
       T val[2];
 #pragma unroll
@@ -863,8 +957,14 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
         }
         val[h] = v;
       }
-      A[lds_i<XP>((f * pairs + p) * RS + x)] = T2{val[0], val[1]};
+      out[m][0] = val[0];
+      out[m][1] = val[1];
     }
+  };
+  auto fill = [&](int q, int f) __attribute__((always_inline)) {  // output q into the rows of field f
+    T out[EPT][2];
+    values(q, out);
+    put_at(f, out);
   };
 #pragma unroll 1
   for (int q = 0; q < 8; q += 2) {
@@ -877,21 +977,25 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
       fill(q + 1, 1);
       xl_forward<T, T2, NN, 2>(a, wo[q], wo[q + 1], A, B, z, j0, pairs);
     } else {
-      fill(q, 0);
-      xl_forward<T, T2, NN, 1>(a, wo[q], nullptr, A, B, z, j0, pairs);
-      fill(q + 1, 0);
-      xl_forward<T, T2, NN, 1>(a, wo[q + 1], nullptr, A, B, z, j0, pairs);
+      T out[EPT][2];
+      values(q, out);
+      fwd1(wo[q], out);
+      values(q + 1, out);
+      fwd1(wo[q + 1], out);
     }
   }
-  fill(8, 0);
-  xl_forward<T, T2, NN, 1>(a, wo[8], nullptr, A, B, z, j0, pairs);
+  {
+    T out[EPT][2];
+    values(8, out);
+    fwd1(wo[8], out);
+  }
 
   // ---- pass 2: r = T (lap T / Pe - rho u . grad T)  (cross.f:1652-1673) ----
   // acc holds the perturbation part; the profile part is per line
   T acc[EPT][2], accz[EPT][2];
   // gradient terms: T_x (4), T_y (5), T_z (6); Z_x (8), Z_y (9), Z_z (10):
   // (R + r)(G + g) - R G = (R + r) g + r G
-  auto grad_term = [&](int c, int gq, const T (&fl)[EPT][2], T (&ac)[EPT][2]) {
+  auto grad_term = [&](int c, int gq, const T (&fl)[EPT][2], T (&ac)[EPT][2]) __attribute__((always_inline)) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const T R_ = (T)(c == 0 ? pu[h] : (c == 1 ? pv[h] : pw[h])), G_ = (T)prof(h, gq);
@@ -1002,15 +1106,10 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
     put_at(1, ru);
     xl_forward<T, T2, NN, 2>(a, wo[9], wo[10], A, B, z, j0, pairs);
   } else {
-    put_at(0, rv);
-    xl_forward<T, T2, NN, 1>(a, wo[9], nullptr, A, B, z, j0, pairs);
-    put_at(0, ru);
-    xl_forward<T, T2, NN, 1>(a, wo[10], nullptr, A, B, z, j0, pairs);
+    fwd1(wo[9], rv);
+    fwd1(wo[10], ru);
   }
-  if constexpr (WZ) {
-    put_at(0, rw);
-    xl_forward<T, T2, NN, 1>(a, wo[11], nullptr, A, B, z, j0, pairs);
-  }
+  if constexpr (WZ) fwd1(wo[11], rw);
 }
 
 // fourz on standalone rows (FFTPACK convention), for the per-operator API
@@ -1086,6 +1185,23 @@ int launch_zfft_fwd(bool fp32, const FftArgs& a, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
+// FP32 x lines at 512 points with 4 pairs take the register-resident
+// transform ends (LOMA_XL_REG32 = 0 turns them off)
+#ifndef LOMA_XL_REG32
+#define LOMA_XL_REG32 1
+#endif
+template <typename T, bool WZ, int E, int NN>
+static void launch_xline_nn(const FftArgs& a, dim3 g, dim3 b, size_t lds, hipStream_t st) {
+  if constexpr (sizeof(T) == 4 && NN == 512 && E == 8 && LOMA_XL_REG32 && xl_pad<T>() == 0 &&
+                xl_nf<T>() == 1) {
+    if (a.L == 8) {
+      hipLaunchKernelGGL((k_xline<T, WZ, E, NN, true>), g, b, lds, st, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_xline<T, WZ, E, NN, false>), g, b, lds, st, a);
+}
+
 int launch_xline(bool fp32, const FftArgs& a, hipStream_t st) {
   const int nyb = (a.rc.my + a.L - 1) / a.L;
   dim3 g(xcd_groups(a.rc.mgalz * nyb)), b(XL_THREADS);
@@ -1095,15 +1211,15 @@ int launch_xline(bool fp32, const FftArgs& a, hipStream_t st) {
   constexpr int E64 = 1024 / XL_THREADS, E32 = 2048 / XL_THREADS;
   if (fp32) {
     if (a.with_z) {
-      LOMA_BY_LENGTH(a.rc.mgalx, hipLaunchKernelGGL((k_xline<float, true, E32, NN>), g, b, lds, st, a));
+      LOMA_BY_LENGTH(a.rc.mgalx, (launch_xline_nn<float, true, E32, NN>(a, g, b, lds, st)));
     } else {
-      LOMA_BY_LENGTH(a.rc.mgalx, hipLaunchKernelGGL((k_xline<float, false, E32, NN>), g, b, lds, st, a));
+      LOMA_BY_LENGTH(a.rc.mgalx, (launch_xline_nn<float, false, E32, NN>(a, g, b, lds, st)));
     }
   } else {
     if (a.with_z) {
-      LOMA_BY_LENGTH(a.rc.mgalx, hipLaunchKernelGGL((k_xline<double, true, E64, NN>), g, b, lds, st, a));
+      LOMA_BY_LENGTH(a.rc.mgalx, (launch_xline_nn<double, true, E64, NN>(a, g, b, lds, st)));
     } else {
-      LOMA_BY_LENGTH(a.rc.mgalx, hipLaunchKernelGGL((k_xline<double, false, E64, NN>), g, b, lds, st, a));
+      LOMA_BY_LENGTH(a.rc.mgalx, (launch_xline_nn<double, false, E64, NN>(a, g, b, lds, st)));
     }
   }
   return (int)hipGetLastError();
