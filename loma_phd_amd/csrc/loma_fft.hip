@@ -328,10 +328,10 @@ __device__ __forceinline__ T2* fft_rows_ct(T2* a, T2* b, int nrows, const T2* __
   }
 }
 
-template <typename T, typename T2, int E, int N, int Ns, int NT, int PAD>
+template <typename T, typename T2, int E, int N, int Ns, int NT, int PAD, int NSTOP = N>
 __device__ __forceinline__ void fft_rows_ip_ct(T2* a, int nrows, const T2* __restrict__ tw,
                                                int s, T2* v) {
-  if constexpr (Ns < N) {
+  if constexpr (Ns < NSTOP) {
     constexpr int R = ct_radix(N / Ns);
     constexpr int nb = N / R;
     constexpr int tstep = N / (Ns * R);
@@ -369,8 +369,23 @@ __device__ __forceinline__ void fft_rows_ip_ct(T2* a, int nrows, const T2* __res
       }
     }
     __syncthreads();
-    fft_rows_ip_ct<T, T2, E, N, Ns * R, NT, PAD>(a, nrows, tw, s, v);
+    fft_rows_ip_ct<T, T2, E, N, Ns * R, NT, PAD, NSTOP>(a, nrows, tw, s, v);
   }
+}
+
+// Ns of the last pass of the compile-time plan of length N
+constexpr int ct_last_ns(int N) {
+  int Ns = 1;
+  while (Ns * ct_radix(N / Ns) < N) Ns *= ct_radix(N / Ns);
+  return Ns;
+}
+template <int R, typename T, typename T2>
+__device__ __forceinline__ void dft_r(T2* w, int s) {
+  if constexpr (R == 2) dft2<T, T2>(w);
+  if constexpr (R == 3) dft3<T, T2>(w, s);
+  if constexpr (R == 4) dft4<T, T2>(w, s);
+  if constexpr (R == 5) dft5<T, T2>(w, s);
+  if constexpr (R == 8) dft8<T, T2>(w, s);
 }
 
 // NN > 0: compile-time length NN; NN = 0: the run-time plan
@@ -405,6 +420,107 @@ __device__ __forceinline__ void fft_any_ip(T2* a, int N, int nrows, const FftPla
 // z transforms: spectral [i][k][j] <-> z-physical [i][z][j]
 // (localyz2zy + fourz(+1), fou3D.f:103-146 / fourz(-1) + localzy2yz)
 // ------------------------------------------------------------------------
+// z transform of a J-row tile with register-resident ends (compile-time
+// length N).  The first radix pass takes its items row-fastest (lanes
+// 0..J-1 are the J y rows at one z), so its R inputs per item come straight
+// from HBM in runs of J entries; it has no twiddles and writes LDS.  The
+// middle passes run in LDS; the last pass, again row-fastest, writes its
+// outputs z = j + r*Ns straight to HBM.  This saves the staging of the tile
+// through LDS on the way in and on the way out (two LDS passes, two
+// barriers).  INV: spectral [i][k][j] -> z-physical [i][z][j] (the dealiased
+// band reads as zero); else the reverse, scaled by 1/N, dealiased modes
+// dropped.
+#ifndef LOMA_ZF_REG
+#define LOMA_ZF_REG 1
+#endif
+template <typename T, typename T2, int E, int N, int PAD, bool INV>
+__device__ __forceinline__ void zfft_reg(const T2* __restrict__ src, T2* __restrict__ dst,
+                                         T2* A, int i, int j0, int J, int my, int myp, int mz,
+                                         int nz, const T2* __restrict__ tw) {
+  constexpr int RS = N + (PAD ? N / 8 : 0) + 1;
+  constexpr int R1 = ct_radix(N), nb1 = N / R1;
+  constexpr int NSL = ct_last_ns(N), RL = N / NSL, nbL = N / RL;
+  static_assert(NSL > 1, "at least two passes");
+  const int sgn = INV ? +1 : -1;
+  T2 v[E + 8];
+  // first pass
+  {
+    constexpr int nit = (E + R1 - 1) / R1;
+    const int total = J * nb1;
+#pragma unroll
+    for (int q = 0; q < nit; ++q) {
+      const int it = threadIdx.x + q * 256;
+      if (it < total) {
+        const int jj = it % J, j = it / J, jy = j0 + jj;
+        T2* w = v + q * R1;
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+          const int z = j + r * nb1;
+          T2 x = T2{0, 0};
+          if (jy < my) {
+            if constexpr (INV) {
+              const int k = z <= nz ? z : (z >= N - nz ? z - N + mz : -1);
+              if (k >= 0) x = src[((size_t)i * mz + k) * myp + jy];
+            } else {
+              x = src[((size_t)i * N + z) * myp + jy];
+            }
+          }
+          w[r] = x;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < nit; ++q) {
+      const int it = threadIdx.x + q * 256;
+      if (it < total) {
+        const int jj = it % J, j = it / J;
+        T2* w = v + q * R1;
+        dft_r<R1, T, T2>(w, sgn);
+#pragma unroll
+        for (int r = 0; r < R1; ++r) A[jj * RS + lds_i<PAD>(j * R1 + r)] = w[r];
+      }
+    }
+  }
+  __syncthreads();
+  // middle passes (Ns = R1 .. NSL)
+  fft_rows_ip_ct<T, T2, E, N, R1, 256, PAD, NSL>(A, J, tw, sgn, v);
+  // last pass
+  {
+    constexpr int nit = (E + RL - 1) / RL;
+    const int total = J * nbL;
+    const T sc = INV ? (T)1 : (T)(1.0 / (double)N);
+#pragma unroll
+    for (int q = 0; q < nit; ++q) {
+      const int it = threadIdx.x + q * 256;
+      if (it < total) {
+        const int jj = it % J, j = it / J, jy = j0 + jj;
+        T2* w = v + q * RL;
+#pragma unroll
+        for (int r = 0; r < RL; ++r) w[r] = A[jj * RS + lds_i<PAD>(j + r * nbL)];
+        twiddle<RL>(w, tw, j, 1, sgn);
+        dft_r<RL, T, T2>(w, sgn);
+        if (jy < my) {
+#pragma unroll
+          for (int r = 0; r < RL; ++r) {
+            const int z = j + r * NSL;
+            if constexpr (INV) {
+              dst[((size_t)i * N + z) * myp + jy] = w[r];
+            } else {
+              const int k = z <= nz ? z : (z >= N - nz ? z - N + mz : -1);
+              if (k >= 0) {
+                T2 y = w[r];
+                y.x *= sc;
+                y.y *= sc;
+                dst[((size_t)i * mz + k) * myp + jy] = y;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
 // One workgroup: J consecutive y entries of one kx plane of one field, all
 // z; one LDS buffer, in-place FFT (J * mgalz <= 256 * E).
 template <typename T, int E, int NN>
@@ -421,6 +537,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
   const int f = blockIdx.y;
   const T2* src = (const T2*)a.spec[f];
   T2* dst = (T2*)a.wbuf[f];
+  if constexpr (NN > 0 && LOMA_ZF_REG) {
+    // first pass from HBM, last pass to HBM (register-resident ends)
+    zfft_reg<T, T2, E, NN, PAD, true>(src, dst, A, i, j0, J, my, myp, mz, nz,
+                                      (const T2*)a.twz);
+    return;
+  }
   // the dealiased band nz+1 .. N-nz-1 is zero
   const int nzero = N - mz;
   for (int it = threadIdx.x; it < nzero * J; it += blockDim.x) {
@@ -456,6 +578,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
   const int f = blockIdx.y;
   const T2* src = (const T2*)a.wout[f];
   T2* dst = (T2*)a.spec[f];
+  if constexpr (NN > 0 && LOMA_ZF_REG) {
+    zfft_reg<T, T2, E, NN, PAD, false>(src, dst, A, i, j0, J, my, myp, mz, nz,
+                                       (const T2*)a.twz);
+    return;
+  }
   for (int it = threadIdx.x; it < N * J; it += blockDim.x) {
     const int z = it / J, jj = it - z * J, j = j0 + jj;
     T2 v = T2{0, 0};

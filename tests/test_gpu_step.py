@@ -112,6 +112,28 @@ def test_steps_mixed_x512(gpu, oracle_mod):
         assert e < 1e-5, (f, e)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_steps_z256(gpu, oracle_mod, precision):
+    """The production z length, 256 points (compile-time plan, transforms
+    that load and store HBM from their first and last radix passes): 3 steps
+    vs the FP64 oracle, 1e-10 per field in FP64 and 1e-5 per physical field
+    in the mixed form."""
+    from loma_phd_amd import capi, diagnostics, ic
+    cfg = _cfg(64, 65, 256, Re=250.0, s=2.5, seed=21)
+    print(f"3 steps 64x65x256 precision {precision}", flush=True)
+    run, S, g = _setup(cfg, precision)
+    run.advance(3)
+    S.step(3)
+    if precision == 0:
+        _compare(run, S, 1e-10, "3 steps fp64 z256")
+        return
+    a, b = run.download(), S.get()
+    errs = diagnostics.field_errors(a, b, ic.HostOps(cfg.y()), cfg.alp, cfg.bet)
+    print("3 steps mixed z256, physical fields", errs, flush=True)
+    for f, e in errs.items():
+        assert e < 1e-5, (f, e)
+
+
 def test_graph_replay_matches_eager(gpu):
     from loma_phd_amd import capi, ic
     cfg = _cfg(48, 97, 36, Re=160.0, s=3.0, seed=2)
