@@ -518,8 +518,8 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   {
     const int e2 = r->fp32 ? 8 : 16;
     // z transforms: J y entries per workgroup, J * mgalz <= 256 * E
-    // (E = 16 in FP32, 8 in FP64): one LDS buffer of at most 32 KiB
-    const int capz = r->fp32 ? 4096 : 2048;
+    // (E = LOMA_ZF_E32 in FP32, LOMA_ZF_E64 in FP64): one LDS buffer
+    const int capz = 256 * (r->fp32 ? LOMA_ZF_E32 : LOMA_ZF_E64);
     int J = 32;
     while (J > 1 && J * mgalz > capz) J /= 2;
     if (J * mgalz > capz) {

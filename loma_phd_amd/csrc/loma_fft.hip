@@ -1291,11 +1291,11 @@ int launch_zfft_inv(bool fp32, const FftArgs& a, hipStream_t st) {
   const int nyb = (a.rc.my + a.J - 1) / a.J;
   dim3 g(a.rc.nplanes * nyb, a.nfields), b(256);
   const size_t lds = zfft_lds(fp32, a);
-  // J * mgalz <= 256 * E: E = 16 in FP32, 8 in FP64 (checked at init)
+  // J * mgalz <= 256 * E (checked at init)
   if (fp32) {
-    LOMA_BY_LENGTH(a.rc.mgalz, hipLaunchKernelGGL((k_zfft_inv<float, 16, NN>), g, b, lds, st, a));
+    LOMA_BY_LENGTH(a.rc.mgalz, hipLaunchKernelGGL((k_zfft_inv<float, LOMA_ZF_E32, NN>), g, b, lds, st, a));
   } else {
-    LOMA_BY_LENGTH(a.rc.mgalz, hipLaunchKernelGGL((k_zfft_inv<double, 8, NN>), g, b, lds, st, a));
+    LOMA_BY_LENGTH(a.rc.mgalz, hipLaunchKernelGGL((k_zfft_inv<double, LOMA_ZF_E64, NN>), g, b, lds, st, a));
   }
   return (int)hipGetLastError();
 }
@@ -1305,9 +1305,9 @@ int launch_zfft_fwd(bool fp32, const FftArgs& a, hipStream_t st) {
   dim3 g(a.rc.nplanes * nyb, a.nfields), b(256);
   const size_t lds = zfft_lds(fp32, a);
   if (fp32) {
-    LOMA_BY_LENGTH(a.rc.mgalz, hipLaunchKernelGGL((k_zfft_fwd<float, 16, NN>), g, b, lds, st, a));
+    LOMA_BY_LENGTH(a.rc.mgalz, hipLaunchKernelGGL((k_zfft_fwd<float, LOMA_ZF_E32, NN>), g, b, lds, st, a));
   } else {
-    LOMA_BY_LENGTH(a.rc.mgalz, hipLaunchKernelGGL((k_zfft_fwd<double, 8, NN>), g, b, lds, st, a));
+    LOMA_BY_LENGTH(a.rc.mgalz, hipLaunchKernelGGL((k_zfft_fwd<double, LOMA_ZF_E64, NN>), g, b, lds, st, a));
   }
   return (int)hipGetLastError();
 }

@@ -65,6 +65,12 @@ struct PhaseArgs {
   int nmodes;                // modes handled by this launch
 };
 
+// complex entries per thread of the z transforms (J * mgalz <= 256 * E)
+#ifndef LOMA_ZF_E64
+#define LOMA_ZF_E64 8
+#endif
+#define LOMA_ZF_E32 16
+
 // device initial condition (loma_seed): seeded mixing-layer perturbation
 struct IcArgs {
   RunConst rc;
