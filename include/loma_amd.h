@@ -44,6 +44,9 @@ extern "C" {
 #define LOMA_FLAG_AUTOTUNE 4  /* time the transform tile shapes for this grid
                                  at init and keep the fastest (no reference
                                  counterpart: a tuning knob of this library) */
+#define LOMA_FLAG_HISTORY 8   /* compute the momentum and vorticity thickness
+                                 on the device every step (the .cf history,
+                                 trunk/lomacte3/src/cross.f:692-699,1414-1429) */
 
 typedef struct loma_run loma_run;
 
@@ -104,6 +107,14 @@ int loma_download_z(loma_run *run, void *z, int fp64);
  * (main.f:36-45); the caller's seeded generator replaces it. */
 int loma_seed(loma_run *run, const double *coef, int nymax, double s,
               double amp, double env_width, double k0);
+
+/* Momentum and vorticity thickness of a LOMA_FLAG_HISTORY run: out[0] =
+ * delta_m of the state at the start of the last step (the reference takes it
+ * at rkstep 1 from the x-z mean velocity, cross.f:1414-1429, with the
+ * trapezoid weights of main.f:689-696), out[1] = delta_w = 1/max|dU/dy| at
+ * the start of that step's third substep (rkstep 3, cross.f:692-699).  Both
+ * are 0 until a step has run; LOMA_ERR_STATE without the flag. */
+int loma_thickness(loma_run *run, double *out);
 
 /* Time advance: nsteps full RK3 steps (cross.f:225-1223 loop body),
  * device resident.  Returns LOMA_ERR_NAN if the state became non-finite. */

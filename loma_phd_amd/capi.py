@@ -17,7 +17,7 @@ LIB = os.environ.get("LOMA_HIP_LIB") or os.path.join(PKG, "_lib", "libloma_hip.s
 LOMA_OK, LOMA_ERR_ARG, LOMA_ERR_NAN, LOMA_ERR_OOM = 0, 1, 2, 3
 LOMA_ERR_DEVICE, LOMA_ERR_STATE, LOMA_ERR_NODEV = 4, 5, 6
 LOMA_FP64, LOMA_MIXED = 0, 1
-LOMA_FLAG_DT_FP32, LOMA_FLAG_SCALAR_Z, LOMA_FLAG_AUTOTUNE = 1, 2, 4
+LOMA_FLAG_DT_FP32, LOMA_FLAG_SCALAR_Z, LOMA_FLAG_AUTOTUNE, LOMA_FLAG_HISTORY = 1, 2, 4, 8
 
 # every symbol the header declares (checked by tests on the CPU)
 EXPORTS = [
@@ -29,7 +29,7 @@ EXPORTS = [
     "loma_inty8_dev", "loma_deryr2", "loma_deryyr2", "loma_lapvdvhom", "loma_solvepsi",
     "loma_fourz_dev", "loma_band5_solve_dev", "loma_advance_many", "loma_advance_async",
     "loma_tiles", "loma_footprint", "loma_step_begin", "loma_step_poll",
-    "loma_visc_dev", "loma_visc", "loma_set_physics", "loma_seed",
+    "loma_visc_dev", "loma_visc", "loma_set_physics", "loma_seed", "loma_thickness",
     "loma_sync", "loma_stream", "loma_profile", "loma_profile_read",
 ]
 KERNEL_CLASSES = ["phaseA", "zfft_inv", "xline", "zfft_fwd", "phaseC", "finish", "phaseC2"]
@@ -67,6 +67,7 @@ def lib():
             "loma_download": ([_vp, _vp, _vp, _vp, _vp, _dp, _dp, _dp, i], i),
             "loma_upload_z": ([_vp, _vp, i], i),
             "loma_seed": ([_vp, _vp, i, d, d, d, d], i),
+            "loma_thickness": ([_vp, _dp], i),
             "loma_download_z": ([_vp, _vp, i], i),
             "loma_advance": ([_vp, i], i),
             "loma_substep": ([_vp, i], i),
@@ -206,6 +207,12 @@ class Run:
             raise ValueError(f"coef shape {c.shape} is not (3, nplanes, mz, nymax)")
         check(lib().loma_seed(self._h, c.ctypes.data_as(_vp), c.shape[3], float(s),
                               float(amp), float(env_width), float(k0)))
+
+    def thickness(self):
+        """(delta_m, delta_w) of a LOMA_FLAG_HISTORY run (loma_thickness)."""
+        out = np.zeros(2)
+        check(lib().loma_thickness(self._h, _d(out)))
+        return float(out[0]), float(out[1])
 
     def download(self):
         f = [np.zeros(self.shape(), np.complex128) for _ in range(4)]

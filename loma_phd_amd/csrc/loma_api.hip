@@ -885,6 +885,9 @@ static int enqueue_substep(loma_run* r, int rkstep, int icfl) {
   PROF(KC_ZFWD, launch_zfft_fwd(r->fp32, fa, st));
   PROF(KC_PHASEC, launch_phaseC(r->M, r->fp32, pa, st));
   PROF(KC_PHASEC2, launch_phaseC2(r->M, r->fp32, pa, st));
+  if ((r->flags & LOMA_FLAG_HISTORY) && rkstep != 2)
+    LCHK(launch_thickness(r->M, r->fp32, pa, r->hy, r->y[r->my - 1] - r->y[0],
+                         rkstep == 1 ? 1 : 3, st));
   return LOMA_OK;
 }
 
@@ -1265,6 +1268,20 @@ int loma_scalars(loma_run* r, double* out) {
   int e = read_scalars(r, &s);
   if (e) return e;
   out[0] = s.dt; out[1] = s.time; out[2] = s.dtr; out[3] = r->istep; out[4] = s.nan;
+  return LOMA_OK;
+}
+
+int loma_thickness(loma_run* r, double* out) {
+  if (!r || !out) return LOMA_ERR_ARG;
+  if (!(r->flags & LOMA_FLAG_HISTORY)) {
+    set_error("loma_thickness: the run was made without LOMA_FLAG_HISTORY");
+    return LOMA_ERR_STATE;
+  }
+  DevScalars s;
+  int e = read_scalars(r, &s);
+  if (e) return e;
+  out[0] = s.dm;
+  out[1] = s.dw;
   return LOMA_OK;
 }
 

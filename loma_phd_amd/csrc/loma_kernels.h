@@ -23,6 +23,8 @@ struct DevScalars {
   unsigned long long cfld_bits;     // max diffusive inverse dt (as bits)
   int nan;
   int pad;
+  double dm;      // momentum thickness of the state at the start of the last step
+  double dw;      // vorticity thickness at the start of its third substep
 };
 
 // per-run physics constants passed by value to kernels
@@ -130,6 +132,8 @@ int launch_phaseA(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
 int launch_phaseC(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
 int launch_phaseC2(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
 int launch_ic(int M, const IcArgs& a, hipStream_t st);
+int launch_thickness(int M, bool fp32, const PhaseArgs& a, const double* hy, double ly,
+                     int which, hipStream_t st);
 int launch_finish_step(DevScalars* sc, RunConst rc, int icfl, hipStream_t st);
 
 int launch_zfft_inv(bool fp32, const FftArgs& a, hipStream_t st);

@@ -24,6 +24,8 @@ template <int M>
 int launch_phase_m(int, bool, const PhaseArgs&, hipStream_t);
 template <int M>
 int launch_ic_m(const IcArgs&, hipStream_t);
+template <int M>
+int launch_thickness_m(bool, const PhaseArgs&, const double*, double, int, hipStream_t);
 
 RkCoef rk_coefficients() {
   RkCoef r;
@@ -98,6 +100,10 @@ int launch_phaseC2(int M, bool fp32, const PhaseArgs& a, hipStream_t st) {
 }
 int launch_ic(int M, const IcArgs& a, hipStream_t st) {
   LOMA_DISPATCH_M(M, launch_ic_m<MM>(a, st));
+}
+int launch_thickness(int M, bool fp32, const PhaseArgs& a, const double* hy, double ly,
+                     int which, hipStream_t st) {
+  LOMA_DISPATCH_M(M, launch_thickness_m<MM>(fp32, a, hy, ly, which, st));
 }
 
 // LU of (dt22 - s dt21) with identity end rows (Lapvdvhom laps.f:349-366),

@@ -1002,6 +1002,51 @@ __global__ __launch_bounds__(256) void k_ic(IcArgs a) {
   if (a.with_z) store_col<M>(a.zf + o, my, lane, st, x);
 }
 
+// Momentum and vorticity thickness from the x-z mean velocity u00(y): the
+// 00 column of the transformed u (spec slot 0) plus its FP64 mean-split
+// profile, as the reference takes them from the 00 mode of foup2's output.
+// which = 1 (rkstep 1): delta_m = Ly/4 - sum_j trp_j (u_{j+1}^2 + u_{j-1}^2)
+// / (u_top - u_bot)^2 with trp = hy/2 (cross.f:1414-1429, main.f:689-696);
+// which = 3 (rkstep 3): delta_w = 1 / max |d u00/dy| (cross.f:692-699).
+// One wavefront; the result goes to the run's device scalars.
+template <int M, typename TS>
+__global__ __launch_bounds__(64) void k_thickness(PhaseArgs a, const double* hy, double ly,
+                                                  int which) {
+  using T2 = typename std::conditional<sizeof(TS) == 4, float2, double2>::type;
+  LOMA_STAGE(M);
+  const int lane = threadIdx.x & 63, my = a.rc.my, r0 = lane * M;
+  double u[2][M];
+  load_col<M>((const T2*)a.spec[0], my, lane, st, u);
+  add_mean<M>(u, lane, my, true, a.mean_out);
+  if (which == 1) {
+    wave_sync();
+    stage_put<M>(st, lane, u);
+    wave_sync();
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+      const int r = r0 + t;
+      if (r >= my) continue;
+      const int lo = r == 0 ? 0 : r - 1, hi = r == my - 1 ? my - 1 : r + 1;
+      const double ul = st[lo].x, uh = st[hi].x;
+      s += 0.5 * hy[r] * (uh * uh + ul * ul);
+    }
+    s = wave_sum(s);
+    const double du = st[my - 1].x - st[0].x;
+    if (lane == 0) a.sc->dm = 0.25 * ly - s / (du * du);
+  } else {
+    double d[2][M];
+    op_dy1<M, 2>(a.T, lane, u, d);
+    double m = 0.0;
+#pragma unroll
+    for (int t = 0; t < M; ++t)
+      if (r0 + t < my) m = fmax(m, fabs(d[0][t]));
+#pragma unroll
+    for (int dd = 32; dd >= 1; dd >>= 1) m = fmax(m, __shfl_xor(m, dd));
+    if (lane == 0) a.sc->dw = 1.0 / m;
+  }
+}
+
 // per-M launchers (instantiated in loma_yops_m*.hip, one chunk size each)
 template <int M>
 int launch_dy_m(int order, const YTab& T, const double* u, double* du, int ncols,
@@ -1062,6 +1107,13 @@ int launch_phase_m(int which, bool fp32, const PhaseArgs& a, hipStream_t st) {
 }
 
 template <int M>
+int launch_thickness_m(bool fp32, const PhaseArgs& a, const double* hy, double ly, int which,
+                       hipStream_t st) {
+  if (fp32) hipLaunchKernelGGL((k_thickness<M, float>), dim3(1), dim3(64), 0, st, a, hy, ly, which);
+  else hipLaunchKernelGGL((k_thickness<M, double>), dim3(1), dim3(64), 0, st, a, hy, ly, which);
+  return (int)hipGetLastError();
+}
+template <int M>
 int launch_ic_m(const IcArgs& a, hipStream_t st) {
   const int nmodes = a.rc.nplanes * a.rc.mz;
   hipLaunchKernelGGL((k_ic<M>), dim3((nmodes + 3) / 4), dim3(256), 0, st, a);
@@ -1081,6 +1133,8 @@ int launch_ic_m(const IcArgs& a, hipStream_t st) {
                                  double*, double*, const double*, double, double, double, \
                                  int, hipStream_t);                                       \
   template int launch_phase_m<MV>(int, bool, const PhaseArgs&, hipStream_t);              \
-  template int launch_ic_m<MV>(const IcArgs&, hipStream_t);
+  template int launch_ic_m<MV>(const IcArgs&, hipStream_t);                              \
+  template int launch_thickness_m<MV>(bool, const PhaseArgs&, const double*, double, int,  \
+                                      hipStream_t);
 
 }  // namespace loma
