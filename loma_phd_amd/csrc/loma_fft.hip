@@ -799,6 +799,30 @@ __device__ __forceinline__ void xl_forward(const FftArgs& a, T2* __restrict__ ds
 // row, which are exactly the entries x0 + 64m the physics assigns to the
 // thread; so the inverse keeps its last pass in registers (no store, no
 // barrier, no reload) and the forward takes its first pass from registers.
+// Barriers of the register-resident path.  Its radix passes are local to a
+// wavefront (wave p owns line pair p), so LOMA_XL_WAVE = 2 (default) puts
+// only a wavefront barrier between them and keeps workgroup barriers around
+// the pack and unpack, whose coalesced lane mapping crosses waves: 2.74
+// against 2.77 ms per launch at 512x513x256.  LOMA_XL_WAVE = 1 also packs
+// and unpacks per wave (16 bytes per lane instead of 64-byte runs, no
+// workgroup barrier at all): 2.80 ms.  0: workgroup barriers throughout.
+#ifndef LOMA_XL_WAVE
+#define LOMA_XL_WAVE 2
+#endif
+// barrier between two wave-local steps (LOMA_XL_WAVE >= 1: the lanes of
+// one wavefront only) and between steps that cross waves (a workgroup
+// barrier unless everything is wave-local, LOMA_XL_WAVE = 1)
+template <typename T, typename T2>
+__device__ __forceinline__ void xl_sync() {
+  if constexpr (LOMA_XL_WAVE >= 1) wave_sync();
+  else __syncthreads();
+}
+template <typename T, typename T2>
+__device__ __forceinline__ void xl_sync_x() {
+  if constexpr (LOMA_XL_WAVE == 1) wave_sync();
+  else __syncthreads();
+}
+
 template <typename T, typename T2, int XR>
 __device__ __forceinline__ void xl_inverse_reg(const FftArgs& a, const T2* __restrict__ src,
                                                T2* A, T2* B, int z, int j0, int pairs,
@@ -807,12 +831,44 @@ __device__ __forceinline__ void xl_inverse_reg(const FftArgs& a, const T2* __res
   constexpr int RS = 512 + LOMA_XL_RS1;
   const T2* tw = (const T2*)a.twx;
   const int row = threadIdx.x >> 6, j = threadIdx.x & 63;
-  xl_pack<T, T2, XR, 512, 1>(a, src, nullptr, A, z, j0, pairs);
-  __syncthreads();
+  if constexpr (LOMA_XL_WAVE == 1) {
+    // wave `row` packs line pair `row`: lane j takes coefficients ii = j + 64k
+    constexpr int np = 512 / 3;
+    const int ja = j0 + 2 * row, jb = ja + 1, my = a.rc.my, mgz = a.rc.mgalz;
+    T2* Ar = A + row * RS;
+    T2 ca[3], cb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int ii = j + 64 * k;
+      ca[k] = T2{0, 0};
+      cb[k] = T2{0, 0};
+      if (ii < np) {
+        const size_t base = ((size_t)ii * mgz + z) * a.rc.myp;
+        if (ja < my) ca[k] = src[base + ja];
+        if (jb < my) cb[k] = src[base + jb];
+      }
+    }
+    for (int q = j; q < 512 - 2 * np + 1; q += 64) Ar[np + q] = T2{0, 0};  // dealiased band
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int ii = j + 64 * k;
+      if (ii < np) {
+        if (ii == 0) {
+          Ar[0] = T2{ca[k].x, cb[k].x};  // kx = 0: imaginary part ignored (rftsingle.f:87-91)
+        } else {
+          Ar[ii] = T2{ca[k].x - cb[k].y, ca[k].y + cb[k].x};
+          Ar[512 - ii] = T2{ca[k].x + cb[k].y, -ca[k].y + cb[k].x};
+        }
+      }
+    }
+  } else {
+    xl_pack<T, T2, XR, 512, 1>(a, src, nullptr, A, z, j0, pairs);
+  }
+  xl_sync_x<T, T2>();
   stockham_item<8, T, T2, 0>(A, B, row * RS, j, 64, 1, 64, tw, +1);
-  __syncthreads();
+  xl_sync<T, T2>();
   stockham_item<8, T, T2, 0>(B, A, row * RS, j, 64, 8, 8, tw, +1);
-  __syncthreads();
+  xl_sync<T, T2>();
   T2 w[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) w[r] = A[row * RS + j + r * 64];
@@ -823,7 +879,7 @@ __device__ __forceinline__ void xl_inverse_reg(const FftArgs& a, const T2* __res
     v[r][0] = w[r].x;
     v[r][1] = w[r].y;
   }
-  __syncthreads();  // A is packed again by the next field
+  xl_sync_x<T, T2>();  // A is packed again by the next field
 }
 
 template <typename T, typename T2>
@@ -840,13 +896,36 @@ __device__ __forceinline__ void xl_forward_reg(const FftArgs& a, T2* __restrict_
   dft8<T, T2>(w, -1);  // first pass (no twiddles): entries j + 64r -> 8j + r
 #pragma unroll
   for (int r = 0; r < 8; ++r) A[row * RS + 8 * j + r] = w[r];
-  __syncthreads();
+  xl_sync<T, T2>();
   stockham_item<8, T, T2, 0>(A, B, row * RS, j, 64, 8, 8, tw, -1);
-  __syncthreads();
+  xl_sync<T, T2>();
   stockham_item<8, T, T2, 0>(B, A, row * RS, j, 64, 64, 1, tw, -1);
-  __syncthreads();
-  xl_unpack<T, T2, 512, 1>(a, A, dst, nullptr, z, j0, pairs);
-  __syncthreads();
+  xl_sync_x<T, T2>();
+  if constexpr (LOMA_XL_WAVE == 1) {
+    // wave `row` unpacks and stores line pair `row`
+    constexpr int np = 512 / 3;
+    const int ja = j0 + 2 * row, jb = ja + 1, my = a.rc.my, mgz = a.rc.mgalz;
+    const T2* Ar = A + row * RS;
+    const T hs = (T)(0.5 / 512.0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int ii = j + 64 * k;
+      if (ii < np) {
+        const T2 zk = Ar[ii];
+        T2 zc = Ar[(512 - ii) & 511];
+        zc.y = -zc.y;
+        T2 ya{(zk.x + zc.x) * hs, (zk.y + zc.y) * hs};
+        T2 yb{(zk.y - zc.y) * hs, -(zk.x - zc.x) * hs};  // (zk - zc) / (2i)
+        if (ii == 0) { ya.y = 0; yb.y = 0; }
+        const size_t base = ((size_t)ii * mgz + z) * a.rc.myp;
+        if (ja < my) dst[base + ja] = ya;
+        if (jb < my) dst[base + jb] = yb;
+      }
+    }
+  } else {
+    xl_unpack<T, T2, 512, 1>(a, A, dst, nullptr, z, j0, pairs);
+  }
+  xl_sync_x<T, T2>();
 }
 
 // Mean split: the transform inputs carry no 00 mode; its y-profile comes in
