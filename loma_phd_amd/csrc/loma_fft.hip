@@ -305,7 +305,15 @@ constexpr int ct_radix(int m) {
   return m % 8 == 0 ? 8 : (m % 4 == 0 ? 4 : (m % 2 == 0 ? 2 : (m % 3 == 0 ? 3 : 5)));
 }
 
-template <typename T, typename T2, int N, int Ns, int NT, int RS, int PAD = LOMA_LDS_PAD>
+// WR: every pass has 64 items per row (length 512 in radix 8), so the items
+// of a row stay in one wavefront from pass to pass; the passes are then
+// separated by wavefront barriers and only the last ends with a workgroup
+// barrier (its result is read across waves).
+#ifndef LOMA_XL_WR
+#define LOMA_XL_WR 1
+#endif
+template <typename T, typename T2, int N, int Ns, int NT, int RS, int PAD = LOMA_LDS_PAD,
+          bool WR = false>
 __device__ __forceinline__ T2* fft_rows_ct(T2* a, T2* b, int nrows, const T2* __restrict__ tw,
                                            int s) {
   if constexpr (Ns >= N) {
@@ -323,8 +331,10 @@ __device__ __forceinline__ T2* fft_rows_ct(T2* a, T2* b, int nrows, const T2* __
       const int row = it / nb, j = it - row * nb;
       stockham_item<R, T, T2, PAD>(a, b, row * RS, j, nb, Ns, tstep, tw, s);
     }
-    __syncthreads();
-    return fft_rows_ct<T, T2, N, Ns * R, NT, RS, PAD>(b, a, nrows, tw, s);
+    static_assert(!WR || (nb == 64 && NT % 64 == 0), "wave-local rows need 64 items per row");
+    if constexpr (WR && Ns * R < N) wave_sync();
+    else __syncthreads();
+    return fft_rows_ct<T, T2, N, Ns * R, NT, RS, PAD, WR>(b, a, nrows, tw, s);
   }
 }
 
@@ -401,7 +411,8 @@ template <typename T, typename T2, int NN, int NT, int PAD = LOMA_LDS_PAD>
 __device__ __forceinline__ T2* fft_any(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
                                        const T2* __restrict__ tw, int s) {
   if constexpr (NN > 0)
-    return fft_rows_ct<T, T2, NN, 1, NT, NN + LOMA_XL_RS1, PAD>(a, b, nrows, tw, s);
+    return fft_rows_ct<T, T2, NN, 1, NT, NN + LOMA_XL_RS1, PAD,
+                       (NN == 512 && LOMA_XL_WR != 0)>(a, b, nrows, tw, s);
   else
     return fft_rows<T, T2, PAD>(a, b, N, N + LOMA_XL_RS1, nrows, pl, tw, s);
 }

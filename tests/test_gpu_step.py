@@ -94,17 +94,24 @@ def test_steps_mixed(gpu, oracle_mod):
         assert e < 1e-3, (f, e)
 
 
-def test_steps_mixed_x512(gpu, oracle_mod):
-    """Mixed form at the production x length, 512 points, where the FP32 x
-    lines keep the ends of their transforms in registers: vs the FP64 oracle
-    after 3 steps, per physical field u, v, w (as rho*u_i), T and Z: 1e-5."""
+@pytest.mark.parametrize("precision", [1, 0])
+def test_steps_x512(gpu, oracle_mod, precision):
+    """The production x length, 512 points (FP32 x lines with the ends of
+    their transforms in registers, FP64 x lines with two fields per
+    transform; both with wavefront barriers between the radix passes): vs
+    the FP64 oracle after 3 steps, 1e-10 per field in FP64 and 1e-5 per
+    physical field u, v, w (as rho*u_i), T and Z in the mixed form."""
     from loma_phd_amd import capi, diagnostics, ic
     cfg = _cfg(512, 97, 48, Re=300.0, s=3.0, seed=9)
-    print("3 steps mixed 512x97x48", flush=True)
-    run, S, g = _setup(cfg, capi.LOMA_MIXED)
-    assert run.tiles()[1] == 8  # 4 line pairs per workgroup: the register-resident path
+    print(f"3 steps 512x97x48 precision {precision}", flush=True)
+    run, S, g = _setup(cfg, precision)
+    if precision == 1:
+        assert run.tiles()[1] == 8  # 4 line pairs per workgroup: the register-resident path
     run.advance(3)
     S.step(3)
+    if precision == 0:
+        _compare(run, S, 1e-10, "3 steps fp64 x512")
+        return
     a, b = run.download(), S.get()
     errs = diagnostics.field_errors(a, b, ic.HostOps(cfg.y()), cfg.alp, cfg.bet)
     print("3 steps mixed 512, physical fields", errs, flush=True)
