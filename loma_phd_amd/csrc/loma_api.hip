@@ -83,6 +83,7 @@ struct loma_run {
          *rf0w = nullptr;
   double *mean_in = nullptr, *mean_out = nullptr, *rbar = nullptr, *rzbar = nullptr;
   DevScalars* sc = nullptr;
+  double* cflp = nullptr;  // [x-line tile][4] CFL maxima
   FftPlan plz{}, plx{};
   void *twz = nullptr, *twx = nullptr;
   double2* twz64 = nullptr;  // double twiddles for the per-operator fourz
@@ -171,6 +172,7 @@ static size_t plan_bytes(int mgalx, int my, int mgalz, bool fp32, bool with_z) {
   b += 6 * align256(8 * (size_t)my);                           // 00-mode work
   b += 2 * align256(8 * 12 * (size_t)my) + 2 * align256(8 * 2 * (size_t)my);  // mean split
   b += align256(sizeof(DevScalars));
+  b += align256(32 * (size_t)cfl_tiles_max(my, mgalz));           // CFL maxima per x-line tile
   b += align256(tw * mgalz) + align256(tw * mgalx) + align256(16 * (size_t)mgalz);  // twiddles
   return b;
 }
@@ -481,6 +483,7 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
     hipStreamSynchronize(sst);  // hyv is a local
   }
   if ((rc = dalloc(r, &r->sc, sizeof(DevScalars)))) { loma_free(r); return rc; }
+  if ((rc = dalloc(r, &r->cflp, 32 * (size_t)cfl_tiles_max(my, mgalz)))) { loma_free(r); return rc; }
   // twiddles
   auto mk_tw = [&](int n, void** dst) -> int {
     std::vector<double> tw(2 * n);
@@ -855,6 +858,7 @@ static FftArgs fft_args(loma_run* r, int rkstep, int do_cfl) {
   a.rzbar = r->rzbar;
   a.hy = r->hy;
   a.sc = r->sc;
+  a.cflp = r->cflp;
   a.rkstep = rkstep;
   a.do_cfl = do_cfl;
   a.with_z = r->with_z ? 1 : 0;
@@ -970,7 +974,8 @@ static int enqueue_step(loma_run* r, int icfl) {
     int e = enqueue_substep(r, rk, icfl);
     if (e) return e;
   }
-  PROF(KC_FINISH, launch_finish_step(r->sc, r->rc, icfl, r->stream));
+  PROF(KC_FINISH, launch_finish_step(r->sc, r->rc, icfl, r->cflp,
+                                     xl_tiles(r->my, r->mgalz, r->L), r->stream));
   return LOMA_OK;
 }
 

@@ -28,6 +28,10 @@ constexpr int WAVE = 64;
 // registers at once.
 __device__ __forceinline__ void op_fence() { __builtin_amdgcn_sched_barrier(0); }
 
+// max that keeps a NaN (fmax drops it): a non-finite state reaches the CFL
+// maxima and from there the step size, where it is flagged
+__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
+
 // ---- buffer addressing ----------------------------------------------------
 // Column and table accesses go through buffer resources held in scalar
 // registers: the per-row and per-entry offsets are scalar (or instruction

@@ -626,11 +626,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE
 //           r = T (lap T / Pe - rho u . grad T), then DT, drho (and DZ).
 // Every global access of a field is a run of 2*PAIRS consecutive y entries.
 // ------------------------------------------------------------------------
-__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
-  if (!(v >= 0.0)) v = __longlong_as_double(0x7ff8000000000000LL);  // NaN -> max bits
-  atomicMax(p, (unsigned long long)__double_as_longlong(v));
-}
-
 // 256 threads per x-line workgroup (512 measured slower on MI355X: more
 // spills at the 128-VGPR budget that 8 waves per SIMD pair impose)
 #ifndef LOMA_XL_THREADS
@@ -983,6 +978,7 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
   using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double gln[12][XL_MAXL];   // output estimates per line
+  __shared__ double cflw[XL_THREADS / 64][4];  // CFL maxima per wave
   const RunConst& rc = a.rc;
   const int N = NN ? NN : rc.mgalx, my = rc.my;
   const int pairs = a.L / 2;            // a power of two dividing XL_THREADS
@@ -1125,23 +1121,37 @@ __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 :
       for (int m = 0; m < EPT; ++m) {
         if (ent(m) >= N) continue;
         const double Tt = (double)tt[m][h] + pT[h];
-        cx = fmax(cx, fabs(((double)ru[m][h] + pu[h]) * Tt));
-        cy = fmax(cy, fabs(((double)rv[m][h] + pv[h]) * Tt) * ihy);  // cross.f:1705-1707
-        cz = fmax(cz, fabs(((double)rw[m][h] + pw[h]) * Tt));
-        cd = fmax(cd, 2.0 * (ipe * Tt) * (hx * hx + hz * hz + ihy * ihy));
+        cx = max_nan(cx, fabs(((double)ru[m][h] + pu[h]) * Tt));
+        cy = max_nan(cy, fabs(((double)rv[m][h] + pv[h]) * Tt) * ihy);  // cross.f:1705-1707
+        cz = max_nan(cz, fabs(((double)rw[m][h] + pw[h]) * Tt));
+        cd = max_nan(cd, 2.0 * (ipe * Tt) * (hx * hx + hz * hz + ihy * ihy));
       }
     }
     for (int d = 32; d >= 1; d >>= 1) {
-      cx = fmax(cx, __shfl_xor(cx, d));
-      cy = fmax(cy, __shfl_xor(cy, d));
-      cz = fmax(cz, __shfl_xor(cz, d));
-      cd = fmax(cd, __shfl_xor(cd, d));
+      cx = max_nan(cx, __shfl_xor(cx, d));
+      cy = max_nan(cy, __shfl_xor(cy, d));
+      cz = max_nan(cz, __shfl_xor(cz, d));
+      cd = max_nan(cd, __shfl_xor(cd, d));
     }
+    // one partial per tile, plain stores (k_finish_step reduces them): the
+    // same-address atomics this replaces serialised on one cache line and
+    // cost 3.1 ms per FP64 launch at 512x513x256
     if ((threadIdx.x & 63) == 0) {
-      atomic_max_nonneg(&a.sc->cflc_bits[0], cx);
-      atomic_max_nonneg(&a.sc->cflc_bits[1], cy);
-      atomic_max_nonneg(&a.sc->cflc_bits[2], cz);
-      atomic_max_nonneg(&a.sc->cfld_bits, cd);
+      const int w = threadIdx.x >> 6;
+      cflw[w][0] = cx; cflw[w][1] = cy; cflw[w][2] = cz; cflw[w][3] = cd;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int w = 1; w < XL_THREADS / 64; ++w) {
+        cx = max_nan(cx, cflw[w][0]);
+        cy = max_nan(cy, cflw[w][1]);
+        cz = max_nan(cz, cflw[w][2]);
+        cd = max_nan(cd, cflw[w][3]);
+      }
+      double2* o = (double2*)(a.cflp + 4 * (size_t)tile);
+      o[0] = double2{cx, cy};
+      o[1] = double2{cz, cd};
     }
   }
   // outputs 0 u 1 v 2 w 3 ruu 4 rvv 5 rww 6 ruv 7 ruw 8 rvw  (cross.f:1613-1651)

@@ -19,8 +19,6 @@ struct DevScalars {
   double dt;      // current step size (Deltat)
   double time;
   double dtr;     // next step size from the CFL estimate
-  unsigned long long cflc_bits[3];  // max |u|, max |v|/hy, max |w| (as bits)
-  unsigned long long cfld_bits;     // max diffusive inverse dt (as bits)
   int nan;
   int pad;
   double dm;      // momentum thickness of the state at the start of the last step
@@ -104,6 +102,9 @@ struct FftArgs {
   double *rbar, *rzbar;        // [2][my] mean-profile RHS of T and Z by substep parity
   const double* hy;          // my
   DevScalars* sc;
+  // CFL maxima of each x-line tile, [tile][4]: max |u|, max |v|/hy, max |w|,
+  // max diffusive inverse dt (written by the substep that takes the CFL)
+  double* cflp;
   int nfields;
   int rkstep;
   int do_cfl;
@@ -134,7 +135,14 @@ int launch_phaseC2(int M, bool fp32, const PhaseArgs& a, hipStream_t st);
 int launch_ic(int M, const IcArgs& a, hipStream_t st);
 int launch_thickness(int M, bool fp32, const PhaseArgs& a, const double* hy, double ly,
                      int which, hipStream_t st);
-int launch_finish_step(DevScalars* sc, RunConst rc, int icfl, hipStream_t st);
+// x-line tiles of a grid (mgalz * ceil(my / L)); at most cfl_tiles_max,
+// since L >= 2
+__host__ __device__ inline int xl_tiles(int my, int mgalz, int L) {
+  return mgalz * ((my + L - 1) / L);
+}
+__host__ __device__ inline int cfl_tiles_max(int my, int mgalz) { return xl_tiles(my, mgalz, 2); }
+int launch_finish_step(DevScalars* sc, RunConst rc, int icfl, const double* cflp, int ntile,
+                       hipStream_t st);
 
 int launch_zfft_inv(bool fp32, const FftArgs& a, hipStream_t st);
 int launch_xline(bool fp32, const FftArgs& a, hipStream_t st);

@@ -161,29 +161,53 @@ int launch_helm_factor(const double* dt21, const double* dt22, int my, int M,
 }
 
 // end of step: time += dt; dt <- CFL estimate (cross.f:1196-1204,
-// 1769-1818).  NaN flag from the CFL maxima.
-__global__ void k_finish_step(DevScalars* sc, RunConst rc, int icfl) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// 1769-1818) from the maxima of the x-line tiles, reduced here by one
+// workgroup.  NaN flag from the CFL maxima.
+constexpr int FINISH_THREADS = 1024;
+__global__ __launch_bounds__(FINISH_THREADS) void k_finish_step(DevScalars* sc, RunConst rc,
+                                                                 int icfl, const double* cflp,
+                                                                 int ntile) {
+  __shared__ double red[FINISH_THREADS / 64][4];
+  double c[4] = {0.0, 0.0, 0.0, 0.0};
+  if (icfl) {
+    for (int t = threadIdx.x; t < ntile; t += FINISH_THREADS) {
+      const double2 p = ((const double2*)cflp)[2 * (size_t)t];
+      const double2 q = ((const double2*)cflp)[2 * (size_t)t + 1];
+      c[0] = max_nan(c[0], p.x);
+      c[1] = max_nan(c[1], p.y);
+      c[2] = max_nan(c[2], q.x);
+      c[3] = max_nan(c[3], q.y);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      for (int d = 32; d >= 1; d >>= 1) c[k] = max_nan(c[k], __shfl_xor(c[k], d));
+    if ((threadIdx.x & 63) == 0)
+      for (int k = 0; k < 4; ++k) red[threadIdx.x >> 6][k] = c[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int w = 1; w < FINISH_THREADS / 64; ++w)
+        for (int k = 0; k < 4; ++k) c[k] = max_nan(c[k], red[w][k]);
+  }
+  if (threadIdx.x != 0) return;
   sc->time = sc->time + sc->dt;
   if (icfl) {
-    const double cx = __longlong_as_double((long long)sc->cflc_bits[0]);
-    const double cy = __longlong_as_double((long long)sc->cflc_bits[1]);
-    const double cz = __longlong_as_double((long long)sc->cflc_bits[2]);
-    const double cd = __longlong_as_double((long long)sc->cfld_bits);
+    const double cx = c[0], cy = c[1], cz = c[2], cd = c[3];
     const double conv = cx * rc.hxalp * 0.5 + cy + cz * rc.hzbet * 0.5;
     double dtr = rc.cfl / (conv + cd);
     if (rc.dt_fp32) dtr = (double)(float)dtr;
     if (!(dtr > 0.0) || isinf(dtr)) sc->nan = 1;
     sc->dtr = dtr;
     sc->dt = dtr;
-    sc->cflc_bits[0] = sc->cflc_bits[1] = sc->cflc_bits[2] = 0ull;
-    sc->cfld_bits = 0ull;
   }
 }
 
 
-int launch_finish_step(DevScalars* sc, RunConst rc, int icfl, hipStream_t st) {
-  hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(64), 0, st, sc, rc, icfl);
+int launch_finish_step(DevScalars* sc, RunConst rc, int icfl, const double* cflp, int ntile,
+                       hipStream_t st) {
+  if (icfl && (!cflp || ntile <= 0 || ntile > cfl_tiles_max(rc.my, rc.mgalz)))
+    return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(FINISH_THREADS), 0, st, sc, rc, icfl, cflp,
+                     ntile);
   return (int)hipGetLastError();
 }
 

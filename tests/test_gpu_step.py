@@ -141,6 +141,36 @@ def test_steps_z256(gpu, oracle_mod, precision):
         assert e < 1e-5, (f, e)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_nonfinite_state_is_reported(gpu, precision):
+    """A NaN in one spectral coefficient spreads over its y rows in physical
+    space, reaches the CFL maxima (which keep a NaN) and the step size, and
+    the step reports a non-finite state (LOMA_ERR_NAN).  The NaN is made on
+    the host; every kernel runs its normal bounded path."""
+    from loma_phd_amd import capi, ic
+    cfg = _cfg(48, 97, 36, Re=160.0, s=2.0, seed=4)
+    print(f"non-finite state precision {precision}", flush=True)
+    st = ic.initial_condition(cfg)
+    vor = st["vor"].copy()
+    vor[1, 2, cfg.my // 2] = np.nan
+    run = capi.Run(cfg.mgalx, cfg.my, cfg.mgalz, st["y"], cfg.Re, cfg.alp, cfg.bet, cfg.pr,
+                   cfg.cfl, cfg.ncfl, cfg.dt0, precision, 0)
+    run.upload(vor, st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"])
+    with pytest.raises(capi.LomaError) as e:
+        run.advance(1)
+    assert e.value.code == capi.LOMA_ERR_NAN
+    assert run.scalars()["nan"] == 1
+    run.close()
+    # the same grid with a finite state steps normally
+    run = capi.Run(cfg.mgalx, cfg.my, cfg.mgalz, st["y"], cfg.Re, cfg.alp, cfg.bet, cfg.pr,
+                   cfg.cfl, cfg.ncfl, cfg.dt0, precision, 0)
+    run.upload(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"])
+    run.advance(2)
+    sc = run.scalars()
+    assert sc["nan"] == 0 and np.isfinite(sc["dt"]) and sc["dt"] > 0.0
+    run.close()
+
+
 def test_graph_replay_matches_eager(gpu):
     from loma_phd_amd import capi, ic
     cfg = _cfg(48, 97, 36, Re=160.0, s=3.0, seed=2)
