@@ -617,7 +617,7 @@ __global__ __launch_bounds__(256, 2) void k_phaseC(PhaseArgs a) {
   //   S0 = kx^2 ruu + 2 kxkz ruw + kz^2 rww,
   //   omega_y = ikz u - ikx w, (curl omega)_y = d/dy(ikx u + ikz w) + rK v.
   // At most three columns are live in registers; X = ikx u + ikz w and G2
-  // wait in LDS.
+  // wait in LDS, and RHS(phi) takes X's place for the last two derivatives.
   double A[2][M], B[2][M];
   // omega_y -> RHS(Omega_y) viscous part in A; X parked
   {
@@ -728,6 +728,12 @@ __global__ __launch_bounds__(256, 2) void k_phaseC(PhaseArgs a) {
   {
     double X[2][M], C[2][M];
     stage_get<M>(pk, lane, X);
+    // RHS(phi) so far waits in the parked column while the two derivatives
+    // run with two columns live (three spilled 124 bytes per lane at M = 9);
+    // the fences keep the compiler from forwarding the stored values, which
+    // would hold them in registers again
+    stage_put<M>(pk, lane, A);
+    asm volatile("" ::: "memory");
     op_dy1<M, 2>(a.T, lane, X, C);
     stage_in<M>(sp[1], my, lane, st);  // v
 #pragma unroll
@@ -737,6 +743,8 @@ __global__ __launch_bounds__(256, 2) void k_phaseC(PhaseArgs a) {
       C[1][t] += rK * q.y;
     }
     op_dy2<M, 2>(a.T, lane, C, X);
+    asm volatile("" ::: "memory");
+    stage_get<M>(pk, lane, A);
 #pragma unroll
     for (int t = 0; t < M; ++t) {
       A[0][t] -= ire * (X[0][t] - rK * C[0][t]);
