@@ -5,6 +5,10 @@ counter runs:
     rocprofv3 --pmc FETCH_SIZE -- python3 -m loma_phd_amd.profile_step fp64
     rocprofv3 --pmc WRITE_SIZE -- python3 -m loma_phd_amd.profile_step fp64
 
+Optional arguments pick another grid and the passive scalar Z:
+
+    python3 -m loma_phd_amd.profile_step mixed 256 257 128 z
+
 The copy moves a known number of bytes (1 GiB read, 1 GiB written), so the
 counters it reports give the factor between a counter's KiB and HBM bytes.
 Graph replay is off so that every kernel is its own dispatch.
@@ -22,15 +26,20 @@ def main(argv=None):
     from . import capi, ic
     from .grid import CONFIG_B
     cfg = dataclasses.replace(CONFIG_B, scalar_z=False)
+    if len(argv) >= 4:
+        cfg = dataclasses.replace(cfg, mgalx=int(argv[1]), my=int(argv[2]), mgalz=int(argv[3]),
+                                  scalar_z=len(argv) >= 5 and argv[4] == "z")
     print("initial condition", flush=True)
     st = ic.initial_condition(cfg)
     p = capi.LOMA_FP64 if prec == "fp64" else capi.LOMA_MIXED
+    flags = capi.LOMA_FLAG_SCALAR_Z if cfg.scalar_z else 0
     run = capi.Run(cfg.mgalx, cfg.my, cfg.mgalz, st["y"], cfg.Re, cfg.alp, cfg.bet, cfg.pr,
-                   cfg.cfl, cfg.ncfl, cfg.dt0, p, 0)
-    run.upload(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"])
+                   cfg.cfl, cfg.ncfl, cfg.dt0, p, flags)
+    run.upload(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"],
+               z=st["z"] if cfg.scalar_z else None)
     run.use_graph(False)
     run.advance(1)
-    print(f"{prec} step done", flush=True)
+    print(f"{prec} step done, {cfg.mgalx}x{cfg.my}x{cfg.mgalz}, tiles {run.tiles()}", flush=True)
     n = 1 << 27  # doubles: 1 GiB
     a = torch.ones(n, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
