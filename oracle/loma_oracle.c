@@ -942,6 +942,7 @@ struct orc_state {
   double *rhst, *rhsz; /* physical [j][z][x] (= reference line order) */
   double *u00wk, *v00wk, *w00wk, *rf0u, *rf0w;
   int have_bounds;
+  int cfl_ranks; /* compute ranks of the reference's CFL reduction (1 = one rank) */
 };
 
 static size_t S_of(const orc_grid *g) { return (size_t)g->nplanes * g->mz * g->my; }
@@ -955,6 +956,7 @@ orc_state *orc_state_create(const orc_grid *g, double dt0, double cfl,
   st->ncfl = ncfl > 0 ? ncfl : 1;
   st->dt = (flags & ORC_DT_FP32) ? (double)(float)dt0 : dt0;
   st->cfl = cfl;
+  st->cfl_ranks = 1;
   size_t S = S_of(g), P = (size_t)g->mgalx * g->my * g->mgalz;
   st->nS = (int)S;
   st->vor = calloc(S, sizeof(cplx));
@@ -1081,6 +1083,9 @@ static void transform_pipeline(orc_state *st, cplx *const *in, cplx *const *out,
   const double hzbet = g->bet * (mgalz - 1) / M_PI;
   double cflc = 0.0, cfld = 0.0;
   double cflx_all = 0.0, cfly_all = 0.0, cflz_all = 0.0;
+  /* per-line maxima, kept only when the reduction is over several ranks */
+  const int nlines = mgalz * my, ranks = st->cfl_ranks;
+  double *lmax = (do_cfl && ranks > 1) ? calloc((size_t)4 * nlines, sizeof(double)) : NULL;
 
 #pragma omp parallel
   {
@@ -1169,8 +1174,13 @@ static void transform_pipeline(orc_state *st, cplx *const *in, cplx *const *out,
           lcx = fmax(lcx, lx);
           lcz = fmax(lcz, lz);
           lcy = fmax(lcy, l0 / g->hy[j]);
-          lct = fmax(lct, 2 * DD * ((hxalp / 2) * (hxalp / 2) + (hzbet / 2) * (hzbet / 2) +
-                                    (1 / g->hy[j]) * (1 / g->hy[j])));
+          double ldiff = 2 * DD * ((hxalp / 2) * (hxalp / 2) + (hzbet / 2) * (hzbet / 2) +
+                                   (1 / g->hy[j]) * (1 / g->hy[j]));
+          lct = fmax(lct, ldiff);
+          if (lmax) { /* line index z + mgalz*j, as jj = (j-1)/mgalz+1 at cross.f:1583 */
+            double *lm = lmax + (size_t)4 * (z + (size_t)mgalz * j);
+            lm[0] = lx; lm[1] = l0 / g->hy[j]; lm[2] = lz; lm[3] = ldiff;
+          }
         }
       }
       /* fourx(-1) with truncation to nplanes modes (rftsingle.f:64-73) */
@@ -1205,12 +1215,37 @@ static void transform_pipeline(orc_state *st, cplx *const *in, cplx *const *out,
     }
     free(zrow); free(plane); free(ph); free(xl); free(oplane);
   }
-  if (do_cfl) {
+  if (do_cfl && lmax) {
+    /* cross.f:1778-1799 over several ranks: each rank sums ITS OWN maxima,
+     * cfl0 = cflx + cfly + cflz, and MPI_ALLREDUCE(MAX) takes the largest
+     * sum; cflT is reduced on its own.  Lines are split as pointers_calc
+     * does (change.f:188-201): the first nlines % ranks ranks get one more. */
+    int n1 = nlines / ranks, n2 = nlines - ranks * n1, lb = 0;
+    cflc = 0.0; cfld = 0.0;
+    for (int r = 0; r < ranks; r++) {
+      int ln = r < n2 ? n1 + 1 : n1;
+      double cx = 0, cy = 0, cz = 0, ct = 0;
+      for (int L = lb; L < lb + ln; L++) {
+        const double *lm = lmax + (size_t)4 * L;
+        cx = fmax(cx, lm[0]); cy = fmax(cy, lm[1]); cz = fmax(cz, lm[2]); ct = fmax(ct, lm[3]);
+      }
+      cflc = fmax(cflc, cx * hxalp * 0.5 + cy + cz * hzbet * 0.5);
+      cfld = fmax(cfld, ct);
+      lb += ln;
+    }
+    cflres[0] = cflc;
+    cflres[1] = cfld;
+  } else if (do_cfl) {
     /* cross.f:1769-1817 */
     cflc = cflx_all * hxalp * 0.5 + cfly_all + cflz_all * hzbet * 0.5;
     cflres[0] = cflc;
     cflres[1] = cfld;
   }
+  free(lmax);
+}
+
+void orc_state_set_cfl_ranks(orc_state *st, int ranks) {
+  st->cfl_ranks = ranks > 1 ? ranks : 1;
 }
 
 int orc_substep(orc_state *st, int rkstep) {

@@ -94,6 +94,10 @@ void orc_state_get(const orc_state *st, double *vor, double *phi, double *psi,
 /* history: ten12/ten13 (S complex), rhst (mgalx*my*mgalz), rf0u/rf0w (my) */
 void orc_state_history(const orc_state *st, double *ten12, double *ten13,
                        double *rhst, double *rf0u, double *rf0w);
+/* Reduce the CFL maxima as the reference does over `ranks` compute ranks
+ * (cross.f:1778-1799, lines split as in change.f:188-201): the largest
+ * per-rank sum of the convective maxima.  1 (the default) is one rank. */
+void orc_state_set_cfl_ranks(orc_state *st, int ranks);
 int orc_step(orc_state *st, int nsteps);           /* returns 1 on NaN */
 int orc_substep(orc_state *st, int rkstep);        /* one RK substep */
 void orc_state_scalars(const orc_state *st, double *out /* dt,time,dtr,istep */);

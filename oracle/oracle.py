@@ -76,6 +76,7 @@ def lib():
         L.orc_substep.restype = ctypes.c_int
         L.orc_state_scalars.argtypes = [vp, _dp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_state_set_cfl_ranks.argtypes = [vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -257,6 +258,11 @@ class State:
 
     def step(self, n=1):
         return lib().orc_step(self._h, n)
+
+    def set_cfl_ranks(self, ranks):
+        """Reduce the CFL maxima over `ranks` line slabs, as the reference's
+        MPI_ALLREDUCE of per-rank sums does (cross.f:1778-1799)."""
+        lib().orc_state_set_cfl_ranks(self._h, int(ranks))
 
     def substep(self, rk):
         return lib().orc_substep(self._h, rk)
