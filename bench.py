@@ -132,9 +132,30 @@ def roofline_of(res, traffic=None):
                      avg_launch_ms=round(res["kernels_ms"][dom], 4))
 
 
+def reference_timing_here():
+    """The reference solver and the oracle timed on one machine (the build
+    container, where the reference exists): per-step seconds of each at
+    192x453x96 (the reference's compile-time grid), stored beside the
+    reference run's outputs by tests/golden/make_ref_cross1.py."""
+    path = os.path.join(HERE, "tests", "golden", "ref_cross1_192x453x96.npz")
+    if not os.path.exists(path):
+        return None
+    with np.load(path, allow_pickle=False) as z:
+        if "oracle_step_s" not in z.files:
+            return None
+        ref_s = float(np.median(z["ref_step_wall_s"]))
+        orc_s = float(np.median(z["oracle_step_s"]))
+        ranks = int(z["ranks"])
+        host = str(z["host"])
+    return dict(grid="192x453x96", reference_step_s=round(ref_s, 3),
+                reference_ranks=ranks, oracle_step_s=round(orc_s, 3),
+                restatement_over_reference_time=round(orc_s / ref_s, 3), host=host)
+
+
 def cpu_baseline(cfg, st, max_threads=16):
-    """FP64 oracle (C restatement) on the host cores: one RK3 substep of the
-    same workload, scaled to grid-pt*RK3-steps/s (x 1/3 of a step)."""
+    """FP64 oracle (C restatement) on the host cores: one full RK3 step of
+    the same workload, as grid-pt*RK3-steps/s, with the reference/oracle
+    timing of the build machine beside it."""
     from oracle import oracle as O
     O.build()
     threads = max(1, min(max_threads, os.cpu_count() or 1))
@@ -144,12 +165,16 @@ def cpu_baseline(cfg, st, max_threads=16):
     S.set(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"],
           z=st["z"] if cfg.scalar_z else None)
     t0 = time.perf_counter()
-    S.substep(1)
+    S.step(1)
     dt = time.perf_counter() - t0
-    return dict(value=cfg.points / (3 * dt), unit="grid-pt*RK3-steps/s", cores=threads,
-                kind="port",
-                sample=f"1 RK3 substep (1/3 step) of {cfg.mgalx}x{cfg.my}x{cfg.mgalz}, "
-                       f"FP64 C oracle, {threads} OpenMP threads, {dt:.1f} s")
+    out = dict(value=cfg.points / dt, unit="grid-pt*RK3-steps/s", cores=threads,
+               kind="port",
+               sample=f"1 RK3 step of {cfg.mgalx}x{cfg.my}x{cfg.mgalz}, "
+                      f"FP64 C oracle, {threads} OpenMP threads, {dt:.1f} s")
+    here = reference_timing_here()
+    if here:
+        out["reference_vs_restatement_on_build_host"] = here
+    return out
 
 
 def bench_ensemble(nruns, steps, warmup, seed=7, workers=8):

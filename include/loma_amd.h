@@ -183,11 +183,31 @@ int loma_lapvdvhom(loma_run *run, const float *phi, float *v, float *dvdy,
                    double rK);                                      /* laps.f:319 */
 int loma_solvepsi(loma_run *run, const float *phi, float *v, float *dvdy,
                   double rK);                                       /* laps.f:580 */
+/* FP64 1-D operators on one real column of my doubles, host pointers:
+ * the first and second derivative and the y-integral with u(1) = 0 */
+int loma_deryr(loma_run *run, const double *u, double *du);         /* cfdiff.f:615 */
+int loma_deryyr(loma_run *run, const double *u, double *du);        /* cfdiff.f:676 */
+int loma_inty8(loma_run *run, const double *du, double *u);         /* laps.f:484 */
 
 /* ---- transforms (device pointers) ------------------------------------- */
 /* fourz (fou3D.f:80) on nrows complex rows of length mgalz, FFTPACK
  * convention: iopt<0 forward scaled by 1/n with e^{-i}, iopt>=0 inverse. */
 int loma_fourz_dev(loma_run *run, double *c, int nrows, int iopt);
+/* ---- transforms with the reference's shapes (host pointers, real(4)) ---
+ * FFTPACK conventions: iopt < 0 forward e^{-i} scaled by 1/n, else inverse
+ * e^{+i} unscaled; computed on the device in FP64, stored back as float. */
+/* cft: m complex vectors of length mgalz; element j of vector q at
+ * c[q*isa + j*ise] (real part) and c[q*isa + j*ise + 1] (imaginary part) */
+int loma_cft(loma_run *run, float *c, int ise, int isa, int m, int iopt); /* cftsingle.f:2 */
+/* rft: m real vectors of length mgalx at stride isa >= mgalx + 2, with the
+ * packed coefficients c[2k] + i c[2k+1], k = 0..mgalx/2 (rftsingle.f:64-73) */
+int loma_rft(loma_run *run, float *c, int isa, int m, int iopt);  /* rftsingle.f:25 */
+/* fourz: a plane of my rows of mgalz complex numbers (cft with ise = 2,
+ * isa = 2*mgalz) */
+int loma_fourz(loma_run *run, float *phys, int iopt);             /* fou3D.f:80 */
+/* fourx: one x-line of mgalx + 2 reals; iopt >= 0 zeroes the modes past mx
+ * and writes the physical line to phys and (in FP64) to phys8 */
+int loma_fourx(loma_run *run, float *phys, double *phys8, int iopt); /* fou3D.f:29 */
 /* visc with the reference's argument list and real(4) (2,my) columns
  * (trunk/mixing/src/laps.f:535; the trailing integer flag is unused) */
 int loma_visc(loma_run *run, float *vorwk, float *vor, float *phi, float *v,

@@ -27,10 +27,13 @@ EXPORTS = [
     "loma_cross1", "loma_set_stream", "loma_use_graph", "loma_deryr2_dev",
     "loma_deryyr2_dev", "loma_lapvdvhom_dev", "loma_solvepsi_dev", "loma_helmholtz_dev",
     "loma_inty8_dev", "loma_deryr2", "loma_deryyr2", "loma_lapvdvhom", "loma_solvepsi",
-    "loma_fourz_dev", "loma_band5_solve_dev", "loma_advance_many", "loma_advance_async",
+    "loma_fourz_dev", "loma_band5_solve_dev",
+    "loma_advance_many", "loma_advance_async",
     "loma_tiles", "loma_footprint", "loma_step_begin", "loma_step_poll",
     "loma_visc_dev", "loma_visc", "loma_set_physics", "loma_seed", "loma_thickness",
     "loma_sync", "loma_stream", "loma_profile", "loma_profile_read",
+    "loma_deryr", "loma_deryyr", "loma_inty8", "loma_cft", "loma_rft", "loma_fourz",
+    "loma_fourx",
 ]
 KERNEL_CLASSES = ["phaseA", "zfft_inv", "xline", "zfft_fwd", "phaseC", "finish", "phaseC2"]
 

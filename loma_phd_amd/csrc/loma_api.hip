@@ -72,6 +72,7 @@ struct loma_run {
   double* dt22 = nullptr;
   double* helm_lu = nullptr;
   double* helm_bc = nullptr;  // per class Neumann constants
+  double* helm_h = nullptr;   // per class homogeneous solutions h1, h2 (lane-major)
   int nclass = 0;
   double2 *vor = nullptr, *phi = nullptr, *psi = nullptr, *scal = nullptr;
   double2 *ten12 = nullptr, *ten13 = nullptr, *zf = nullptr;
@@ -166,6 +167,7 @@ static size_t plan_bytes(int mgalx, int my, int mgalz, bool fp32, bool with_z) {
   size_t b = align256(8 * (size_t)ytab_offset(YT_COUNT, M));  // per-row and scan tables
   b += 2 * align256(8 * 5 * (size_t)my);                       // band copies
   b += align256(8 * nclass) + align256(8 * 5 * R * nclass) + align256(32 * nclass);
+  b += align256(8 * 2 * R * nclass);                           // homogeneous solutions
   b += nst * align256(S * 16);                                 // state and histories
   b += nsp * (align256(Sp * es) + 2 * align256(Wp * es));      // transform buffers
   b += (with_z ? 2 : 1) * align256(P * 8);                     // physical RK histories
@@ -239,6 +241,7 @@ struct EnsembleGraph {
   std::vector<int> icfl;
   hipStream_t master = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<hipEvent_t> ready;  // per run: work already queued on its stream
 };
 static std::mutex g_ens_mu;
 static std::vector<EnsembleGraph> g_ens;
@@ -251,6 +254,7 @@ static void ensemble_forget(loma_run* r) {
     if (has) {
       if (g_ens[q].exec) hipGraphExecDestroy(g_ens[q].exec);
       if (g_ens[q].master) hipStreamDestroy(g_ens[q].master);
+      for (hipEvent_t e : g_ens[q].ready) hipEventDestroy(e);
       g_ens.erase(g_ens.begin() + q);
     } else {
       ++q;
@@ -436,8 +440,11 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
     if ((rc = dalloc(r, &r->helm_bc, sizeof(double) * 4 * (size_t)r->nclass))) {
       loma_free(r); return rc;
     }
+    if ((rc = dalloc(r, &r->helm_h, sizeof(double) * 2 * R * (size_t)r->nclass))) {
+      loma_free(r); return rc;
+    }
     if (launch_helm_factor(r->dt21, r->dt22, my, M, dsh, r->nclass, r->helm_lu, sst) ||
-        launch_helm_bc(M, r->T, r->helm_lu, r->helm_bc, r->nclass, sst) ||
+        launch_helm_bc(M, r->T, r->helm_lu, r->helm_bc, r->helm_h, r->nclass, sst) ||
         hipStreamSynchronize(sst) != hipSuccess) {
       set_error("loma_init: factor launch failed"); loma_free(r); return LOMA_ERR_DEVICE;
     }
@@ -653,8 +660,11 @@ static int upload_tail(loma_run* r, const double* u00, const double* v00, const 
   r->inflight = false;
   r->istep = 0;
   r->uploaded = true;
+  // the step graphs and every ensemble graph with this run hold its
+  // constants (boundary densities and scalar values) by value
   for (int q = 0; q < 2; ++q)
     if (r->gexec[q]) { hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
+  ensemble_forget(r);
   return LOMA_OK;
 }
 
@@ -687,6 +697,11 @@ int loma_upload(loma_run* r, const void* vor, const void* phi, const void* psi,
   double2* dst[4] = {r->vor, r->phi, r->psi, r->scal};
   for (int q = 0; q < 4; ++q)
     if ((e = put_field(r, src[q], fp64, S, dst[q]))) return e;
+  if (r->zf) {
+    // Z starts at zero until loma_upload_z sets it, never stale arena memory
+    HIPCHK(hipMemsetAsync(r->zf, 0, S * 16, r->stream));
+    r->rc.zbot = r->rc.ztop = 0.0;
+  }
   return upload_tail(r, u00, v00, w00);
 }
 
@@ -779,6 +794,7 @@ int loma_upload_z(loma_run* r, const void* z, int fp64) {
   if ((e = put_field(r, z, fp64, S, r->zf))) return e;
   for (int q = 0; q < 2; ++q)
     if (r->gexec[q]) { hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
+  ensemble_forget(r);
   return LOMA_OK;
 }
 
@@ -823,6 +839,7 @@ static PhaseArgs phase_args(loma_run* r, int rkstep) {
   a.T = r->T;
   a.helm_lu = r->helm_lu;
   a.helm_bc = r->helm_bc;
+  a.helm_h = r->helm_h;
   a.lu_stride = r->lu_stride;
   a.vor = r->vor; a.phi = r->phi; a.psi = r->psi; a.scal = r->scal;
   a.ten12 = r->ten12; a.ten13 = r->ten13; a.zf = r->zf;
@@ -1149,6 +1166,7 @@ int loma_advance_many(loma_run** runs, int nruns, int nsteps) {
   for (int q = 0; q < nruns; ++q) {
     loma_run* r = runs[q];
     if (!r || !r->uploaded) { set_error("loma_advance_many: run not uploaded"); return LOMA_ERR_STATE; }
+    if (r->inflight) { set_error("loma_advance_many: a run has a step in flight"); return LOMA_ERR_STATE; }
     for (int p = 0; p < q; ++p)
       if (runs[p] == r) { set_error("loma_advance_many: run listed twice"); return LOMA_ERR_ARG; }
     int e = ensure_stream(r);  // every run needs a stream of its own
@@ -1199,8 +1217,16 @@ int loma_advance_many(loma_run** runs, int nruns, int nsteps) {
         hipError_t ie = hipGraphInstantiate(&c.exec, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         HIPCHK(ie);
+        c.ready.resize(nruns, nullptr);
+        for (auto& e : c.ready) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         g_ens.push_back(c);
         eg = &g_ens.back();
+      }
+      // the graph runs on its own stream: order it after whatever is already
+      // queued on each member run's stream (an earlier asynchronous advance)
+      for (int q = 0; q < nruns; ++q) {
+        HIPCHK(hipEventRecord(eg->ready[q], runs[q]->stream));
+        HIPCHK(hipStreamWaitEvent(eg->master, eg->ready[q], 0));
       }
       HIPCHK(hipGraphLaunch(eg->exec, eg->master));
     }
@@ -1328,7 +1354,7 @@ static int solve_cols(loma_run* r, const double* phi, double* v, double* dvdy,
   int rc = LOMA_OK;
   if (hipMemcpy(sh, h.data(), sizeof(double) * ncols, hipMemcpyHostToDevice) != hipSuccess ||
       launch_helm_factor(r->dt21, r->dt22, r->my, r->M, sh, ncols, lu, r->stream) ||
-      launch_helm_bc(r->M, r->T, lu, bc, ncols, r->stream) ||
+      launch_helm_bc(r->M, r->T, lu, bc, nullptr, ncols, r->stream) ||
       launch_lapvdvhom(r->M, r->T, lu, bc, phi, v, dvdy, rK, ncols, mode, r->stream) ||
       hipStreamSynchronize(r->stream) != hipSuccess) {
     set_error("solve_cols: device failure");
@@ -1369,7 +1395,7 @@ int loma_visc_dev(loma_run* r, double* vorwk, double* vor, double* phi, double* 
   int rc = LOMA_OK;
   if (launch_helm_factor(r->dt21, r->dt22, r->my, r->M, rk1, ncols, lu, r->stream) ||
       launch_helm_factor(r->dt21, r->dt22, r->my, r->M, rk2, ncols, lu + L, r->stream) ||
-      launch_helm_bc(r->M, r->T, lu + L, bc, ncols, r->stream) ||
+      launch_helm_bc(r->M, r->T, lu + L, bc, nullptr, ncols, r->stream) ||
       launch_visc(r->M, r->T, lu, lu + L, bc, vorwk, vor, phi, v, dvdy, phiwk, rk2, dalbe,
                   dtri, dtxi, ncols, r->stream) ||
       hipStreamSynchronize(r->stream) != hipSuccess) {
@@ -1462,6 +1488,178 @@ int loma_visc(loma_run* r, float* vorwk, float* vor, float* phi, float* v, float
   return rc;
 }
 
+// ---- host-pointer transforms and 1-D operators with the reference's shapes
+// One complex FFT of `nrows` contiguous rows of length n on the device
+// (FFTPACK conventions, as k_fourz: iopt < 0 forward e^{-i} scaled by 1/n,
+// else inverse e^{+i} unscaled), in and out of a host array of doubles.
+static int host_cfft(loma_run* r, int n, std::vector<double>& rows, int nrows, int iopt) {
+  if (nrows <= 0) return LOMA_OK;
+  const FftPlan pl = make_plan(n);
+  if (pl.npass < 0) { set_error("transform length has no plan"); return LOMA_ERR_ARG; }
+  std::vector<double> tw(2 * (size_t)n);
+  for (int t = 0; t < n; ++t) {
+    tw[2 * t] = std::cos(2.0 * M_PI * t / n);
+    tw[2 * t + 1] = -std::sin(2.0 * M_PI * t / n);
+  }
+  double* d = nullptr;
+  const size_t nb = rows.size() * sizeof(double), tb = tw.size() * sizeof(double);
+  HIPCHK(hipMalloc(&d, nb + tb));
+  double* dtw = d + rows.size();
+  int rc = LOMA_OK;
+  if (hipMemcpy(d, rows.data(), nb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dtw, tw.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
+      launch_fourz(pl, (const double2*)dtw, (double2*)d, nrows, iopt, r->stream) ||
+      hipStreamSynchronize(r->stream) != hipSuccess ||
+      hipMemcpy(rows.data(), d, nb, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("host transform: device failure");
+    rc = LOMA_ERR_DEVICE;
+  }
+  hipFree(d);
+  return rc;
+}
+
+// cft (cftsingle.f:2): m complex vectors of length mgalz, element j of
+// vector q at c[q*isa + j*ise] (re) and the next float (im)
+int loma_cft(loma_run* r, float* c, int ise, int isa, int m, int iopt) {
+  if (!r || !c || m < 0 || ise < 2 || isa < 0) return LOMA_ERR_ARG;
+  const int n = r->mgalz;
+  std::vector<double> h(2 * (size_t)n * m);
+  for (int q = 0; q < m; ++q)
+    for (int j = 0; j < n; ++j) {
+      const size_t o = (size_t)q * isa + (size_t)j * ise;
+      h[2 * ((size_t)q * n + j)] = c[o];
+      h[2 * ((size_t)q * n + j) + 1] = c[o + 1];
+    }
+  int e = host_cfft(r, n, h, m, iopt);
+  if (e) return e;
+  for (int q = 0; q < m; ++q)
+    for (int j = 0; j < n; ++j) {
+      const size_t o = (size_t)q * isa + (size_t)j * ise;
+      c[o] = (float)h[2 * ((size_t)q * n + j)];
+      c[o + 1] = (float)h[2 * ((size_t)q * n + j) + 1];
+    }
+  return LOMA_OK;
+}
+
+// fourz (fou3D.f:80): a real(4) plane of my rows of mgalz complex numbers
+int loma_fourz(loma_run* r, float* phys, int iopt) {
+  if (!r) return LOMA_ERR_ARG;
+  return loma_cft(r, phys, 2, 2 * r->mgalz, r->my, iopt);
+}
+
+// rft (rftsingle.f:25): m real vectors of n = mgalx points, vector q at
+// c[q*isa], isa >= n + 2.  Forward: the packed coefficients
+// c[2k] + i c[2k+1] = (1/n) sum_j x_j e^{-2 pi i jk/n}, k = 0..n/2, with the
+// imaginary parts of k = 0 and n/2 set to zero (rftsingle.f:64-73).
+// Inverse: x_j = Re c0 + 2 sum_{0<k<n/2} Re(ck e^{2 pi i jk/n}) + Re c_{n/2}
+// (-1)^j, imaginary parts of k = 0, n/2 ignored; the two entries past the n
+// physical points keep their values, as the reference's shift leaves them.
+int loma_rft(loma_run* r, float* c, int isa, int m, int iopt) {
+  const int n = r ? r->mgalx : 0;
+  if (!r || !c || m < 0 || isa < n + 2) return LOMA_ERR_ARG;
+  std::vector<double> h(2 * (size_t)n * m);
+  for (int q = 0; q < m; ++q) {
+    const float* v = c + (size_t)q * isa;
+    double* row = h.data() + 2 * (size_t)q * n;
+    if (iopt < 0) {
+      for (int j = 0; j < n; ++j) { row[2 * j] = v[j]; row[2 * j + 1] = 0.0; }
+    } else {
+      for (int k = 0; k < n; ++k) { row[2 * k] = 0.0; row[2 * k + 1] = 0.0; }
+      row[0] = v[0];
+      for (int k = 1; k < n / 2; ++k) {
+        row[2 * k] = v[2 * k];
+        row[2 * k + 1] = v[2 * k + 1];
+        row[2 * (n - k)] = v[2 * k];
+        row[2 * (n - k) + 1] = -v[2 * k + 1];
+      }
+      row[2 * (n / 2)] = v[n];
+    }
+  }
+  int e = host_cfft(r, n, h, m, iopt);
+  if (e) return e;
+  for (int q = 0; q < m; ++q) {
+    float* v = c + (size_t)q * isa;
+    const double* row = h.data() + 2 * (size_t)q * n;
+    if (iopt < 0) {
+      for (int k = 0; k <= n / 2; ++k) {
+        v[2 * k] = (float)row[2 * k];
+        v[2 * k + 1] = (float)row[2 * k + 1];
+      }
+      v[1] = 0.0f;
+      v[n + 1] = 0.0f;
+    } else {
+      for (int j = 0; j < n; ++j) v[j] = (float)row[2 * j];
+    }
+  }
+  return LOMA_OK;
+}
+
+// fourx (fou3D.f:29): one x-line of mgalx + 2 reals.  iopt >= 0: modes past
+// mx are zeroed, the line goes to physical space, and phys8 receives the
+// physical values (computed in FP64, so phys8 is not the rounded phys).
+// iopt < 0: forward transform of phys in place (phys8 unused).
+int loma_fourx(loma_run* r, float* phys, double* phys8, int iopt) {
+  if (!r || !phys || (iopt >= 0 && !phys8)) return LOMA_ERR_ARG;
+  const int n = r->mgalx;
+  if (iopt < 0) return loma_rft(r, phys, n + 2, 1, -1);
+  for (int j = r->mx; j < n + 2; ++j) phys[j] = 0.0f;
+  std::vector<double> h(2 * (size_t)n, 0.0);
+  h[0] = phys[0];
+  for (int k = 1; k < n / 2; ++k) {
+    h[2 * k] = phys[2 * k];
+    h[2 * k + 1] = phys[2 * k + 1];
+    h[2 * (n - k)] = phys[2 * k];
+    h[2 * (n - k) + 1] = -phys[2 * k + 1];
+  }
+  h[2 * (n / 2)] = phys[n];
+  int e = host_cfft(r, n, h, 1, 1);
+  if (e) return e;
+  for (int j = 0; j < n; ++j) {
+    phys8[j] = h[2 * j];
+    phys[j] = (float)h[2 * j];
+  }
+  phys8[n] = phys[n];
+  phys8[n + 1] = phys[n + 1];
+  return LOMA_OK;
+}
+
+// FP64 1-D y operators on one real column of my doubles: deryr
+// (cfdiff.f:615), deryyr (cfdiff.f:676), inty8 (laps.f:484)
+static int host_realop(loma_run* r, const double* u, double* out, int op) {
+  if (!r || !u || !out) return LOMA_ERR_ARG;
+  const int my = r->my;
+  std::vector<double> h(2 * (size_t)my, 0.0);
+  if (op == 3) {
+    for (int j = 0; j < my; ++j) h[j] = u[j];
+  } else {
+    for (int j = 0; j < my; ++j) h[2 * j] = u[j];
+  }
+  double* d = nullptr;
+  HIPCHK(hipMalloc(&d, 2 * h.size() * sizeof(double)));
+  double* o = d + h.size();
+  int rc = LOMA_OK;
+  if (hipMemcpy(d, h.data(), h.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      (op == 3 ? launch_inty8(r->M, r->T, d, o, r->stream)
+               : launch_dy(r->M, op, r->T, d, o, 1, r->stream)) ||
+      hipStreamSynchronize(r->stream) != hipSuccess ||
+      hipMemcpy(h.data(), o, h.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("host 1-D operator: device failure");
+    rc = LOMA_ERR_DEVICE;
+  }
+  hipFree(d);
+  if (rc == LOMA_OK) {
+    if (op == 3) {
+      for (int j = 0; j < my; ++j) out[j] = h[j];
+    } else {
+      for (int j = 0; j < my; ++j) out[j] = h[2 * j];
+    }
+  }
+  return rc;
+}
+int loma_deryr(loma_run* r, const double* u, double* du) { return host_realop(r, u, du, 1); }
+int loma_deryyr(loma_run* r, const double* u, double* du) { return host_realop(r, u, du, 2); }
+int loma_inty8(loma_run* r, const double* du, double* u) { return host_realop(r, du, u, 3); }
+
 int loma_fourz_dev(loma_run* r, double* c, int nrows, int iopt) {
   if (!r || !c || nrows < 0) return LOMA_ERR_ARG;
   LCHK(launch_fourz(r->plz, r->twz64, (double2*)c, nrows, iopt, r->stream));
@@ -1469,7 +1667,10 @@ int loma_fourz_dev(loma_run* r, double* c, int nrows, int iopt) {
   return LOMA_OK;
 }
 
-// scratch of the batched banded solve, kept per device and grown on demand
+// scratch of the batched banded solve, kept per device and grown on demand.
+// One area per device serves every call on it, so a call holds the lock from
+// the size check until its solve has finished: concurrent calls on a device
+// run one after the other instead of sharing (or freeing) each other's area.
 static std::mutex g_band_mu;
 static std::vector<std::pair<void*, size_t>> g_band_scr;
 
@@ -1483,26 +1684,22 @@ int loma_band5_solve_dev(const double* band, void* rhs, double* x, int n, int ns
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   const size_t need = (size_t)n * 5 * nsys * sizeof(double);
-  double* scr = nullptr;
-  {
-    std::lock_guard<std::mutex> lk(g_band_mu);
-    if ((int)g_band_scr.size() <= dev) g_band_scr.resize(dev + 1, {nullptr, 0});
-    auto& e = g_band_scr[dev];
-    if (e.second < need) {
-      if (e.first) hipFree(e.first);
-      e.first = nullptr;
-      e.second = 0;
-      if (hipMalloc(&e.first, need) != hipSuccess) {
-        set_error("loma_band5_solve_dev: scratch allocation failed");
-        return LOMA_ERR_OOM;
-      }
-      e.second = need;
+  std::lock_guard<std::mutex> lk(g_band_mu);
+  if ((int)g_band_scr.size() <= dev) g_band_scr.resize(dev + 1, {nullptr, 0});
+  auto& e = g_band_scr[dev];
+  if (e.second < need) {
+    if (e.first) hipFree(e.first);
+    e.first = nullptr;
+    e.second = 0;
+    if (hipMalloc(&e.first, need) != hipSuccess) {
+      set_error("loma_band5_solve_dev: scratch allocation failed");
+      return LOMA_ERR_OOM;
     }
-    scr = (double*)e.first;
+    e.second = need;
   }
   double* out = x ? x : (double*)rhs;
   hipStream_t st = (hipStream_t)stream;
-  LCHK(launch_band5(band, rhs, out, scr, n, nsys, rhs_fp32, st));
+  LCHK(launch_band5(band, rhs, out, (double*)e.first, n, nsys, rhs_fp32, st));
   HIPCHK(hipStreamSynchronize(st));
   return LOMA_OK;
 }

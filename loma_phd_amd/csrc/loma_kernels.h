@@ -23,6 +23,8 @@ struct DevScalars {
   int pad;
   double dm;      // momentum thickness of the state at the start of the last step
   double dw;      // vorticity thickness at the start of its third substep
+  double trapz;   // integral of drho00 over y at the last substep (cross.f:1034-1036)
+  double mass[3]; // y-integrals of the 00 modes of rho u, rho v, rho w (cross.f:1013-1019)
 };
 
 // per-run physics constants passed by value to kernels
@@ -50,6 +52,7 @@ struct PhaseArgs {
   YTab T;
   const double* helm_lu;     // per (i,|kz|) class LU tables (lane-major)
   const double* helm_bc;     // per class Neumann constants (4 doubles)
+  const double* helm_h;      // per class h1, h2 (lane-major, 2 entries per row)
   int lu_stride;             // doubles per class LU table (5 * 64 * M)
   double2 *vor, *phi, *psi, *scal, *ten12, *ten13, *zf;
   void* spec[12];            // S-sized transform buffers (float2 or double2)
@@ -118,7 +121,10 @@ int launch_dy(int M, int order, const YTab& T, const double* u, double* du, int 
               hipStream_t st);
 int launch_helm_factor(const double* dt21, const double* dt22, int my, int M,
                        const double* shifts, int nsys, double* lu, hipStream_t st);
-int launch_helm_bc(int M, const YTab& T, const double* lu, double* bc, int nclass,
+// per-class Neumann constants (bc, 4 per class) and, when h is not null,
+// the homogeneous solutions h1 = LU^-1 e_0, h2 = LU^-1 e_{my-1} of each
+// class (lane-major, 2 entries per row, 2*64*M doubles per class)
+int launch_helm_bc(int M, const YTab& T, const double* lu, double* bc, double* h, int nclass,
                    hipStream_t st);
 int launch_lapvdvhom(int M, const YTab& T, const double* lu, const double* bc,
                      const double* phi, double* v, double* dv, const double* rK,

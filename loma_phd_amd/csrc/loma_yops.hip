@@ -13,7 +13,7 @@ template <int M>
 int launch_lapvdvhom_m(const YTab&, const double*, int, const double*, const double*,
                        double*, double*, const double*, int, int, hipStream_t);
 template <int M>
-int launch_helm_bc_m(const YTab&, const double*, int, double*, int, hipStream_t);
+int launch_helm_bc_m(const YTab&, const double*, int, double*, double*, int, hipStream_t);
 template <int M>
 int launch_inty8_m(const YTab&, const double*, double*, hipStream_t);
 template <int M>
@@ -70,10 +70,10 @@ int launch_lapvdvhom(int M, const YTab& T, const double* lu, const double* bc,
   LOMA_DISPATCH_M(M, launch_lapvdvhom_m<MM>(T, lu, 5 * WAVE * MM, bc, phi, v, dv, rK, ncols,
                                             mode, st));
 }
-int launch_helm_bc(int M, const YTab& T, const double* lu, double* bc, int nclass,
+int launch_helm_bc(int M, const YTab& T, const double* lu, double* bc, double* h, int nclass,
                    hipStream_t st) {
   if (nclass <= 0) return 0;
-  LOMA_DISPATCH_M(M, launch_helm_bc_m<MM>(T, lu, 5 * WAVE * MM, bc, nclass, st));
+  LOMA_DISPATCH_M(M, launch_helm_bc_m<MM>(T, lu, 5 * WAVE * MM, bc, h, nclass, st));
 }
 int launch_inty8(int M, const YTab& T, const double* du, double* u, hipStream_t st) {
   LOMA_DISPATCH_M(M, launch_inty8_m<MM>(T, du, u, st));

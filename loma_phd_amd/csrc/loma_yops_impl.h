@@ -101,7 +101,7 @@ __device__ __forceinline__ void solve_neumann(const YTab& T, const double* lu,
 // bc = {w0.h1, w0.h2, w1.h1, w1.h2}
 template <int M>
 __global__ __launch_bounds__(256) void k_helm_bc(YTab T, const double* lu, int lu_stride,
-                                                 double* bc, int nclass) {
+                                                 double* bc, double* hout, int nclass) {
   const int wave = wave_uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (wave >= nclass) return;
@@ -115,6 +115,15 @@ __global__ __launch_bounds__(256) void k_helm_bc(YTab T, const double* lu, int l
   }
   lsolve<M, 2, 2>(tab(L, 5, 3, lane), lane, h);
   usolve<M, 2, 2>(tab(L, 5, 1, lane), tab(L, 5, 0, lane), lane, h);
+  if (hout) {
+    // h1, h2 lane-major (entry 0: h1, entry 1: h2 of row lane*M + t)
+    double* H = hout + (size_t)wave * 2 * WAVE * M;
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+      H[(t * 2 + 0) * WAVE + lane] = h[0][t];
+      H[(t * 2 + 1) * WAVE + lane] = h[1][t];
+    }
+  }
   double d[4];
   bd_dots<M>(T, lane, h, d);  // {w0.h1, w0.h2, w1.h1, w1.h2}
   if (lane == 0) {
@@ -791,46 +800,116 @@ __global__ __launch_bounds__(256, 2) void k_phaseC(PhaseArgs a) {
 // (8) psi from the low-Mach constraint and the 00 mode of rho*v
 // (cross.f:1027-1104).  A kernel of its own: it reads only drho and writes
 // psi, so splitting it from the rest of phase C adds no HBM traffic.
+// One wavefront per Helmholtz class (kx, |kz|): the +kz and -kz modes of a
+// class share its LU and its homogeneous solutions h1, h2, so the wavefront
+// solves both at once (four real columns) and reads the class tables once
+// for two modes.  The Neumann combination is v + c1 h1 + c2 h2 with h1, h2
+// from the class table (the second Dirichlet solve of solvepsi, laps.f:
+// 700-726, by linearity).  The class with kx = kz = 0 is the 00 mode alone.
+template <int M>
+__device__ __forceinline__ void bd_dots4(const YTab& T, int lane, const double (&x)[4][M],
+                                         double (&out)[8]) {
+  const TabRef w = ytab<M>(T, YT_WBD, 2, lane);
+  double s[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) s[q] = 0.0;
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    const double a = tget(w, t, 0), b = tget(w, t, 1);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      s[n] += a * x[n][t];
+      s[4 + n] += b * x[n][t];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) out[q] = wave_sum(s[q]);
+}
+
 template <int M, typename TS>
 __global__ __launch_bounds__(256) void k_phaseC2(PhaseArgs a) {
   using T2 = typename std::conditional<sizeof(TS) == 4, float2, double2>::type;
   LOMA_STAGE(M);
   const RunConst& rc = a.rc;
-  const int seq = wave_uniform(a.mode_base + blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int cls = wave_uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
-  if (seq >= a.mode_base + a.nmodes) return;
-  const int w = mode_seq(rc, seq);
-  int i, k;
-  double kx, kz;
-  mode_of(rc, w, i, k, kx, kz);
+  const int nka = rc.nz + 1;
+  if (cls >= rc.nplanes * nka) return;
+  const int i = cls / nka, ka = cls - i * nka;
+  const int k1 = ka, k2 = rc.mz - ka;  // k2 only for ka > 0
   const int my = rc.my, r0 = lane * M;
-  const double rK = kx * kx + kz * kz;
-  const size_t o = (size_t)w * my;
-  const bool m00 = (i == 0 && k == 0);
-  double dr[2][M], ps[2][M];
-  load_col<M>((const T2*)a.spec[9] + (size_t)w * rc.myp, my, lane, st, dr);
-  add_mean<M>(dr, lane, my, m00, a.mean_out + 9 * (size_t)my);
-  if (m00) {
-    // rho v 00 mode = integral of drho00 + acoustic split (cross.f:1027-1072)
-    double x1[1][M];
+  const int w1 = i * rc.mz + k1, w2 = i * rc.mz + k2;
+  double x[4][M];
+  {
+    double d[2][M];
+    load_col<M>((const T2*)a.spec[9] + (size_t)w1 * rc.myp, my, lane, st, d);
 #pragma unroll
-    for (int t = 0; t < M; ++t) x1[0][t] = dr[0][t];
-    op_inty<M>(a.T, lane, x1);
-    const double trapz = -col_value<M>(x1[0], my - 1);
-    const double add = trapz / (1.0 + sqrt(rc.rhotop / rc.rhobot));
-#pragma unroll
-    for (int t = 0; t < M; ++t)
-      if (r0 + t < my) a.v00wk[r0 + t] = x1[0][t] + add;
+    for (int t = 0; t < M; ++t) { x[0][t] = d[0][t]; x[1][t] = d[1][t]; }
   }
-  if (rK == 0.0) {
+  if (ka == 0) {
+    if (i == 0) {
+      // 00 mode: rho v 00 = integral of drho00 + acoustic split
+      // (cross.f:1027-1072); psi 00 = 0
+      double d[2][M];
 #pragma unroll
-    for (int t = 0; t < M; ++t) ps[0][t] = ps[1][t] = 0.0;
+      for (int t = 0; t < M; ++t) { d[0][t] = x[0][t]; d[1][t] = x[1][t]; }
+      add_mean<M>(d, lane, my, true, a.mean_out + 9 * (size_t)my);
+      double x1[1][M];
+#pragma unroll
+      for (int t = 0; t < M; ++t) x1[0][t] = d[0][t];
+      op_inty<M>(a.T, lane, x1);
+      const double trapz = -col_value<M>(x1[0], my - 1);
+      const double add = trapz / (1.0 + sqrt(rc.rhotop / rc.rhobot));
+#pragma unroll
+      for (int t = 0; t < M; ++t)
+        if (r0 + t < my) a.v00wk[r0 + t] = x1[0][t] + add;
+      if (lane == 0) a.sc->trapz = trapz;
+      double z[2][M];
+#pragma unroll
+      for (int t = 0; t < M; ++t) z[0][t] = z[1][t] = 0.0;
+      store_col<M>(a.psi + (size_t)w1 * my, my, lane, st, z);
+      return;
+    }
+#pragma unroll
+    for (int t = 0; t < M; ++t) x[2][t] = x[3][t] = 0.0;
   } else {
-    const int cls = helm_class(rc, i, k);
-    solve_neumann<M>(a.T, a.helm_lu + (size_t)cls * a.lu_stride, a.helm_bc + (size_t)cls * 4,
-                     lane, dr, ps);
+    double d[2][M];
+    wave_sync();  // the stage is reused
+    load_col<M>((const T2*)a.spec[9] + (size_t)w2 * rc.myp, my, lane, st, d);
+#pragma unroll
+    for (int t = 0; t < M; ++t) { x[2][t] = d[0][t]; x[3][t] = d[1][t]; }
   }
-  store_col<M>(a.psi + o, my, lane, st, ps);
+  double v[4][M];
+  op_helm<M, 4>(a.T, a.helm_lu + (size_t)cls * a.lu_stride, lane, x, v);
+  double dp[8];
+  bd_dots4<M>(a.T, lane, v, dp);
+  const double* bc = a.helm_bc + (size_t)cls * 4;
+  const double a11 = bc[0], a12 = bc[1], a21 = bc[2], a22 = bc[3];
+  const double det = a11 * a22 - a12 * a21;
+  double c1[4], c2[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    c1[n] = (-dp[n] * a22 + a12 * dp[4 + n]) / det;
+    c2[n] = (-a11 * dp[4 + n] + a21 * dp[n]) / det;
+  }
+  const TabRef H = tab(a.helm_h + (size_t)cls * 2 * WAVE * M, 2, 0, lane);
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    const double h1 = tget(H, t, 0), h2 = tget(H, t, 1);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) v[n][t] += c1[n] * h1 + c2[n] * h2;
+  }
+  {
+    double o[2][M];
+#pragma unroll
+    for (int t = 0; t < M; ++t) { o[0][t] = v[0][t]; o[1][t] = v[1][t]; }
+    store_col<M>(a.psi + (size_t)w1 * my, my, lane, st, o);
+    if (ka > 0) {
+#pragma unroll
+      for (int t = 0; t < M; ++t) { o[0][t] = v[2][t]; o[1][t] = v[3][t]; }
+      store_col<M>(a.psi + (size_t)w2 * my, my, lane, st, o);
+    }
+  }
 }
 
 // Seeded mixing-layer initial condition on the device, one wavefront per
@@ -1087,10 +1166,10 @@ int launch_visc_m(const YTab& T, const double* lu1, const double* lu2, int lu_st
   return (int)hipGetLastError();
 }
 template <int M>
-int launch_helm_bc_m(const YTab& T, const double* lu, int lu_stride, double* bc, int nclass,
-                     hipStream_t st) {
+int launch_helm_bc_m(const YTab& T, const double* lu, int lu_stride, double* bc, double* h,
+                     int nclass, hipStream_t st) {
   dim3 g((nclass + 3) / 4), b(256);
-  hipLaunchKernelGGL((k_helm_bc<M>), g, b, 0, st, T, lu, lu_stride, bc, nclass);
+  hipLaunchKernelGGL((k_helm_bc<M>), g, b, 0, st, T, lu, lu_stride, bc, h, nclass);
   return (int)hipGetLastError();
 }
 template <int M>
@@ -1108,8 +1187,10 @@ int launch_phase_m(int which, bool fp32, const PhaseArgs& a, hipStream_t st) {
     if (fp32) hipLaunchKernelGGL((k_phaseC<M, float>), g, b, 0, st, a);
     else hipLaunchKernelGGL((k_phaseC<M, double>), g, b, 0, st, a);
   } else {
-    if (fp32) hipLaunchKernelGGL((k_phaseC2<M, float>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_phaseC2<M, double>), g, b, 0, st, a);
+    // one wavefront per Helmholtz class (the +-kz pair)
+    dim3 gc((a.rc.nplanes * (a.rc.nz + 1) + 3) / 4);
+    if (fp32) hipLaunchKernelGGL((k_phaseC2<M, float>), gc, b, 0, st, a);
+    else hipLaunchKernelGGL((k_phaseC2<M, double>), gc, b, 0, st, a);
   }
   return (int)hipGetLastError();
 }
@@ -1133,8 +1214,8 @@ int launch_ic_m(const IcArgs& a, hipStream_t st) {
   template int launch_lapvdvhom_m<MV>(const YTab&, const double*, int, const double*,     \
                                       const double*, double*, double*, const double*, int, \
                                       int, hipStream_t);                                  \
-  template int launch_helm_bc_m<MV>(const YTab&, const double*, int, double*, int,        \
-                                    hipStream_t);                                         \
+  template int launch_helm_bc_m<MV>(const YTab&, const double*, int, double*, double*,    \
+                                    int, hipStream_t);                                    \
   template int launch_inty8_m<MV>(const YTab&, const double*, double*, hipStream_t);      \
   template int launch_visc_m<MV>(const YTab&, const double*, const double*, int,          \
                                  const double*, double*, double*, double*, double*,       \

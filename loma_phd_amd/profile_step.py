@@ -29,14 +29,12 @@ def main(argv=None):
     if len(argv) >= 4:
         cfg = dataclasses.replace(cfg, mgalx=int(argv[1]), my=int(argv[2]), mgalz=int(argv[3]),
                                   scalar_z=len(argv) >= 5 and argv[4] == "z")
-    print("initial condition", flush=True)
-    st = ic.initial_condition(cfg)
     p = capi.LOMA_FP64 if prec == "fp64" else capi.LOMA_MIXED
     flags = capi.LOMA_FLAG_SCALAR_Z if cfg.scalar_z else 0
-    run = capi.Run(cfg.mgalx, cfg.my, cfg.mgalz, st["y"], cfg.Re, cfg.alp, cfg.bet, cfg.pr,
+    run = capi.Run(cfg.mgalx, cfg.my, cfg.mgalz, cfg.y(), cfg.Re, cfg.alp, cfg.bet, cfg.pr,
                    cfg.cfl, cfg.ncfl, cfg.dt0, p, flags)
-    run.upload(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"],
-               z=st["z"] if cfg.scalar_z else None)
+    print("initial condition (built on the device)", flush=True)
+    ic.seed_run(run, cfg)
     run.use_graph(False)
     run.advance(1)
     print(f"{prec} step done, {cfg.mgalx}x{cfg.my}x{cfg.mgalz}, tiles {run.tiles()}", flush=True)

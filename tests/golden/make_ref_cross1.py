@@ -22,7 +22,9 @@ The images written after 1, 3 and 10 steps are reduced to a small fixture:
   * the same column norms of the FP32 initial condition, so that a test can
     first check that it rebuilt the same input;
   * the reference's per-step wall time (stdout column 15, cross.f:1183), the
-    rank count and the host it ran on.
+    rank count and the host it ran on, and the per-step time of the FP64
+    oracle on the same grid and host with all its cores (OpenMP), so that
+    the restatement/reference speed ratio is measured on one machine.
 
 Run in the container that holds the reference tree (about 2 minutes on 8
 cores):
@@ -70,6 +72,23 @@ def fp32_initial_condition(cfg):
 
 def column_norms(f):
     return np.sqrt(np.sum(np.abs(f) ** 2, axis=2)).astype(np.float32)
+
+
+def time_oracle(cfg, s32, steps=4):
+    """Per-step seconds of the FP64 oracle (all cores) on the same state;
+    the first step is left out like the reference's."""
+    import time
+    from oracle import oracle as O
+    O.build()
+    g = O.Grid(cfg.mgalx, cfg.my, cfg.mgalz, s32["y"], cfg.Re, cfg.alp, cfg.bet, cfg.pr)
+    S = O.State(g, dt0=cfg.dt0, cfl=cfg.cfl, ncfl=cfg.ncfl, flags=1)
+    S.set(s32["vor"], s32["phi"], s32["psi"], s32["scal"], s32["u00"], s32["v00"], s32["w00"])
+    out = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        S.step(1)
+        out.append(time.perf_counter() - t)
+    return out[1:]
 
 
 def hre_dat(cfg, d, nstep):
@@ -137,9 +156,10 @@ def main():
     except OSError:
         pass
     fx["host"] = np.array(f"{cpu}; {os.cpu_count()} cores; {platform.machine()}")
+    fx["oracle_step_s"] = np.array(time_oracle(cfg, ic32))
     np.savez_compressed(OUT, **fx)
     print(f"wrote {OUT}: median reference step {np.median(w):.3f} s on {RANKS} ranks "
-          f"({os.cpu_count()} cores)")
+          f"({os.cpu_count()} cores); oracle step {np.median(fx['oracle_step_s']):.3f} s")
 
 
 if __name__ == "__main__":
