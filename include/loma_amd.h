@@ -225,6 +225,16 @@ int loma_visc(loma_run *run, float *vorwk, float *vor, float *phi, float *v,
  * per device and grown on demand. */
 int loma_band5_solve_dev(const double *band, void *rhs, double *x, int n,
                          int nsys, int rhs_fp32, void *stream);
+/* loma_band5_solve_dev without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernel and results; the scratch is a fixed
+ * array in the library's code object, 4,194,304 doubles (32 MiB) that every
+ * GPU the library runs on holds until the process ends, so n*5*nsys must not
+ * exceed 4,194,304 (LOMA_ERR_ARG otherwise; e.g. 1635 systems at n = 513).
+ * Calls that overlap on one GPU share that array and race. */
+int loma_band5_solve_dev_noalloc(const double *band, void *rhs, double *x, int n,
+                                 int nsys, int rhs_fp32, void *stream);
 
 /* Re-target a run at new physics without re-allocating: Reynolds number,
  * Prandtl number, CFL number and first time step (the grid, precision and

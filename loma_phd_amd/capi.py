@@ -27,7 +27,7 @@ EXPORTS = [
     "loma_cross1", "loma_set_stream", "loma_use_graph", "loma_deryr2_dev",
     "loma_deryyr2_dev", "loma_lapvdvhom_dev", "loma_solvepsi_dev", "loma_helmholtz_dev",
     "loma_inty8_dev", "loma_deryr2", "loma_deryyr2", "loma_lapvdvhom", "loma_solvepsi",
-    "loma_fourz_dev", "loma_band5_solve_dev",
+    "loma_fourz_dev", "loma_band5_solve_dev", "loma_band5_solve_dev_noalloc",
     "loma_advance_many", "loma_advance_async",
     "loma_tiles", "loma_footprint", "loma_step_begin", "loma_step_poll",
     "loma_visc_dev", "loma_visc", "loma_set_physics", "loma_seed", "loma_thickness",
@@ -93,6 +93,7 @@ def lib():
             "loma_solvepsi": ([_vp, _fp, _fp, _fp, d], i),
             "loma_fourz_dev": ([_vp, _vp, i, i], i),
             "loma_band5_solve_dev": ([_vp, _vp, _vp, i, i, i, _vp], i),
+            "loma_band5_solve_dev_noalloc": ([_vp, _vp, _vp, i, i, i, _vp], i),
             "loma_advance_many": ([ctypes.POINTER(_vp), i, i], i),
             "loma_tiles": ([_vp, ctypes.POINTER(i), ctypes.POINTER(i)], i),
             "loma_footprint": ([i, i, i, i, i, ctypes.POINTER(ctypes.c_longlong)], i),
@@ -139,6 +140,14 @@ def band5_solve_dev(band_ptr, rhs_ptr, x_ptr, n, nsys, rhs_fp32=False, stream=No
     (complex64 when rhs_fp32), x [nsys][n] complex128 or None for in place."""
     check(lib().loma_band5_solve_dev(band_ptr, rhs_ptr, x_ptr, n, nsys, 1 if rhs_fp32 else 0,
                                      stream))
+
+
+def band5_solve_dev_noalloc(band_ptr, rhs_ptr, x_ptr, n, nsys, rhs_fp32=False, stream=None):
+    """band5_solve_dev without device allocation: the scratch is a fixed
+    32 MiB array per GPU, so n*5*nsys <= 4194304; overlapping calls on one
+    GPU race."""
+    check(lib().loma_band5_solve_dev_noalloc(band_ptr, rhs_ptr, x_ptr, n, nsys,
+                                             1 if rhs_fp32 else 0, stream))
 
 
 def footprint(mgalx, my, mgalz, precision=LOMA_FP64, flags=0) -> int:

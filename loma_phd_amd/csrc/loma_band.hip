@@ -13,6 +13,10 @@
 // the backward sweep, which stages its output the same way.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
+#include "loma_internal.h"
 #include "loma_kernels.h"
 
 namespace loma {
@@ -143,4 +147,68 @@ int launch_band5(const double* band, const void* rhs, double* x, double* scratch
   return (int)hipGetLastError();
 }
 
+// ---- the allocation-free variant -------------------------------------------
+// Scratch of loma_band5_solve_dev_noalloc: a fixed device array that every
+// GPU the library runs on holds until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in
+// symbol); it uses the device address that hipGetSymbolAddress returns for
+// the current device, kept per device in g_scr_dev.
+constexpr size_t BAND5_SCRATCH_DOUBLES = (size_t)1 << 22;  // 4194304 doubles, 32 MiB
+static __device__ __align__(256) double scratch[BAND5_SCRATCH_DOUBLES];
+
+static std::mutex g_scr_mu;
+static std::vector<double*> g_scr_dev;  // device address of scratch, per device
+
+static int scratch_address(int dev, double** out) {
+  std::lock_guard<std::mutex> lk(g_scr_mu);
+  if ((int)g_scr_dev.size() <= dev) g_scr_dev.resize(dev + 1, nullptr);
+  if (!g_scr_dev[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      set_error("loma_band5_solve_dev_noalloc: no device address for the scratch array");
+      return LOMA_ERR_DEVICE;
+    }
+    g_scr_dev[dev] = (double*)p;
+  }
+  *out = g_scr_dev[dev];
+  return LOMA_OK;
+}
+
 }  // namespace loma
+
+extern "C" int loma_band5_solve_dev_noalloc(const double* band, void* rhs, double* x, int n,
+                                            int nsys, int rhs_fp32, void* stream) {
+  using namespace loma;
+  if (!band || !rhs || n < 3 || nsys < 0 || (rhs_fp32 && !x)) {
+    set_error("loma_band5_solve_dev_noalloc: bad arguments (n >= 3; x required for float rhs)");
+    return LOMA_ERR_ARG;
+  }
+  if ((size_t)n * 5 * (size_t)nsys > BAND5_SCRATCH_DOUBLES) {
+    set_error("loma_band5_solve_dev_noalloc: n*5*nsys exceeds the fixed scratch of "
+              "4194304 doubles (32 MiB); split the batch");
+    return LOMA_ERR_ARG;
+  }
+  if (nsys == 0) return LOMA_OK;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    set_error("loma_band5_solve_dev_noalloc: hipGetDevice failed");
+    return LOMA_ERR_DEVICE;
+  }
+  double* scr = nullptr;
+  int e = scratch_address(dev, &scr);
+  if (e) return e;
+  hipStream_t st = (hipStream_t)stream;
+  // every scratch entry the kernel reads it wrote first in the same launch,
+  // so nothing is cleared
+  e = launch_band5(band, rhs, x ? x : (double*)rhs, scr, n, nsys, rhs_fp32, st);
+  if (e) {
+    set_error("loma_band5_solve_dev_noalloc: launch failed");
+    return LOMA_ERR_DEVICE;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) {
+    set_error("loma_band5_solve_dev_noalloc: solve failed");
+    return LOMA_ERR_DEVICE;
+  }
+  return LOMA_OK;
+}
+
