@@ -112,9 +112,60 @@ def bench_precision(cfg, st, precision, steps, warmup, prof_steps):
         kernel_bytes=kb,
         step_bytes=step_bytes,
         S=run.S, W=run.W, P=run.P, device_bytes=run.bytes,
+        nsteps_total=warmup + steps + prof_steps,
     )
+    out["final_state"] = run.download()
     run.close()
     return out
+
+
+def bench_band(nsys=16384, sizes=(257, 385, 513), reps=20, seed=3):
+    """Batched banded solves in isolation: nsys = 128 x 128 (kx, kz) modes of
+    n-row pentadiagonal systems with seeded diagonally dominant bands
+    (|diag| >= 2.5 x the off-diagonal sum) plus a log-uniform shift in
+    [1, 1e4], complex N(0,1) right-hand sides in FP32 and FP64 (FP64 factor
+    and solve), through loma_band5_solve_dev.  Algorithmic bytes per system:
+    n x (40 band + 16 or 8 rhs + 16 solution); the factor scratch the kernel
+    writes and reads back (2 x 40 bytes per row) is not counted."""
+    import torch
+    from loma_phd_amd import capi
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    out = {}
+    for n in sizes:
+        band = torch.rand((nsys, n, 5), dtype=torch.float64, generator=g) * 2.0 - 1.0
+        shift = 10.0 ** (4.0 * torch.rand((nsys, 1), dtype=torch.float64, generator=g))
+        off = band[:, :, [0, 1, 3, 4]].abs().sum(dim=2)
+        band[:, :, 2] = 2.5 * off + 0.1 + shift
+        for j, d in ((0, 0), (0, 1), (1, 0), (n - 1, 4), (n - 1, 3), (n - 2, 4)):
+            band[:, j, d] = 0.0
+        bd = band.to(dev)
+        for rhs_fp32 in (False, True):
+            dt = torch.complex64 if rhs_fp32 else torch.complex128
+            rhs = torch.complex(torch.randn((nsys, n), dtype=torch.float64, generator=g),
+                                torch.randn((nsys, n), dtype=torch.float64, generator=g))
+            r = rhs.to(dt).to(dev)
+            x = torch.empty((nsys, n), dtype=torch.complex128, device=dev)
+            stream = torch.cuda.current_stream()
+            capi.band5_solve_dev(bd.data_ptr(), r.data_ptr(), x.data_ptr(), n, nsys,
+                                 rhs_fp32, stream.cuda_stream)
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            ev0.record(stream)
+            for _ in range(reps):
+                capi.band5_solve_dev(bd.data_ptr(), r.data_ptr(), x.data_ptr(), n, nsys,
+                                     rhs_fp32, stream.cuda_stream)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            ms = ev0.elapsed_time(ev1) / reps
+            alg = nsys * n * (40 + (8 if rhs_fp32 else 16) + 16)
+            out[f"n{n}_{'fp32' if rhs_fp32 else 'fp64'}_rhs"] = dict(
+                ms=round(ms, 4), algorithmic_bytes=alg,
+                achieved_GBps=round(alg / (ms * 1e-3) / 1e9, 1),
+                frac=round(alg / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                scratch_bytes=nsys * n * 80)
+    return dict(systems=nsys, form="per-system bands (bandec/banbks), one thread per system",
+                results=out)
 
 
 def roofline_of(res, traffic=None):
@@ -268,6 +319,8 @@ def main():
     ap.add_argument("--config", default="B", choices=["A", "B"])
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--band", type=int, default=1,
+                    help="1: time the batched banded solves in isolation (16,384 systems)")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--ensemble-runs", type=int, default=32,
                     help="runs in the ensemble-batch leg (0 skips it)")
@@ -339,6 +392,16 @@ def main():
     if "fp64" in res and "mixed" in res:
         line["speedup_mixed_vs_fp64"] = round(res["mixed"]["gpts_per_s"] /
                                               res["fp64"]["gpts_per_s"], 3)
+        # per-field error of the mixed state against the FP64 state after the
+        # same number of steps from the same initial condition
+        from loma_phd_amd import diagnostics
+        errs = diagnostics.field_errors(res["mixed"]["final_state"], res["fp64"]["final_state"],
+                                        ic.HostOps(st["y"]), cfg.alp, cfg.bet)
+        line["mixed_vs_fp64_field_errors"] = dict(
+            steps=res["mixed"]["nsteps_total"], **{k: float(f"{v:.3e}") for k, v in errs.items()})
+    if args.band:
+        print("[bench] batched banded solves", file=sys.stderr, flush=True)
+        line["band"] = bench_band()
     if args.ensemble_runs > 0:
         print(f"[bench] ensemble batch of {args.ensemble_runs} runs", file=sys.stderr,
               flush=True)

@@ -115,6 +115,13 @@ int loma_seed(loma_run *run, const double *coef, int nymax, double s,
  * the start of that step's third substep (rkstep 3, cross.f:692-699).  Both
  * are 0 until a step has run; LOMA_ERR_STATE without the flag. */
 int loma_thickness(loma_run *run, double *out);
+/* The columns of the reference's .cf history line that do not need the
+ * statistics window (cross.f:1166-1177): out[0..7] = time, dm, dw,
+ * Redw = dw*Re, trapz (the y-integral of the 00 mode of the constraint
+ * source at the last substep, cross.f:1034-1036), massu, massv, massw (the
+ * y-integrals of the 00 modes of rho u, rho v, rho w with the hy weights,
+ * cross.f:1013-1019).  dm, dw and Redw are NaN without LOMA_FLAG_HISTORY. */
+int loma_history(loma_run *run, double *out);
 
 /* Time advance: nsteps full RK3 steps (cross.f:225-1223 loop body),
  * device resident.  Returns LOMA_ERR_NAN if the state became non-finite. */
@@ -193,6 +200,25 @@ int loma_inty8(loma_run *run, const double *du, double *u);         /* laps.f:48
 /* fourz (fou3D.f:80) on nrows complex rows of length mgalz, FFTPACK
  * convention: iopt<0 forward scaled by 1/n with e^{-i}, iopt>=0 inverse. */
 int loma_fourz_dev(loma_run *run, double *c, int nrows, int iopt);
+/* foup2 (cross.f:1282-1430, with phys2 cross.f:1440-1828): the transform
+ * stage of a substep with the reference's argument list (the work array,
+ * statistics array and rank are dropped).  Arrays are the reference's
+ * complex*8 (my, mgalz, nplanes) blocks with the spectral content in the
+ * first mz kz slots of each plane.  On entry up1..up3 = rho u, rho v,
+ * rho w, ten22 = lap T, ten12, ten13, ten23 = dT/dx, dT/dy, dT/dz,
+ * drho = T; on return up1..up3 = u, v, w, ten11, ten22, ten33 = rho uu,
+ * rho vv, rho ww, ten12, ten13, ten23 = rho uv, rho uw, rho vw, drho = the
+ * constraint source, tnext = DT.  rhst is the real*8 (mgalx, mgalz*my)
+ * physical RHS history of T (line z + mgalz*y), updated.  dt is the
+ * substep's step size (Deltat); at rkstep 1 *dtr (if not NULL) receives
+ * CFL/(convective + diffusive maxima) of one rank.  Uses the run's device
+ * buffers, so the run must be uploaded again before it advances.  Runs
+ * without scalar Z only. */
+int loma_foup2(loma_run *run, float *up1, float *up2, float *up3, float *ten11,
+               float *ten22, float *ten33, double *rhst, float *drho, float *tnext,
+               float *ten12, float *ten13, float *ten23, int rkstep, double dt,
+               double *dtr);                                         /* cross.f:1282 */
+
 /* ---- transforms with the reference's shapes (host pointers, real(4)) ---
  * FFTPACK conventions: iopt < 0 forward e^{-i} scaled by 1/n, else inverse
  * e^{+i} unscaled; computed on the device in FP64, stored back as float. */
@@ -243,6 +269,21 @@ int loma_band5_solve_dev_noalloc(const double *band, void *rhs, double *x, int n
  * the same grid.  No reference counterpart (initcr, main.f:461-800, sets
  * these once per job). */
 int loma_set_physics(loma_run *run, double Re, double pr, double cfl, double dt0);
+
+/* Device checkpoint of a run's time-advance state: the spectral fields, the
+ * RK histories (RHS of Omega_y and phi, the physical RHS of T and Z), the
+ * 00-mode work arrays, time, dt and the step count.  loma_snapshot copies
+ * them into a device buffer the run keeps (allocated on the first call,
+ * about 7 S-sized complex fields plus the physical histories, outside
+ * loma_footprint); loma_restore copies them back.  For restart-on-NaN: the
+ * reference's initnan (trunk/mixing/src/main.f:953) re-reads the restart
+ * file and calls cross1 again; here the run returns to its last snapshot on
+ * the device. */
+int loma_snapshot(loma_run *run);
+int loma_restore(loma_run *run);
+/* Change the CFL number of a running run (the state, histories and time
+ * are kept; captured step graphs are rebuilt on the next step). */
+int loma_set_cfl(loma_run *run, double cfl);
 
 /* Transform tile shapes in use: y entries per z-transform workgroup (J) and
  * lines per x-line workgroup (L).  No reference counterpart. */

@@ -33,7 +33,8 @@ EXPORTS = [
     "loma_visc_dev", "loma_visc", "loma_set_physics", "loma_seed", "loma_thickness",
     "loma_sync", "loma_stream", "loma_profile", "loma_profile_read",
     "loma_deryr", "loma_deryyr", "loma_inty8", "loma_cft", "loma_rft", "loma_fourz",
-    "loma_fourx",
+    "loma_fourx", "loma_history", "loma_snapshot", "loma_restore", "loma_set_cfl",
+    "loma_foup2",
 ]
 KERNEL_CLASSES = ["phaseA", "zfft_inv", "xline", "zfft_fwd", "phaseC", "finish", "phaseC2"]
 
@@ -71,6 +72,10 @@ def lib():
             "loma_upload_z": ([_vp, _vp, i], i),
             "loma_seed": ([_vp, _vp, i, d, d, d, d], i),
             "loma_thickness": ([_vp, _dp], i),
+            "loma_history": ([_vp, _dp], i),
+            "loma_snapshot": ([_vp], i),
+            "loma_restore": ([_vp], i),
+            "loma_set_cfl": ([_vp, ctypes.c_double], i),
             "loma_download_z": ([_vp, _vp, i], i),
             "loma_advance": ([_vp, i], i),
             "loma_substep": ([_vp, i], i),
@@ -219,6 +224,26 @@ class Run:
             raise ValueError(f"coef shape {c.shape} is not (3, nplanes, mz, nymax)")
         check(lib().loma_seed(self._h, c.ctypes.data_as(_vp), c.shape[3], float(s),
                               float(amp), float(env_width), float(k0)))
+
+    def history(self):
+        """The .cf history columns that need no statistics window
+        (cross.f:1166-1177): time, dm, dw, Redw, trapz, massu, massv, massw
+        of the last step (dm, dw, Redw NaN without LOMA_FLAG_HISTORY)."""
+        o = np.zeros(8)
+        check(lib().loma_history(self._h, o.ctypes.data_as(_dp)))
+        return dict(zip(("time", "dm", "dw", "Redw", "trapz", "massu", "massv", "massw"), o))
+
+    def snapshot(self):
+        """Device checkpoint of the time-advance state (loma_snapshot)."""
+        check(lib().loma_snapshot(self._h))
+
+    def restore(self):
+        """Return to the last device checkpoint (loma_restore)."""
+        check(lib().loma_restore(self._h))
+
+    def set_cfl(self, cfl):
+        """Change the CFL number of the run, keeping its state (loma_set_cfl)."""
+        check(lib().loma_set_cfl(self._h, float(cfl)))
 
     def thickness(self):
         """(delta_m, delta_w) of a LOMA_FLAG_HISTORY run (loma_thickness)."""

@@ -11,6 +11,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -103,6 +104,10 @@ struct loma_run {
   hipEvent_t sev[2] = {nullptr, nullptr};
   DevScalars* hsc = nullptr;  // pinned copy of the scalars after the step
   bool inflight = false;
+  // device checkpoint of the time-advance state (loma_snapshot / loma_restore)
+  void* snap = nullptr;
+  size_t snap_bytes = 0;
+  int snap_istep = -1;
 };
 
 // kernel classes for profiling
@@ -276,6 +281,7 @@ int loma_free(loma_run* r) {
   pinned_put(r->hsc, sizeof(DevScalars));
   pinned_put(r->hstage, r->hstage_n * sizeof(double));
   if (r->arena) hipFree(r->arena);
+  if (r->snap) hipFree(r->snap);
   if (r->owns_stream && r->stream) hipStreamDestroy(r->stream);
   delete r;
   return LOMA_OK;
@@ -847,6 +853,7 @@ static PhaseArgs phase_args(loma_run* r, int rkstep) {
   a.u00wk = r->u00wk; a.v00wk = r->v00wk; a.w00wk = r->w00wk;
   a.rf0u = r->rf0u; a.rf0w = r->rf0w;
   a.mean_in = r->mean_in; a.mean_out = r->mean_out;
+  a.hy = r->hy;
   a.sc = r->sc;
   a.rkstep = rkstep;
   a.with_z = r->with_z ? 1 : 0;
@@ -1022,6 +1029,78 @@ static int build_graph(loma_run* r, int icfl) {
   hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   HIPCHK(ie);
+  return LOMA_OK;
+}
+
+// The pieces of a run that carry the time advance from one step to the next
+// (spectral state, RK histories, 00-mode work arrays, scalars); everything
+// else is rebuilt within a step or constant.
+static int snapshot_parts(loma_run* r, std::vector<std::pair<void*, size_t>>& parts) {
+  const size_t S = (size_t)r->nplanes * r->mz * r->my;
+  const size_t P8 = sizeof(double) * (size_t)r->mgalx * r->my * r->mgalz;
+  const size_t m8 = sizeof(double) * (size_t)r->my;
+  parts = {{r->vor, S * 16}, {r->phi, S * 16}, {r->psi, S * 16}, {r->scal, S * 16},
+           {r->ten12, S * 16}, {r->ten13, S * 16}, {r->rhst, P8},
+           {r->u00wk, m8}, {r->v00wk, m8}, {r->w00wk, m8}, {r->rf0u, m8}, {r->rf0w, m8},
+           {r->rbar, 2 * m8}, {r->rzbar, 2 * m8}, {r->sc, sizeof(DevScalars)}};
+  if (r->zf) parts.push_back({r->zf, S * 16});
+  if (r->rhsz) parts.push_back({r->rhsz, P8});
+  return LOMA_OK;
+}
+
+int loma_snapshot(loma_run* r) {
+  if (!r || !r->uploaded) { set_error("loma_snapshot: nothing uploaded"); return LOMA_ERR_STATE; }
+  if (r->inflight) { set_error("loma_snapshot: a step is in flight"); return LOMA_ERR_STATE; }
+  std::vector<std::pair<void*, size_t>> parts;
+  snapshot_parts(r, parts);
+  size_t need = 0;
+  for (auto& p : parts) need += align256(p.second);
+  if (!r->snap) {
+    if (hipMalloc(&r->snap, need) != hipSuccess) {
+      r->snap = nullptr;
+      set_error("loma_snapshot: device allocation failed");
+      return LOMA_ERR_OOM;
+    }
+    r->snap_bytes = need;
+  }
+  char* d = (char*)r->snap;
+  for (auto& p : parts) {
+    HIPCHK(hipMemcpyAsync(d, p.first, p.second, hipMemcpyDeviceToDevice, r->stream));
+    d += align256(p.second);
+  }
+  HIPCHK(hipStreamSynchronize(r->stream));
+  r->snap_istep = r->istep;
+  return LOMA_OK;
+}
+
+int loma_restore(loma_run* r) {
+  if (!r || !r->snap || r->snap_istep < 0) {
+    set_error("loma_restore: no snapshot taken");
+    return LOMA_ERR_STATE;
+  }
+  if (r->inflight) { set_error("loma_restore: a step is in flight"); return LOMA_ERR_STATE; }
+  std::vector<std::pair<void*, size_t>> parts;
+  snapshot_parts(r, parts);
+  const char* d = (const char*)r->snap;
+  for (auto& p : parts) {
+    HIPCHK(hipMemcpyAsync(p.first, d, p.second, hipMemcpyDeviceToDevice, r->stream));
+    d += align256(p.second);
+  }
+  HIPCHK(hipStreamSynchronize(r->stream));
+  r->istep = r->snap_istep;
+  return LOMA_OK;
+}
+
+int loma_set_cfl(loma_run* r, double cfl) {
+  if (!r || !(cfl > 0.0)) { set_error("loma_set_cfl: cfl must be positive"); return LOMA_ERR_ARG; }
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
+  if (r->stream) HIPCHK(hipStreamSynchronize(r->stream));
+  r->cfl = cfl;
+  r->rc.cfl = cfl;
+  // the step graphs and ensemble graphs hold the run constants by value
+  for (int q = 0; q < 2; ++q)
+    if (r->gexec[q]) { hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
+  ensemble_forget(r);
   return LOMA_OK;
 }
 
@@ -1316,6 +1395,24 @@ int loma_thickness(loma_run* r, double* out) {
   return LOMA_OK;
 }
 
+int loma_history(loma_run* r, double* out) {
+  if (!r || !out) return LOMA_ERR_ARG;
+  DevScalars s;
+  int e = read_scalars(r, &s);
+  if (e) return e;
+  const bool h = (r->flags & LOMA_FLAG_HISTORY) != 0;
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  out[0] = s.time;
+  out[1] = h ? s.dm : nan;
+  out[2] = h ? s.dw : nan;
+  out[3] = h ? s.dw * r->Re : nan;
+  out[4] = s.trapz;
+  out[5] = s.mass[0];
+  out[6] = s.mass[1];
+  out[7] = s.mass[2];
+  return LOMA_OK;
+}
+
 int loma_cross1(loma_run* r, float* vor, float* phi, float* psi, float* scal, double* u00,
                 double* w00, double* v00, int nstep) {
   int e = loma_upload(r, vor, phi, psi, scal, u00, v00, w00, 0);
@@ -1486,6 +1583,135 @@ int loma_visc(loma_run* r, float* vorwk, float* vor, float* phi, float* v, float
     for (int c = 0; c < 6; ++c)
       for (size_t q = 0; q < n; ++q) cols[c][q] = (float)h[c * n + q];
   return rc;
+}
+
+// ---- foup2: the transform stage with the reference's argument list -------
+// Arrays as foup2 declares them (cross.f:1282-1310): complex*8
+// (0:my-1, 0:mgalz-1, planes), the spectral content in the first mz kz
+// slots of every plane block; rhst real*8 (0:mgalx-1, lines), line
+// z + mgalz*y.  The run's device buffers carry the stage; dt is the step
+// size of the substep (the tem module's Deltat).
+static int put_spec(loma_run* r, const float* in, int q, double* mean) {
+  const int np = r->nplanes, mz = r->mz, my = r->my, mgz = r->mgalz, myp = r->rc.myp;
+  const size_t n = (size_t)np * mz * myp;
+  std::vector<double> h(2 * n, 0.0);
+  for (int i = 0; i < np; ++i)
+    for (int k = 0; k < mz; ++k)
+      for (int j = 0; j < my; ++j) {
+        const size_t src = 2 * (((size_t)i * mgz + k) * my + j);
+        const size_t dst = 2 * (((size_t)i * mz + k) * myp + j);
+        if (i == 0 && k == 0) {
+          // the 00 mode travels beside the transform (mean_in), real part only
+          mean[(size_t)q * my + j] = in[src];
+          continue;
+        }
+        h[dst] = in[src];
+        h[dst + 1] = in[src + 1];
+      }
+  if (r->fp32) {
+    std::vector<float> f(h.begin(), h.end());
+    HIPCHK(hipMemcpy(r->spec[q], f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(hipMemcpy(r->spec[q], h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  }
+  return LOMA_OK;
+}
+static int get_spec(loma_run* r, int q, const double* mean, float* out) {
+  const int np = r->nplanes, mz = r->mz, my = r->my, mgz = r->mgalz, myp = r->rc.myp;
+  const size_t n = (size_t)np * mz * myp;
+  std::vector<double> h(2 * n);
+  if (r->fp32) {
+    std::vector<float> f(2 * n);
+    HIPCHK(hipMemcpy(f.data(), r->spec[q], f.size() * 4, hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < 2 * n; ++e) h[e] = f[e];
+  } else {
+    HIPCHK(hipMemcpy(h.data(), r->spec[q], h.size() * 8, hipMemcpyDeviceToHost));
+  }
+  for (int i = 0; i < np; ++i)
+    for (int k = 0; k < mz; ++k)
+      for (int j = 0; j < my; ++j) {
+        const size_t dst = 2 * (((size_t)i * mgz + k) * my + j);
+        const size_t src = 2 * (((size_t)i * mz + k) * myp + j);
+        double re = h[src], im = h[src + 1];
+        if (i == 0 && k == 0) { re += mean[(size_t)q * my + j]; im = 0.0; }
+        out[dst] = (float)re;
+        out[dst + 1] = (float)im;
+      }
+  return LOMA_OK;
+}
+
+int loma_foup2(loma_run* r, float* up1, float* up2, float* up3, float* ten11, float* ten22,
+               float* ten33, double* rhst, float* drho, float* tnext, float* ten12,
+               float* ten13, float* ten23, int rkstep, double dt, double* dtr) {
+  if (!r || !up1 || !up2 || !up3 || !ten11 || !ten22 || !ten33 || !rhst || !drho ||
+      !tnext || !ten12 || !ten13 || !ten23 || rkstep < 1 || rkstep > 3 || !(dt > 0.0)) {
+    set_error("loma_foup2: null array, rkstep outside 1..3 or dt <= 0");
+    return LOMA_ERR_ARG;
+  }
+  if (r->with_z) { set_error("loma_foup2: the reference's foup2 carries no scalar Z"); return LOMA_ERR_ARG; }
+  if (r->inflight) { set_error("loma_foup2: a step is in flight"); return LOMA_ERR_STATE; }
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
+  int e = ensure_stream(r);
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(r->stream));
+  const int my = r->my, mgx = r->mgalx, mgz = r->mgalz;
+  // inputs in the order the transform kernels take them (cross.f:1324-1332)
+  const float* in[8] = {up1, up2, up3, ten22, ten12, ten13, ten23, drho};
+  std::vector<double> mean(12 * (size_t)my, 0.0);
+  for (int q = 0; q < 8; ++q)
+    if ((e = put_spec(r, in[q], q, mean.data()))) return e;
+  HIPCHK(hipMemcpy(r->mean_in, mean.data(), mean.size() * 8, hipMemcpyHostToDevice));
+  // rhst: lines z + mgalz*y (reference) -> [z][y][x] (here)
+  {
+    std::vector<double> h((size_t)mgx * my * mgz);
+    for (int j = 0; j < my; ++j)
+      for (int z = 0; z < mgz; ++z)
+        std::memcpy(&h[((size_t)z * my + j) * mgx], &rhst[((size_t)j * mgz + z) * mgx],
+                    sizeof(double) * mgx);
+    HIPCHK(hipMemcpy(r->rhst, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  }
+  // the step size of the substep
+  DevScalars s{};
+  HIPCHK(hipMemcpy(&s, r->sc, sizeof(s), hipMemcpyDeviceToHost));
+  s.dt = dt;
+  HIPCHK(hipMemcpy(r->sc, &s, sizeof(s), hipMemcpyHostToDevice));
+  const int do_cfl = rkstep == 1 ? 1 : 0;
+  FftArgs fa = fft_args(r, rkstep, do_cfl);
+  fa.nfields = 8;
+  LCHK(launch_zfft_inv(r->fp32, fa, r->stream));
+  LCHK(launch_xline(r->fp32, fa, r->stream));
+  fa.nfields = 11;
+  LCHK(launch_zfft_fwd(r->fp32, fa, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  HIPCHK(hipMemcpy(mean.data(), r->mean_out, mean.size() * 8, hipMemcpyDeviceToHost));
+  float* out[11] = {up1, up2, up3, ten11, ten22, ten33, ten12, ten13, ten23, drho, tnext};
+  for (int q = 0; q < 11; ++q)
+    if ((e = get_spec(r, q, mean.data(), out[q]))) return e;
+  {
+    std::vector<double> h((size_t)mgx * my * mgz);
+    HIPCHK(hipMemcpy(h.data(), r->rhst, h.size() * 8, hipMemcpyDeviceToHost));
+    for (int j = 0; j < my; ++j)
+      for (int z = 0; z < mgz; ++z)
+        std::memcpy(&rhst[((size_t)j * mgz + z) * mgx], &h[((size_t)z * my + j) * mgx],
+                    sizeof(double) * mgx);
+  }
+  if (dtr && do_cfl) {
+    // the CFL step of cross.f:1778-1818 from the x-line tiles' maxima, one rank
+    const int ntile = xl_tiles(my, mgz, r->L);
+    std::vector<double> c(4 * (size_t)ntile);
+    HIPCHK(hipMemcpy(c.data(), r->cflp, c.size() * 8, hipMemcpyDeviceToHost));
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < ntile; ++t)
+      for (int q = 0; q < 4; ++q) {
+        const double v = c[4 * (size_t)t + q];
+        if (v > m[q] || v != v) m[q] = v;
+      }
+    const double conv = m[0] * r->rc.hxalp * 0.5 + m[1] + m[2] * r->rc.hzbet * 0.5;
+    *dtr = r->rc.cfl / (conv + m[3]);
+  }
+  // the run's state no longer matches its step graphs' assumptions
+  r->uploaded = false;
+  return LOMA_OK;
 }
 
 // ---- host-pointer transforms and 1-D operators with the reference's shapes

@@ -3,14 +3,20 @@
 // triangular sweeps (banbks, cfdiff.f:954-974) for many independent systems
 // with complex right-hand sides, FP64 arithmetic.
 //
-// One thread owns one system and runs the elimination row by row; a
-// 64-thread workgroup serves 64 consecutive systems.  The inputs are
-// system-major ([sys][n][5] bands, [sys][n] complex right-hand sides), so
-// each workgroup stages RT rows of its 64 systems through LDS: the HBM side
-// reads 64 runs of RT*5 (RT*2) contiguous doubles, the thread side reads its
-// own rows.  The factors and the forward-substituted right-hand side go to a
-// scratch area laid out [row][field][system] (system fastest, coalesced) for
-// the backward sweep, which stages its output the same way.
+// A system is solved by two threads, one per wavefront of a 128-thread
+// workgroup: both run the elimination row by row on the (real) bands, wave
+// 0 on the real part of the right-hand side and wave 1 on the imaginary
+// part, so a workgroup serves 64 consecutive systems with two waves.  The
+// inputs are system-major ([sys][n][5] bands, [sys][n] complex right-hand
+// sides); tiles of RT rows of the workgroup's 64 systems go through LDS,
+// the HBM side reading 64 runs of RT*5 (RT*2) contiguous doubles.  The tiles
+// are double-buffered: the loads of tile t+1 are issued into registers
+// before the rows of tile t are eliminated and written to LDS after, so a
+// tile's load latency hides behind the previous tile's rows.  The factors
+// (wave 0) and the forward-substituted right-hand sides go to a scratch area
+// laid out [row][field][system] (system fastest, coalesced); the backward
+// sweep prefetches the scratch rows of the next tile into registers the
+// same way and stages its output through LDS for coalesced stores.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -21,53 +27,79 @@
 
 namespace loma {
 
-constexpr int B5_SYS = 64;  // systems per workgroup
-constexpr int B5_RT = 8;    // rows per LDS tile
+constexpr int B5_SYS = 64;                 // systems per workgroup
+constexpr int B5_RT = 8;                   // rows per tile
+constexpr int B5_THREADS = 2 * B5_SYS;     // real and imaginary wave
+constexpr int B5_PB = B5_SYS * B5_RT * 5 / B5_THREADS;  // band doubles per thread per tile
+constexpr int B5_PR = B5_SYS * B5_RT * 2 / B5_THREADS;  // rhs values per thread per tile
 
-__global__ __launch_bounds__(B5_SYS) void k_band5(const double* __restrict__ band,
-                                                  const void* __restrict__ rhs, int rhs_fp32,
-                                                  double* __restrict__ x,
-                                                  double* __restrict__ scr, int n, int nsys) {
-  __shared__ double tb[B5_SYS][B5_RT * 5 + 1];
-  __shared__ double tr[B5_SYS][B5_RT * 2 + 1];
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(B5_THREADS) void k_band5(const double* __restrict__ band,
+                                                      const void* __restrict__ rhs, int rhs_fp32,
+                                                      double* __restrict__ x,
+                                                      double* __restrict__ scr, int n, int nsys) {
+  __shared__ double tb[2][B5_SYS][B5_RT * 5 + 1];
+  __shared__ double tr[2][B5_SYS][B5_RT * 2 + 1];
+  const int tid = threadIdx.x, lane = tid & 63, part = tid >> 6;
   const int s0 = blockIdx.x * B5_SYS;
   const int s = s0 + lane;
   const int ns = min(B5_SYS, nsys - s0);
   const size_t ss = (size_t)nsys;  // scratch stride per field
-  // cooperative tile load of rows [r0, r0 + RT) of the workgroup's systems
-  auto load_tile = [&](int r0) {
-    const int rows = min(B5_RT, n - r0);
-    for (int it = lane; it < ns * B5_RT * 5; it += B5_SYS) {
-      const int q = it / (B5_RT * 5), e = it - q * (B5_RT * 5);
-      tb[q][e] = (e < rows * 5) ? band[((size_t)(s0 + q) * n + r0) * 5 + e] : 0.0;
+  const int ntile = (n + B5_RT - 1) / B5_RT;
+  // ---- tile prefetch: global -> registers -> LDS buffer ----
+  double pb[B5_PB], pr[B5_PR];
+  auto fetch = [&](int tl) {
+    const int r0 = tl * B5_RT, rows = min(B5_RT, n - r0);
+#pragma unroll
+    for (int q = 0; q < B5_PB; ++q) {
+      const int it = tid + q * B5_THREADS;
+      const int sy = it / (B5_RT * 5), e = it - sy * (B5_RT * 5);
+      pb[q] = (sy < ns && e < rows * 5) ? band[((size_t)(s0 + sy) * n + r0) * 5 + e] : 0.0;
     }
-    for (int it = lane; it < ns * B5_RT * 2; it += B5_SYS) {
-      const int q = it / (B5_RT * 2), e = it - q * (B5_RT * 2);
+#pragma unroll
+    for (int q = 0; q < B5_PR; ++q) {
+      const int it = tid + q * B5_THREADS;
+      const int sy = it / (B5_RT * 2), e = it - sy * (B5_RT * 2);
       double v = 0.0;
-      if (e < rows * 2) {
-        const size_t idx = ((size_t)(s0 + q) * n + r0) * 2 + e;
+      if (sy < ns && e < rows * 2) {
+        const size_t idx = ((size_t)(s0 + sy) * n + r0) * 2 + e;
         v = rhs_fp32 ? (double)((const float*)rhs)[idx] : ((const double*)rhs)[idx];
       }
-      tr[q][e] = v;
+      pr[q] = v;
     }
-    __syncthreads();
   };
-  // window of three rows of the working matrix (5 entries, column j-2+d)
-  double w[3][5], yr[3], yi[3];
-  auto fetch = [&](int j, double* row, double& br, double& bi) {
-    const int t = j % B5_RT;
+  auto commit = [&](int buf) {
 #pragma unroll
-    for (int d = 0; d < 5; ++d) row[d] = tb[lane][t * 5 + d];
-    br = tr[lane][t * 2];
-    bi = tr[lane][t * 2 + 1];
+    for (int q = 0; q < B5_PB; ++q) {
+      const int it = tid + q * B5_THREADS;
+      const int sy = it / (B5_RT * 5), e = it - sy * (B5_RT * 5);
+      tb[buf][sy][e] = pb[q];
+    }
+#pragma unroll
+    for (int q = 0; q < B5_PR; ++q) {
+      const int it = tid + q * B5_THREADS;
+      const int sy = it / (B5_RT * 2), e = it - sy * (B5_RT * 2);
+      tr[buf][sy][e] = pr[q];
+    }
   };
-  load_tile(0);
+  // row j of the working matrix (5 entries, column j-2+d) and this wave's
+  // part of its right-hand side, from the tile buffer holding row j
+  auto row_at = [&](int j, double* w, double& y) {
+    const int buf = (j / B5_RT) & 1, t = j % B5_RT;
+#pragma unroll
+    for (int d = 0; d < 5; ++d) w[d] = tb[buf][lane][t * 5 + d];
+    y = tr[buf][lane][t * 2 + part];
+  };
+  fetch(0);
+  commit(0);
+  __syncthreads();
+  if (ntile > 1) fetch(1);
+  // window of three rows
+  double w[3][5], y[3];
   for (int q = 0; q < 3; ++q) {
-    if (q < n) fetch(q, w[q], yr[q], yi[q]);
+    if (q < n) row_at(q, w[q], y[q]);
     else {
       for (int d = 0; d < 5; ++d) w[q][d] = 0.0;
-      yr[q] = yi[q] = 0.0;
+      y[q] = 0.0;
     }
   }
   // forward elimination: row k eliminates column k from rows k+1, k+2
@@ -78,72 +110,94 @@ __global__ __launch_bounds__(B5_SYS) void k_band5(const double* __restrict__ ban
       const double l = w[i][2 - i] * iv;
       w[i][2 - i] = 0.0;
       for (int d = 1; d <= 2; ++d) w[i][2 - i + d] -= l * w[0][2 + d];
-      yr[i] -= l * yr[0];
-      yi[i] -= l * yi[0];
+      y[i] -= l * y[0];
     }
     if (s < nsys) {
       double* o = scr + (size_t)k * 5 * ss + s;
-      o[0] = iv;
-      o[ss] = (k + 1 < n) ? w[0][3] : 0.0;
-      o[2 * ss] = (k + 2 < n) ? w[0][4] : 0.0;
-      o[3 * ss] = yr[0];
-      o[4 * ss] = yi[0];
+      if (part == 0) {
+        o[0] = iv;
+        o[ss] = (k + 1 < n) ? w[0][3] : 0.0;
+        o[2 * ss] = (k + 2 < n) ? w[0][4] : 0.0;
+      }
+      o[(3 + part) * ss] = y[0];
     }
     // shift the window; row k+3 enters
     for (int d = 0; d < 5; ++d) {
       w[0][d] = w[1][d];
       w[1][d] = w[2][d];
     }
-    yr[0] = yr[1]; yi[0] = yi[1];
-    yr[1] = yr[2]; yi[1] = yi[2];
+    y[0] = y[1];
+    y[1] = y[2];
     const int jn = k + 3;
     if (jn < n) {
       if (jn % B5_RT == 0) {
+        // tile jn/RT was prefetched; its buffer held tile jn/RT - 2, whose
+        // rows every thread has left
+        const int tl = jn / B5_RT;
         __syncthreads();
-        load_tile(jn);
+        commit(tl & 1);
+        __syncthreads();
+        if (tl + 1 < ntile) fetch(tl + 1);
       }
-      fetch(jn, w[2], yr[2], yi[2]);
+      row_at(jn, w[2], y[2]);
     } else {
       for (int d = 0; d < 5; ++d) w[2][d] = 0.0;
-      yr[2] = yi[2] = 0.0;
+      y[2] = 0.0;
     }
   }
-  // backward sweep x_k = (y_k - U1 x_{k+1} - U2 x_{k+2}) / pivot, staged
-  // through the right-hand-side tile for coalesced stores
-  double x1r = 0.0, x1i = 0.0, x2r = 0.0, x2i = 0.0;
-  __syncthreads();
-  const int ntile = (n + B5_RT - 1) / B5_RT;
+  // backward sweep x_k = (y_k - U1 x_{k+1} - U2 x_{k+2}) / pivot, tile by
+  // tile from the last; the next tile's scratch rows are loaded into
+  // registers before the current tile is solved
+  double cur[B5_RT][4], nxt[B5_RT][4];
+  auto sload = [&](int tl, double (&v)[B5_RT][4]) {
+    const int r0 = tl * B5_RT;
+#pragma unroll
+    for (int t = 0; t < B5_RT; ++t) {
+      const int k = r0 + t;
+      if (s < nsys && k < n) {
+        const double* o = scr + (size_t)k * 5 * ss + s;
+        v[t][0] = o[0];
+        v[t][1] = o[ss];
+        v[t][2] = o[2 * ss];
+        v[t][3] = o[(3 + part) * ss];
+      } else {
+        v[t][0] = v[t][1] = v[t][2] = v[t][3] = 0.0;
+      }
+    }
+  };
+  __syncthreads();  // the scratch rows of this workgroup are written (same threads)
+  sload(ntile - 1, cur);
+  double x1 = 0.0, x2 = 0.0;
   for (int tl = ntile - 1; tl >= 0; --tl) {
+    if (tl > 0) sload(tl - 1, nxt);
     const int r0 = tl * B5_RT;
     const int rows = min(B5_RT, n - r0);
-    for (int t = rows - 1; t >= 0; --t) {
-      const int k = r0 + t;
-      double xr = 0.0, xi = 0.0;
-      if (s < nsys) {
-        const double* o = scr + (size_t)k * 5 * ss + s;
-        const double iv = o[0], u1 = o[ss], u2 = o[2 * ss];
-        xr = (o[3 * ss] - u1 * x1r - u2 * x2r) * iv;
-        xi = (o[4 * ss] - u1 * x1i - u2 * x2i) * iv;
-      }
-      tr[lane][t * 2] = xr;
-      tr[lane][t * 2 + 1] = xi;
-      x2r = x1r; x2i = x1i;
-      x1r = xr; x1i = xi;
+#pragma unroll
+    for (int t = B5_RT - 1; t >= 0; --t) {
+      if (t >= rows) continue;
+      const double xv = (cur[t][3] - cur[t][1] * x1 - cur[t][2] * x2) * cur[t][0];
+      tr[0][lane][t * 2 + part] = xv;
+      x2 = x1;
+      x1 = xv;
     }
     __syncthreads();
-    for (int it = lane; it < ns * rows * 2; it += B5_SYS) {
+    for (int it = tid; it < ns * rows * 2; it += B5_THREADS) {
       const int q = it / (rows * 2), e = it - q * (rows * 2);
-      x[((size_t)(s0 + q) * n + r0) * 2 + e] = tr[q][e];
+      x[((size_t)(s0 + q) * n + r0) * 2 + e] = tr[0][q][e];
     }
     __syncthreads();
+#pragma unroll
+    for (int t = 0; t < B5_RT; ++t)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) cur[t][c] = nxt[t][c];
   }
 }
 
 int launch_band5(const double* band, const void* rhs, double* x, double* scratch, int n,
                  int nsys, int rhs_fp32, hipStream_t st) {
   if (nsys <= 0 || n <= 0) return 0;
-  hipLaunchKernelGGL(k_band5, dim3((nsys + B5_SYS - 1) / B5_SYS), dim3(B5_SYS), 0, st, band,
-                     rhs, rhs_fp32, x, scratch, n, nsys);
+  hipLaunchKernelGGL(k_band5, dim3((nsys + B5_SYS - 1) / B5_SYS), dim3(B5_THREADS), 0, st,
+                     band, rhs, rhs_fp32, x, scratch, n, nsys);
   return (int)hipGetLastError();
 }
 

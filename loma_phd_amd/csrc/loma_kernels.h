@@ -61,6 +61,7 @@ struct PhaseArgs {
   // (phase A writes them, the FP32/FP64 transform carries zero there) and
   // the per-y estimates subtracted from each output ([12][my] each)
   double *mean_in, *mean_out;
+  const double* hy;          // trapezoid half-spacings of y (main.f:689-696)
   DevScalars* sc;
   int rkstep;
   int with_z;

@@ -860,10 +860,31 @@ __global__ __launch_bounds__(256) void k_phaseC2(PhaseArgs a) {
       op_inty<M>(a.T, lane, x1);
       const double trapz = -col_value<M>(x1[0], my - 1);
       const double add = trapz / (1.0 + sqrt(rc.rhotop / rc.rhobot));
+      // y-integrals of the 00 modes for the .cf history, taken after the
+      // gamma update of rho u00, rho w00 and before that of rho v00
+      // (cross.f:1013-1019)
+      double su = 0.0, sv = 0.0, sw = 0.0;
+#pragma unroll
+      for (int t = 0; t < M; ++t) {
+        const int r = r0 + t;
+        if (r < my) {
+          su += a.u00wk[r] * a.hy[r];
+          sv += a.v00wk[r] * a.hy[r];
+          sw += a.w00wk[r] * a.hy[r];
+        }
+      }
+      su = wave_sum(su);
+      sv = wave_sum(sv);
+      sw = wave_sum(sw);
 #pragma unroll
       for (int t = 0; t < M; ++t)
         if (r0 + t < my) a.v00wk[r0 + t] = x1[0][t] + add;
-      if (lane == 0) a.sc->trapz = trapz;
+      if (lane == 0) {
+        a.sc->trapz = trapz;
+        a.sc->mass[0] = su;
+        a.sc->mass[1] = sv;
+        a.sc->mass[2] = sw;
+      }
       double z[2][M];
 #pragma unroll
       for (int t = 0; t < M; ++t) z[0][t] = z[1][t] = 0.0;

@@ -943,6 +943,7 @@ struct orc_state {
   double *u00wk, *v00wk, *w00wk, *rf0u, *rf0w;
   int have_bounds;
   int cfl_ranks; /* compute ranks of the reference's CFL reduction (1 = one rank) */
+  double trapz, mass[3]; /* .cf history: cross.f:1013-1019, 1034-1036 */
 };
 
 static size_t S_of(const orc_grid *g) { return (size_t)g->nplanes * g->mz * g->my; }
@@ -1244,6 +1245,41 @@ static void transform_pipeline(orc_state *st, cplx *const *in, cplx *const *out,
   free(lmax);
 }
 
+/* foup2 (cross.f:1282-1430) on its own: the 8 transform inputs (rho u,
+ * rho v, rho w, lap T, T_x, T_y, T_z, T; S complex each, mode-major
+ * [i][k][j]) to the 11 outputs (u, v, w, rho uu, rho vv, rho ww, rho uv,
+ * rho uw, rho vw, drho, DT) with the state's rhst history, for substep
+ * rkstep of step size dt; at rkstep 1 *dtr receives CFL/(conv + diff) of
+ * one rank (cross.f:1769-1818). */
+void orc_foup2(orc_state *st, const double *in, double *out, int rkstep, double dt,
+               double *dtr) {
+  const orc_grid *g = st->g;
+  const size_t S = S_of(g);
+  cplx *pin[8], *pout[11];
+  for (int q = 0; q < 8; q++) pin[q] = (cplx *)in + q * S;
+  for (int q = 0; q < 11; q++) pout[q] = (cplx *)out + q * S;
+  double alpha[3], beta[3];
+  rk_coefs(alpha, beta);
+  const double dtxi = dt * RK_XI[rkstep - 1];
+  const double dtgamma = dt * RK_GAMA[rkstep - 1];
+  const double dtbeta = dt * beta[rkstep - 1];
+  double cfl[2] = {0, 0};
+  const int do_cfl = rkstep == 1;
+  transform_pipeline(st, pin, pout, NULL, NULL, rkstep, dtxi, dtgamma, dtbeta, do_cfl, cfl);
+  if (dtr && do_cfl) *dtr = st->cfl / (cfl[0] + cfl[1]);
+}
+
+void orc_state_set_rhst(orc_state *st, const double *rhst) {
+  memcpy(st->rhst, rhst, (size_t)st->g->mgalx * st->g->my * st->g->mgalz * sizeof(double));
+}
+
+void orc_state_cf(const orc_state *st, double *out) {
+  out[0] = st->trapz;
+  out[1] = st->mass[0];
+  out[2] = st->mass[1];
+  out[3] = st->mass[2];
+}
+
 void orc_state_set_cfl_ranks(orc_state *st, int ranks) {
   st->cfl_ranks = ranks > 1 ? ranks : 1;
 }
@@ -1444,12 +1480,21 @@ int orc_substep(orc_state *st, int rkstep) {
     st->u00wk[j] = st->u00wk[j] + dtgamma * rf0u[j];
     st->w00wk[j] = st->w00wk[j] + dtgamma * rf0w[j];
   }
+  /* momentum of the 00 modes for the .cf history (cross.f:1013-1019),
+   * before rho v 00 is updated */
+  st->mass[0] = st->mass[1] = st->mass[2] = 0.0;
+  for (int j = 0; j < my; j++) {
+    st->mass[0] += st->u00wk[j] * g->hy[j];
+    st->mass[1] += st->v00wk[j] * g->hy[j];
+    st->mass[2] += st->w00wk[j] * g->hy[j];
+  }
   /* (8) 00 mode of rho v from the constraint (cross.f:1027-1072) */
   {
     double *d00 = malloc(my * sizeof(double)), *rf0v = malloc(my * sizeof(double));
     for (int j = 0; j < my; j++) d00[j] = creal(drho[SIDX(g, 0, 0, j)]);
     orc_inty8(g, d00, rf0v);
     double trapz = -rf0v[my - 1];
+    st->trapz = trapz;
     for (int j = 0; j < my; j++)
       st->v00wk[j] = rf0v[j] + trapz / (1.0 + sqrt(st->rhotop / st->rhobot));
     free(d00); free(rf0v);

@@ -98,6 +98,15 @@ void orc_state_history(const orc_state *st, double *ten12, double *ten13,
  * (cross.f:1778-1799, lines split as in change.f:188-201): the largest
  * per-rank sum of the convective maxima.  1 (the default) is one rank. */
 void orc_state_set_cfl_ranks(orc_state *st, int ranks);
+/* .cf history columns of the last substep: trapz, massu, massv, massw
+ * (cross.f:1013-1019, 1034-1036) */
+void orc_state_cf(const orc_state *st, double *out);
+/* foup2 (cross.f:1282-1430) alone: 8 transform inputs -> 11 outputs with the
+ * state's rhst, substep rkstep of step size dt; *dtr at rkstep 1 */
+void orc_foup2(orc_state *st, const double *in, double *out, int rkstep, double dt,
+               double *dtr);
+/* set the physical RHS history of T ([y][z][x], cross.f:92) */
+void orc_state_set_rhst(orc_state *st, const double *rhst);
 int orc_step(orc_state *st, int nsteps);           /* returns 1 on NaN */
 int orc_substep(orc_state *st, int rkstep);        /* one RK substep */
 void orc_state_scalars(const orc_state *st, double *out /* dt,time,dtr,istep */);

@@ -77,6 +77,9 @@ def lib():
         L.orc_state_scalars.argtypes = [vp, _dp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_state_set_cfl_ranks.argtypes = [vp, ctypes.c_int]
+        L.orc_state_cf.argtypes = [vp, _dp]
+        L.orc_foup2.argtypes = [vp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp]
+        L.orc_state_set_rhst.argtypes = [vp, _dp]
         _lib = L
     return _lib
 
@@ -258,6 +261,31 @@ class State:
 
     def step(self, n=1):
         return lib().orc_step(self._h, n)
+
+    def foup2(self, inputs, rkstep, dt):
+        """The transform stage alone (cross.f:1282-1430): inputs (8, nplanes,
+        mz, my) complex -> outputs (11, nplanes, mz, my), with the state's
+        rhst history updated; returns (outputs, dtr) (dtr at rkstep 1)."""
+        shp = self._shape()
+        a = np.ascontiguousarray(inputs, dtype=np.complex128).reshape((8,) + shp)
+        o = np.zeros((11,) + shp, np.complex128)
+        d = np.zeros(1)
+        lib().orc_foup2(self._h, _p(a.view(np.float64)), _p(o.view(np.float64)), int(rkstep),
+                        float(dt), _p(d))
+        return o, float(d[0])
+
+    def set_rhst(self, rhst):
+        """Set the physical RHS history of T ([y][z][x], the reference's line
+        order, cross.f:92)."""
+        r = np.ascontiguousarray(rhst, dtype=np.float64)
+        assert r.size == self.g.my * self.g.mgalz * self.g.mgalx
+        lib().orc_state_set_rhst(self._h, _p(r))
+
+    def history_cf(self):
+        """trapz, massu, massv, massw of the last substep (cross.f:1013-1036)."""
+        o = np.zeros(4)
+        lib().orc_state_cf(self._h, _p(o))
+        return dict(trapz=o[0], massu=o[1], massv=o[2], massw=o[3])
 
     def set_cfl_ranks(self, ranks):
         """Reduce the CFL maxima over `ranks` line slabs, as the reference's

@@ -21,6 +21,8 @@ The images written after 1, 3 and 10 steps are reduced to a small fixture:
     (kx, kz) column (float32), after 1, 3 and 10 steps;
   * the same column norms of the FP32 initial condition, so that a test can
     first check that it rebuilt the same input;
+  * the .cf history lines of every step (cross.f:1166-1177) without the
+    two timer columns;
   * the reference's per-step wall time (stdout column 15, cross.f:1183), the
     rank count and the host it ran on, and the per-step time of the FP64
     oracle on the same grid and host with all its cores (OpenMP), so that
@@ -123,6 +125,12 @@ def main():
                 walls.append((int(cols[0]), float(cols[14].replace("D", "E"))))
         imgs = {n: restart.read(os.path.join(d, "out.%03d" % n), prefer_sidecar=False)
                 for n in STEPS}
+        # the .cf history, one line per step (format 324 of cross.f:1172-1176:
+        # time, dm, dw, Redw, trapz, masst, enerdis, energy(9), massu,
+        # massv, massw, wall, comm)
+        with open(os.path.join(d, "sta.cf")) as f:
+            cf = np.array([[float(v.replace("D", "E")) for v in line.split()]
+                           for line in f if line.strip()])
     fx = dict(grid=np.array([cfg.mgalx, cfg.my, cfg.mgalz]),
               physics=np.array([cfg.Re, cfg.s, cfg.alp, cfg.bet, cfg.pr, cfg.cfl,
                                 cfg.amp, cfg.seed, cfg.dt0]),
@@ -139,6 +147,7 @@ def main():
         for n in STEPS:
             fx[f"cols_{f}_{n}"] = imgs[n][f][modes[:, 0], modes[:, 1]].astype(np.complex64)
             fx[f"norms_{f}_{n}"] = column_norms(imgs[n][f])
+    fx["cf"] = cf[:, :19]  # the wall and communication timers are left out
     for n in STEPS:
         fx[f"time_{n}"] = np.float32(imgs[n]["time"])
         for f in ("u00", "v00", "w00"):

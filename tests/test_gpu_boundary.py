@@ -64,15 +64,25 @@ def test_cross1_one_step_matches_reference(gpu):
            for f in ("u00", "w00", "v00")}
     start = {f: f32[f].astype(np.complex128) for f in f32}
     run = capi.Run(cfg.mgalx, cfg.my, cfg.mgalz, st["y"], cfg.Re, cfg.alp, cfg.bet, cfg.pr,
-                   cfg.cfl, cfg.ncfl, cfg.dt0, capi.LOMA_FP64, capi.LOMA_FLAG_DT_FP32)
+                   cfg.cfl, cfg.ncfl, cfg.dt0, capi.LOMA_FP64,
+                   capi.LOMA_FLAG_DT_FP32 | capi.LOMA_FLAG_HISTORY)
     L = capi.lib()
     cross1 = _bind(L, "loma_cross1", [ctypes.c_void_p] + [_fp] * 4 + [_dp] * 3 + [ctypes.c_int])
     fv = [f32[f].view(np.float32) for f in ("vor", "phi", "psi", "scal")]
     rc = cross1(run._h, *[_f(a) for a in fv], _d(m00["u00"]), _d(m00["w00"]), _d(m00["v00"]), 1)
     assert rc == 0, L.loma_last_error()
     t = run.scalars()["time"]
+    hist = run.history()
     run.close()
     assert abs(t - float(ref["time_1"])) <= 1e-6 * float(ref["time_1"])
+    # the .cf history line of the step (cross.f:1166-1177); the reference
+    # sums it in FP32 from its FP32 state
+    cf = ref["cf"][0]
+    for name, col, tol in (("dm", 1, 1e-6), ("dw", 2, 1e-5), ("trapz", 4, 5e-8),
+                           ("massu", 16, 2e-5), ("massv", 17, 1e-6), ("massw", 18, 1e-12)):
+        err = abs(hist[name] - cf[col])
+        print(f"  .cf {name}: {hist[name]:.9e} vs {cf[col]:.9e}", flush=True)
+        assert err <= tol, (name, hist[name], cf[col])
     for f, key in (("u00", "u00_1"), ("v00", "v00_1"), ("w00", "w00_1")):
         err = np.abs(m00[f] - ref[key]).max()
         print(f"  {f}: max abs err {err:.2e}", flush=True)

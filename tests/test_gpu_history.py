@@ -48,3 +48,33 @@ def test_thickness_needs_flag(gpu):
     with pytest.raises(capi.LomaError):
         run.thickness()
     run.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-6)])
+def test_cf_history_matches_oracle(gpu, oracle_mod, precision, tol):
+    """trapz and the momentum integrals of the 00 modes (the .cf columns of
+    cross.f:1013-1036) after two steps, GPU against the FP64 oracle."""
+    from loma_phd_amd import capi, ic
+    from loma_phd_amd.grid import MixingLayerConfig
+    O = oracle_mod
+    print("cf history precision", precision, flush=True)
+    cfg = MixingLayerConfig(mgalx=48, my=97, mgalz=36, s=3.0, Re=300.0, seed=6,
+                            scalar_z=False)
+    st = ic.initial_condition(cfg)
+    run = capi.Run(cfg.mgalx, cfg.my, cfg.mgalz, st["y"], cfg.Re, cfg.alp, cfg.bet, cfg.pr,
+                   cfg.cfl, cfg.ncfl, cfg.dt0, precision, 0)
+    run.upload(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"])
+    run.advance(2)
+    h = run.history()
+    run.close()
+    g = O.Grid(cfg.mgalx, cfg.my, cfg.mgalz, st["y"], cfg.Re, cfg.alp, cfg.bet, cfg.pr)
+    S = O.State(g, dt0=cfg.dt0, cfl=cfg.cfl, ncfl=cfg.ncfl, flags=0)
+    S.set(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"])
+    assert S.step(2) == 0
+    o = S.history_cf()
+    print("gpu", h, "oracle", o, flush=True)
+    assert h["dm"] != h["dm"]  # NaN without LOMA_FLAG_HISTORY
+    for name in ("trapz", "massu", "massv", "massw"):
+        scale = max(abs(o["massu"]), 1.0) if name != "trapz" else max(abs(o["trapz"]), 1e-12)
+        assert abs(h[name] - o[name]) <= tol * scale, (name, h[name], o[name])
+

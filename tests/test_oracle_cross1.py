@@ -32,6 +32,10 @@ INC_TOL = {1: 1e-2, 3: 2e-3, 10: 2e-3}   # change since the start, relative
 NORM_TOL = 1e-3       # per-mode column norms, energy weighted (FP32 noise: phi 2e-4)
 TIME_TOL = 1e-6       # time after n steps, relative
 MEAN_TOL = 1e-6       # u00, v00, w00 (momentum of the mean), absolute
+# .cf history columns, absolute: the reference sums them in FP32 from its
+# FP32 state (measured: trapz 2.8e-8, massu 5.9e-6 on -6.5, massv 4.1e-7,
+# massw 7e-14 over the 10 steps)
+CF_TOL = {"trapz": 5e-8, "massu": 2e-5, "massv": 1e-6, "massw": 1e-12}
 
 
 @pytest.fixture(scope="module")
@@ -96,6 +100,15 @@ def test_steps_match_reference_cross1(ref, start, oracle_mod):
             err = np.abs(a[f] - ref[f"{f}_{n}"]).max()
             print(f"  {f}: max abs err {err:.2e}")
             assert err <= MEAN_TOL, (n, f, err)
+        # .cf history of step n (cross.f:1166-1177): trapz (column 5) and the
+        # momentum of the 00 modes (columns 17-19)
+        cf = ref["cf"][n - 1]
+        h = S.history_cf()
+        for name, col in (("trapz", 4), ("massu", 16), ("massv", 17), ("massw", 18)):
+            tol = CF_TOL[name]
+            err = abs(h[name] - cf[col])
+            print(f"  .cf {name}: {h[name]:.9e} vs {cf[col]:.9e} (abs err {err:.1e})")
+            assert err <= tol, (n, name, h[name], cf[col])
         for f in ("vor", "phi", "psi", "scal"):
             m = ref[f"modes_{f}"]
             cr = ref[f"cols_{f}_{n}"].astype(np.complex128)
