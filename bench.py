@@ -164,7 +164,7 @@ def bench_band(nsys=16384, sizes=(257, 385, 513), reps=20, seed=3):
                 achieved_GBps=round(alg / (ms * 1e-3) / 1e9, 1),
                 frac=round(alg / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
                 scratch_bytes=nsys * n * 80)
-    return dict(systems=nsys, form="per-system bands (bandec/banbks), one thread per system",
+    return dict(systems=nsys, form="per-system bands (bandec/banbks), two wavefronts per 64 systems (real and imaginary right-hand side)",
                 results=out)
 
 

@@ -89,9 +89,9 @@ def test_steps_mixed(gpu, oracle_mod):
     for f, e in errs.items():
         assert e < 1e-5, (f, e)
     # the perturbation spectra carry FP32 transform rounding relative to the
-    # mean flow: bounded, not FP64-small
+    # mean flow (about 2e-6 observed): bounded, not FP64-small
     for f, e in spec.items():
-        assert e < 1e-3, (f, e)
+        assert e < 1e-5, (f, e)
 
 
 @pytest.mark.parametrize("precision", [1, 0])
