@@ -944,10 +944,216 @@ struct orc_state {
   int have_bounds;
   int cfl_ranks; /* compute ranks of the reference's CFL reduction (1 = one rank) */
   double trapz, mass[3]; /* .cf history: cross.f:1013-1019, 1034-1036 */
+  struct orc_stats *stats; /* statistics window, NULL until configured */
 };
 
 static size_t S_of(const orc_grid *g) { return (size_t)g->nplanes * g->mz * g->my; }
 #define SIDX(g, i, k, j) (((size_t)(i) * (g)->mz + (k)) * (g)->my + (j))
+
+/* ------------------------------------------------------------------------ */
+/* statistics window: addstats1 (cross.f:1981-2063), the phys2 sums          */
+/* (cross.f:1586-1607), addstats2 (cross.f:2070-2287), writestats            */
+/* (escru.f:276-484) and resetstats (escru.f:485-560)                        */
+/* ------------------------------------------------------------------------ */
+/* Accumulated columns, in the order writestats puts them in a .sta record:
+ *  0 um  1 vm  2 wm  3 rum  4 rvm  5 rwm  6 Tm  7 w1m  8 w2m  9 w3m 10 them
+ * 11 up 12 vp 13 wp 14 uvr 15 uwr 16 vwr 17 w1p 18 w2p 19 w3p 20 Tp 21 ep
+ * 22 ruu 23 ruv 24 ruw 25 rvv 26 rvw 27 rww 28 rhom 29 thep 30 theup
+ * 31 mum 32 num  (22..32 are written times 1/(mgalx mgalz)). */
+#define ORC_NSTA 33
+typedef struct orc_stats orc_stats;
+static void col_dy1(const orc_grid *g, const cplx *u, cplx *du);
+struct orc_stats {
+  int nsp, *jsp;        /* spectra rows (0-based), jsptot of main.f:698-703 */
+  int on, nacum, nacumsp;
+  double *acc;          /* [ORC_NSTA][my] */
+  double *spe;          /* [nplanes][12][nsp][nz+1], the .spe record order */
+  double ener[9];       /* uner of the last accumulated step (cross.f:802-816) */
+};
+
+static void orc_stats_free(orc_state *st) {
+  if (!st->stats) return;
+  free(st->stats->jsp);
+  free(st->stats->acc);
+  free(st->stats->spe);
+  free(st->stats);
+  st->stats = NULL;
+}
+
+/* resetstats: every accumulator and both counters to zero */
+void orc_stats_reset(orc_state *st) {
+  orc_stats *q = st->stats;
+  if (!q) return;
+  const orc_grid *g = st->g;
+  memset(q->acc, 0, sizeof(double) * ORC_NSTA * g->my);
+  memset(q->spe, 0, sizeof(double) * (size_t)g->nplanes * 12 * q->nsp * (g->nz + 1));
+  memset(q->ener, 0, sizeof q->ener);
+  q->nacum = q->nacumsp = 0;
+}
+
+/* Spectra rows (0-based, each < my) and a reset; returns 0, or -1 for a row
+ * outside the grid. */
+int orc_stats_config(orc_state *st, int nsp, const int *rows) {
+  const orc_grid *g = st->g;
+  for (int n = 0; n < nsp; n++)
+    if (rows[n] < 0 || rows[n] >= g->my) return -1;
+  orc_stats_free(st);
+  orc_stats *q = calloc(1, sizeof *q);
+  q->nsp = nsp;
+  q->jsp = malloc(sizeof(int) * (nsp > 0 ? nsp : 1));
+  memcpy(q->jsp, rows, sizeof(int) * nsp);
+  q->acc = calloc((size_t)ORC_NSTA * g->my, sizeof(double));
+  q->spe = calloc((size_t)g->nplanes * 12 * (nsp > 0 ? nsp : 1) * (g->nz + 1), sizeof(double));
+  st->stats = q;
+  return 0;
+}
+
+/* istati: accumulate at rkstep 1 of the following steps while on */
+void orc_stats_accumulate(orc_state *st, int on) {
+  if (st->stats) st->stats->on = on != 0;
+}
+
+static int stats_active(const orc_state *st, int rkstep) {
+  return rkstep == 1 && st->stats && st->stats->on;
+}
+static int kz_abs(const orc_grid *g, int k) { return k <= g->nz ? k : g->mz - k; }
+static double cabs2(cplx a) { return creal(a) * creal(a) + cimag(a) * cimag(a); }
+static double *spe_at(const orc_state *st, int i, int q, int iy, int ka) {
+  const orc_stats *s = st->stats;
+  return s->spe + (((size_t)i * 12 + q) * s->nsp + iy) * (st->g->nz + 1) + ka;
+}
+
+/* addstats1 on the transform inputs: rho u_i (mxwk, mywk, mzwk) and T
+ * (drho), after the 00 modes are set (cross.f:578-586). */
+static void addstats1(orc_state *st, const cplx *ru, const cplx *rv, const cplx *rw,
+                      const cplx *T) {
+  const orc_grid *g = st->g;
+  orc_stats *q = st->stats;
+  const int my = g->my;
+  double *acc = q->acc;
+  for (int i = 0; i < g->nplanes; i++)
+    for (int k = 0; k < g->mz; k++) {
+      const size_t o = SIDX(g, i, k, 0);
+      const double f = i == 0 ? 1.0 : 2.0;
+      const int ka = kz_abs(g, k);
+      for (int n = 0; n < q->nsp; n++) {
+        const int j = q->jsp[n];
+        *spe_at(st, i, 8, n, ka) += f * cabs2(ru[o + j]);
+        *spe_at(st, i, 9, n, ka) += f * cabs2(rv[o + j]);
+        *spe_at(st, i, 10, n, ka) += f * cabs2(rw[o + j]);
+        *spe_at(st, i, 11, n, ka) += f * cabs2(T[o + j]);
+      }
+      for (int j = 0; j < my; j++) acc[20 * my + j] += f * cabs2(T[o + j]);
+      if (i == 0 && k == 0)
+        for (int j = 0; j < my; j++) {
+          acc[3 * my + j] += creal(ru[o + j]);
+          acc[4 * my + j] += creal(rv[o + j]);
+          acc[5 * my + j] += creal(rw[o + j]);
+          acc[6 * my + j] += creal(T[o + j]);
+        }
+    }
+}
+
+/* addstats2 on the velocities u, v, w that foup2 returns, with the
+ * vorticities of cross.f:748-767 (o1 = dw/dy - ikz v, o2 = ikz u - ikx w,
+ * o3 = ikx v - du/dy) and dv/dy (cross.f:794-817, 2070-2287). */
+static void addstats2(orc_state *st, const cplx *u, const cplx *v, const cplx *w) {
+  const orc_grid *g = st->g;
+  orc_stats *q = st->stats;
+  const int my = g->my;
+  double *acc = q->acc;
+  cplx *du = malloc(my * sizeof(cplx)), *dv = malloc(my * sizeof(cplx)),
+       *dw = malloc(my * sizeof(cplx));
+  double ener[9] = {0};
+  for (int i = 0; i < g->nplanes; i++)
+    for (int k = 0; k < g->mz; k++) {
+      const size_t o = SIDX(g, i, k, 0);
+      const double f = i == 0 ? 1.0 : 2.0;
+      const int ka = kz_abs(g, k);
+      const cplx ikx = I * g->kx[i], ikz = I * g->kz[k];
+      const double rK = g->alp2[i] + g->bet2[k];
+      col_dy1(g, u + o, du);
+      col_dy1(g, v + o, dv);
+      col_dy1(g, w + o, dw);
+      for (int j = 0; j < my; j++) {
+        const cplx uu = u[o + j], vv = v[o + j], ww = w[o + j];
+        const cplx o1 = dw[j] - ikz * vv, o2 = ikz * uu - ikx * ww, o3 = ikx * vv - du[j];
+        const cplx cc = ikx * uu + dv[j] + ikz * ww;
+        double e[9];
+        e[0] = cabs2(uu); e[1] = cabs2(vv); e[2] = cabs2(ww);
+        e[3] = creal(uu * conj(vv)); e[4] = creal(uu * conj(ww)); e[5] = creal(ww * conj(vv));
+        e[6] = cabs2(o1); e[7] = cabs2(o2); e[8] = cabs2(o3);
+        for (int n = 0; n < 9; n++) {
+          acc[(11 + n) * my + j] += f * e[n];
+          ener[n] += f * e[n] * g->hy[j];
+        }
+        acc[29 * my + j] += f * cabs2(cc);
+        acc[30 * my + j] += f * creal(cc * conj(vv));
+        /* pseudo-dissipation: k^2 |u_i|^2 + |dv/dy|^2 + |o1 + ikz v|^2 +
+         * |o3 - ikx v|^2 */
+        const double aa = rK * (e[0] + e[1] + e[2]) + cabs2(dv[j]) + cabs2(o1 + ikz * vv) +
+                          cabs2(o3 - ikx * vv);
+        acc[21 * my + j] += f * aa;
+        if (i == 0 && k == 0) {
+          acc[0 * my + j] += creal(uu);
+          acc[1 * my + j] += creal(vv);
+          acc[2 * my + j] += creal(ww);
+          acc[7 * my + j] += creal(o1);
+          acc[8 * my + j] += creal(o2);
+          acc[9 * my + j] += creal(o3);
+          acc[10 * my + j] += creal(dv[j]);
+        }
+      }
+      for (int n = 0; n < q->nsp; n++) {
+        const int j = q->jsp[n];
+        const cplx uu = u[o + j], vv = v[o + j], ww = w[o + j];
+        const cplx uv = uu * conj(vv);
+        const double sp[8] = {cabs2(uu), cabs2(vv), cabs2(ww), creal(uv), cimag(uv),
+                              cabs2(dw[j] - ikz * vv), cabs2(ikz * uu - ikx * ww),
+                              cabs2(ikx * vv - du[j])};
+        for (int c = 0; c < 8; c++) *spe_at(st, i, c, n, ka) += f * sp[c];
+      }
+    }
+  memcpy(q->ener, ener, sizeof ener);
+  q->nacum++;
+  q->nacumsp++;
+  free(du); free(dv); free(dw);
+}
+
+/* writestats' outputs: the .sta columns (22..32 scaled by 1/(mgalx mgalz),
+ * [my][33] as one record row per y), the spectra ([nplanes][12][nsp][nz+1])
+ * and info = {nacum, nacumsp, masst, enerdis, energy(9)}. */
+void orc_stats_get(const orc_state *st, double *sta, double *spe, double *info) {
+  const orc_stats *q = st->stats;
+  const orc_grid *g = st->g;
+  const int my = g->my;
+  if (!q) return;
+  const double fac = 1.0 / ((double)g->mgalx * g->mgalz);
+  if (sta)
+    for (int j = 0; j < my; j++)
+      for (int c = 0; c < ORC_NSTA; c++)
+        sta[(size_t)j * ORC_NSTA + c] = q->acc[(size_t)c * my + j] * (c >= 22 ? fac : 1.0);
+  if (spe)
+    memcpy(spe, q->spe, sizeof(double) * (size_t)g->nplanes * 12 * q->nsp * (g->nz + 1));
+  if (info) {
+    info[0] = q->nacum;
+    info[1] = q->nacumsp;
+    /* masst (escru.f:372-376) runs j = 0..my-1 over wkstats(j,18) with
+     * hy(j+1); wkstats is declared (my,22), so its j = 0 entry is the
+     * element before wkstats(1,18) in memory, wkstats(my,17) (rww of the
+     * top row).  The reference's sum is restated as it runs. */
+    double masst = q->acc[27 * my + (my - 1)] * fac * g->hy[0];
+    for (int j = 1; j < my; j++) masst += fac * q->acc[28 * my + (j - 1)] * g->hy[j];
+    info[2] = masst;
+    /* enerdis: sum_j ep(j) / nacum * hy(j)  (cross.f:1152-1158) */
+    double enerdis = 0.0;
+    for (int j = 0; j < my; j++)
+      enerdis += q->nacum ? q->acc[21 * my + j] / q->nacum * g->hy[j] : 0.0;
+    info[3] = enerdis;
+    /* energy(i) = sqrt(|uner(i)|) (cross.f:811-816) */
+    for (int n = 0; n < 9; n++) info[4 + n] = sqrt(fabs(q->ener[n]));
+  }
+}
 
 orc_state *orc_state_create(const orc_grid *g, double dt0, double cfl,
                             int ncfl, int flags) {
@@ -986,6 +1192,7 @@ void orc_state_destroy(orc_state *st) {
   free(st->ten12); free(st->ten13); free(st->rhst);
   free(st->zf); free(st->rhsz);
   free(st->u00wk); free(st->v00wk); free(st->w00wk); free(st->rf0u); free(st->rf0w);
+  orc_stats_free(st);
   free(st);
 }
 
@@ -1137,6 +1344,27 @@ static void transform_pipeline(orc_state *st, cplx *const *in, cplx *const *out,
         for (int q = 0; q < nout; q++) o[q] = ph + ((size_t)(nin + q) * mgalz + z) * mgalx;
         double *rh = st->rhst + ((size_t)j * mgalz + z) * mgalx;
         double lx = 0, l0 = 0, lz = 0, Tmax = 0;
+        if (stats_active(st, rkstep)) {
+          /* statistics sums over the line (cross.f:1586-1607, sigma = 0:
+           * mu/mu0 = T^0, nu/nu0 = T^1); line j is this iteration's own */
+          double *acc = st->stats->acc;
+          for (int x = 0; x < mgalx; x++) {
+            double a = ru[x] * T[x];
+            acc[22 * my + j] += a * ru[x];
+            acc[23 * my + j] += a * rv[x];
+            acc[24 * my + j] += a * rw[x];
+            a = rv[x] * T[x];
+            acc[25 * my + j] += a * rv[x];
+            acc[26 * my + j] += a * rw[x];
+            a = rw[x] * T[x];
+            acc[27 * my + j] += a * rw[x];
+          }
+          for (int x = 0; x < mgalx; x++) {
+            acc[28 * my + j] += 1.0 / T[x];
+            acc[31 * my + j] += pow(T[x], 0.0);
+            acc[32 * my + j] += pow(T[x], 1.0);
+          }
+        }
         for (int x = 0; x < mgalx; x++) {
           double u = ru[x] * T[x], v = rv[x] * T[x], w = rw[x] * T[x];
           double tnext = T[x] + dtxi * rh[x];
@@ -1395,6 +1623,8 @@ int orc_substep(orc_state *st, int rkstep) {
     }
   }
 
+  if (stats_active(st, rkstep)) addstats1(st, mxw, myw, mzw, drho);
+
   /* (5) foup2 */
   int do_cfl = (rkstep == 1 && st->icfl);
   double cflres[2] = {0, 0};
@@ -1409,6 +1639,7 @@ int orc_substep(orc_state *st, int rkstep) {
                        dtgamma, dtbeta, do_cfl, cflres);
   }
   if (do_cfl) st->dtr = st->cfl / (cflres[0] + cflres[1]); /* cross.f:1810-1818 */
+  if (stats_active(st, rkstep)) addstats2(st, mxw, myw, mzw);
 
   /* (6a) T += DT (cross.f:619-627); Z += DZ */
   for (size_t s = 0; s < S; s++) st->scal[s] += chwk2[s];

@@ -111,6 +111,15 @@ int orc_step(orc_state *st, int nsteps);           /* returns 1 on NaN */
 int orc_substep(orc_state *st, int rkstep);        /* one RK substep */
 void orc_state_scalars(const orc_state *st, double *out /* dt,time,dtr,istep */);
 
+/* statistics window (addstats1/2, phys2 sums, writestats): spectra rows
+ * (0-based), accumulate on/off, read, reset */
+int orc_stats_config(orc_state *st, int nsp, const int *rows);
+void orc_stats_accumulate(orc_state *st, int on);
+void orc_stats_get(const orc_state *st, double *sta /* [my][33] */,
+                   double *spe /* [nplanes][12][nsp][nz+1] */,
+                   double *info /* nacum, nacumsp, masst, enerdis, energy[9] */);
+void orc_stats_reset(orc_state *st);
+
 /* flags for orc_state_create */
 #define ORC_DT_FP32 1   /* round dt to float like tem module (modulos.f:89) */
 #define ORC_SCALAR_Z 2  /* transport a passive scalar Z (lomaHZB cross.f:2736) */

@@ -80,6 +80,11 @@ def lib():
         L.orc_state_cf.argtypes = [vp, _dp]
         L.orc_foup2.argtypes = [vp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp]
         L.orc_state_set_rhst.argtypes = [vp, _dp]
+        L.orc_stats_config.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.orc_stats_config.restype = ctypes.c_int
+        L.orc_stats_accumulate.argtypes = [vp, ctypes.c_int]
+        L.orc_stats_get.argtypes = [vp, _dp, _dp, _dp]
+        L.orc_stats_reset.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -294,6 +299,34 @@ class State:
 
     def substep(self, rk):
         return lib().orc_substep(self._h, rk)
+
+    def stats_config(self, rows):
+        """Spectra rows (0-based y indices, jsptot of main.f:698-703) and a
+        reset of the statistics window."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        self._nsp = int(r.size)
+        rc = lib().orc_stats_config(self._h, self._nsp,
+                                    r.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        if rc != 0:
+            raise ValueError("spectra row outside the grid")
+
+    def stats_accumulate(self, on=True):
+        """istati: accumulate at rkstep 1 of the following steps."""
+        lib().orc_stats_accumulate(self._h, 1 if on else 0)
+
+    def stats(self):
+        """writestats' outputs: sta (my, 33) .sta columns, spe (nplanes, 12,
+        nsp, nz+1) spectra, and nacum, nacumsp, masst, enerdis, energy(9)."""
+        g = self.g
+        sta = np.zeros((g.my, 33))
+        spe = np.zeros((g.nplanes, 12, self._nsp, g.nz + 1))
+        info = np.zeros(13)
+        lib().orc_stats_get(self._h, _p(sta), _p(spe), _p(info))
+        return dict(sta=sta, spe=spe, nacum=int(info[0]), nacumsp=int(info[1]),
+                    masst=info[2], enerdis=info[3], energy=info[4:13].copy())
+
+    def stats_reset(self):
+        lib().orc_stats_reset(self._h)
 
     def scalars(self):
         o = np.zeros(4)
