@@ -123,6 +123,29 @@ int loma_thickness(loma_run *run, double *out);
  * cross.f:1013-1019).  dm, dw and Redw are NaN without LOMA_FLAG_HISTORY. */
 int loma_history(loma_run *run, double *out);
 
+/* Statistics window (replaces the istati/addstats machinery of cross1:
+ * addstats1 cross.f:1981-2063 at the transform inputs, the phys2 line sums
+ * cross.f:1586-1607, addstats2 cross.f:2070-2287 at the transform outputs,
+ * writestats escru.f:276-484 and resetstats escru.f:485-560).
+ * loma_stats_config: the nspec spectra rows jspecy (1-based y indices, each
+ * in 1..my; the reference's ctes3D jspecy), to which the centre row
+ * (my+1)/2 is appended as main.f:698-703 does; allocates the accumulators
+ * (once per call) and resets them.  loma_stats_accumulate(on): while on,
+ * rkstep 1 of every step adds to the window (istati = 1, cross.f:236-237);
+ * such steps run without the captured step graph, and loma_advance_many
+ * refuses the run.  loma_stats_read: sta = my x 33 rows of the .sta record
+ * (um vm wm rum rvm rwm Tm w1m w2m w3m them, up vp wp uvr uwr vwr w1p w2p
+ * w3p Tp ep, then ruu ruv ruw rvv rvw rww rhom thep theup mum num times
+ * 1/(mgalx mgalz)); spe = nplanes records of 12 x (nspec+1) x (nz+1)
+ * floats, the .spe order spwk(0:nz1, 1:nspec+1, 12, plane); info[13] =
+ * nacum, nacumsp, masst, enerdis, energy(9) (the .cf columns 6..16,
+ * cross.f:1152-1158, 811-816; escru.f:372-376).  Any output may be NULL.
+ * loma_stats_reset: resetstats (zeros, nacum = 0). */
+int loma_stats_config(loma_run *run, int nspec, const int *jspecy);
+int loma_stats_accumulate(loma_run *run, int on);
+int loma_stats_read(loma_run *run, double *sta, float *spe, double *info);
+int loma_stats_reset(loma_run *run);
+
 /* Time advance: nsteps full RK3 steps (cross.f:225-1223 loop body),
  * device resident.  Returns LOMA_ERR_NAN if the state became non-finite. */
 int loma_advance(loma_run *run, int nsteps);

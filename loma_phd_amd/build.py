@@ -20,10 +20,14 @@ LIB = os.path.join(LIBDIR, "libloma_hip.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LOMA_OFFLOAD_ARCH", "gfx950")
-# packed FP32 instructions are disabled for all device code
+# packed FP32 instructions are disabled for all device code.  Multiply-adds
+# are fused only where one source expression holds both (-ffp-contract=on,
+# not the default "fast", under which the backend fuses across statements
+# depending on the surrounding code): a kernel body compiled into the
+# single-run and the batched kernel then rounds identically in both.
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-Wno-unused-command-line-argument"]
+             "-ffp-contract=on", "-Wno-unused-command-line-argument"]
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17"]
 
 
@@ -36,7 +40,7 @@ def _sources():
 def _compile(src: str, force: bool) -> str:
     base = os.path.splitext(os.path.basename(src))[0]
     obj = os.path.join(OBJ, base + ".o")
-    headers = glob.glob(os.path.join(CSRC, "*.h")) + [
+    headers = glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(CSRC, "*.inc")) + [
         os.path.join(os.path.dirname(PKG), "include", "loma_amd.h")]
     newest = max([os.path.getmtime(src)] + [os.path.getmtime(h) for h in headers])
     if not force and os.path.exists(obj) and os.path.getmtime(obj) >= newest:

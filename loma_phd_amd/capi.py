@@ -34,7 +34,8 @@ EXPORTS = [
     "loma_sync", "loma_stream", "loma_profile", "loma_profile_read",
     "loma_deryr", "loma_deryyr", "loma_inty8", "loma_cft", "loma_rft", "loma_fourz",
     "loma_fourx", "loma_history", "loma_snapshot", "loma_restore", "loma_set_cfl",
-    "loma_foup2",
+    "loma_foup2", "loma_stats_config", "loma_stats_accumulate", "loma_stats_read",
+    "loma_stats_reset",
 ]
 KERNEL_CLASSES = ["phaseA", "zfft_inv", "xline", "zfft_fwd", "phaseC", "finish", "phaseC2"]
 
@@ -73,6 +74,10 @@ def lib():
             "loma_seed": ([_vp, _vp, i, d, d, d, d], i),
             "loma_thickness": ([_vp, _dp], i),
             "loma_history": ([_vp, _dp], i),
+            "loma_stats_config": ([_vp, i, ctypes.POINTER(i)], i),
+            "loma_stats_accumulate": ([_vp, i], i),
+            "loma_stats_read": ([_vp, _dp, _fp, _dp], i),
+            "loma_stats_reset": ([_vp], i),
             "loma_snapshot": ([_vp], i),
             "loma_restore": ([_vp], i),
             "loma_set_cfl": ([_vp, ctypes.c_double], i),
@@ -232,6 +237,35 @@ class Run:
         o = np.zeros(8)
         check(lib().loma_history(self._h, o.ctypes.data_as(_dp)))
         return dict(zip(("time", "dm", "dw", "Redw", "trapz", "massu", "massv", "massw"), o))
+
+    def stats_config(self, jspecy):
+        """Open a statistics window: spectra rows jspecy (1-based y indices,
+        the reference's ctes3D list; the centre row (my+1)/2 is appended) and
+        zeroed accumulators (loma_stats_config)."""
+        rows = np.ascontiguousarray(jspecy, dtype=np.int32)
+        self._nsp = int(rows.size) + 1
+        check(lib().loma_stats_config(self._h, int(rows.size),
+                                      rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
+    def stats_accumulate(self, on=True):
+        """istati: rkstep 1 of the following steps adds to the window."""
+        check(lib().loma_stats_accumulate(self._h, 1 if on else 0))
+
+    def stats_read(self):
+        """writestats' outputs: sta (my, 33) .sta record rows, spe (nplanes,
+        12, nspec+1, nz+1) float32 spectra, and nacum, nacumsp, masst,
+        enerdis, energy(9)."""
+        sta = np.zeros((self.my, 33))
+        spe = np.zeros((self.nplanes, 12, self._nsp, self.nz + 1), np.float32)
+        info = np.zeros(13)
+        check(lib().loma_stats_read(self._h, sta.ctypes.data_as(_dp), spe.ctypes.data_as(_fp),
+                                    info.ctypes.data_as(_dp)))
+        return dict(sta=sta, spe=spe, nacum=int(info[0]), nacumsp=int(info[1]),
+                    masst=info[2], enerdis=info[3], energy=info[4:13].copy())
+
+    def stats_reset(self):
+        """resetstats: zero the window's accumulators and counters."""
+        check(lib().loma_stats_reset(self._h))
 
     def snapshot(self):
         """Device checkpoint of the time-advance state (loma_snapshot)."""

@@ -7,9 +7,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -108,6 +110,14 @@ struct loma_run {
   void* snap = nullptr;
   size_t snap_bytes = 0;
   int snap_istep = -1;
+  // statistics window (loma_stats_*): device accumulators, the spectra rows
+  // and the host-side switch and counter; allocated by loma_stats_config
+  double *st_acc = nullptr, *st_spe = nullptr, *st_part = nullptr, *st_rho = nullptr,
+         *st_last = nullptr;
+  int* st_jsp = nullptr;
+  int st_nsp = 0;
+  bool st_on = false;
+  int st_nacum = 0;
 };
 
 // kernel classes for profiling
@@ -247,6 +257,8 @@ struct EnsembleGraph {
   hipStream_t master = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<hipEvent_t> ready;  // per run: work already queued on its stream
+  void* desc = nullptr;           // device argument blocks of the batched launches
+  std::vector<hipStream_t> lanes; // one capture stream per group of runs
 };
 static std::mutex g_ens_mu;
 static std::vector<EnsembleGraph> g_ens;
@@ -257,9 +269,13 @@ static void ensemble_forget(loma_run* r) {
     bool has = false;
     for (loma_run* x : g_ens[q].runs) has = has || x == r;
     if (has) {
+      // a launch of the graph may still read the argument blocks
+      if (g_ens[q].master) hipStreamSynchronize(g_ens[q].master);
       if (g_ens[q].exec) hipGraphExecDestroy(g_ens[q].exec);
       if (g_ens[q].master) hipStreamDestroy(g_ens[q].master);
       for (hipEvent_t e : g_ens[q].ready) hipEventDestroy(e);
+      if (g_ens[q].desc) hipFree(g_ens[q].desc);
+      for (hipStream_t l : g_ens[q].lanes) hipStreamDestroy(l);
       g_ens.erase(g_ens.begin() + q);
     } else {
       ++q;
@@ -282,6 +298,9 @@ int loma_free(loma_run* r) {
   pinned_put(r->hstage, r->hstage_n * sizeof(double));
   if (r->arena) hipFree(r->arena);
   if (r->snap) hipFree(r->snap);
+  for (void* p : {(void*)r->st_acc, (void*)r->st_spe, (void*)r->st_part, (void*)r->st_rho,
+                  (void*)r->st_last, (void*)r->st_jsp})
+    if (p) hipFree(p);
   if (r->owns_stream && r->stream) hipStreamDestroy(r->stream);
   delete r;
   return LOMA_OK;
@@ -901,11 +920,32 @@ static FftArgs fft_args(loma_run* r, int rkstep, int do_cfl) {
     if (pe_) return pe_;                     \
   } while (0)
 
+static StatsArgs stats_args(loma_run* r) {
+  StatsArgs a{};
+  a.rc = r->rc;
+  a.T = r->T;
+  for (int q = 0; q < 12; ++q) a.spec[q] = r->spec[q];
+  a.mean_in = r->mean_in;
+  a.mean_out = r->mean_out;
+  a.jsp = r->st_jsp;
+  a.nsp = r->st_nsp;
+  a.acc = r->st_acc;
+  a.spe = r->st_spe;
+  a.part = r->st_part;
+  a.rho = r->st_rho;
+  a.last = r->st_last;
+  return a;
+}
+
 static int enqueue_substep(loma_run* r, int rkstep, int icfl) {
   hipStream_t st = r->stream;
+  // the statistics window accumulates at rkstep 1 (cross.f:579, 795, 1586)
+  const bool stats = r->st_on && rkstep == 1;
   PhaseArgs pa = phase_args(r, rkstep);
   PROF(KC_PHASEA, launch_phaseA(r->M, r->fp32, pa, st));
+  if (stats) LCHK(launch_stats(r->M, r->fp32, 1, stats_args(r), st));
   FftArgs fa = fft_args(r, rkstep, (rkstep == 1 && icfl) ? 1 : 0);
+  fa.stat_rho = stats ? r->st_rho : nullptr;
   fa.nfields = r->with_z ? 12 : 8;
   PROF(KC_ZINV, launch_zfft_inv(r->fp32, fa, st));
   PROF(KC_XLINE, launch_xline(r->fp32, fa, st));
@@ -913,6 +953,12 @@ static int enqueue_substep(loma_run* r, int rkstep, int icfl) {
   PROF(KC_ZFWD, launch_zfft_fwd(r->fp32, fa, st));
   PROF(KC_PHASEC, launch_phaseC(r->M, r->fp32, pa, st));
   PROF(KC_PHASEC2, launch_phaseC2(r->M, r->fp32, pa, st));
+  if (stats) {
+    const StatsArgs sa = stats_args(r);
+    LCHK(launch_stats(r->M, r->fp32, 2, sa, st));
+    LCHK(launch_stats(r->M, r->fp32, 3, sa, st));
+    r->st_nacum++;
+  }
   if ((r->flags & LOMA_FLAG_HISTORY) && rkstep != 2)
     LCHK(launch_thickness(r->M, r->fp32, pa, r->hy, r->y[r->my - 1] - r->y[0],
                          rkstep == 1 ? 1 : 3, st));
@@ -1000,6 +1046,160 @@ static int enqueue_step(loma_run* r, int icfl) {
   }
   PROF(KC_FINISH, launch_finish_step(r->sc, r->rc, icfl, r->cflp,
                                      xl_tiles(r->my, r->mgalz, r->L), r->stream));
+  return LOMA_OK;
+}
+
+// ---- batched ensemble step ------------------------------------------------
+// The runs of an ensemble are grouped by everything that selects a kernel
+// instantiation: M and precision (phase kernels), mgalz and the z tile J, mgalx
+// and the x tile L, and Z.  Each group takes one launch per kernel and
+// substage; the launch's blocks are the runs' own grids one after another and
+// each block reads its run's argument block from device memory, so a run's
+// blocks compute exactly what its own launch computes.  Each group's step
+// runs on a stream of its own inside the capture (forked from the ensemble's
+// stream and joined back), so the groups' kernels overlap on the device as
+// the runs' own kernels did.
+struct EnsLaunch {
+  enum Kind { PHASE, ZINV, XLINE, ZFWD, THICK, FINISH } kind;
+  int which = 0;      // phase: 0 A, 1 C, 2 C2; thickness: 1 or 3
+  int M = 0;
+  bool fp32 = false;
+  FftArgs proto{};    // host copy of the group's first run (lengths, tiles)
+  size_t args_off = 0, first_off = 0;
+  int nrun = 0, nblocks = 0;
+  loma_run* run = nullptr;  // thickness
+  PhaseArgs pa{};           // thickness
+};
+struct EnsPlan {
+  std::vector<unsigned char> bytes;
+  std::vector<std::vector<EnsLaunch>> lanes;  // one launch list per group
+  size_t put(const void* p, size_t n) {
+    const size_t off = (bytes.size() + 255) / 256 * 256;
+    bytes.resize(off + n);
+    std::memcpy(bytes.data() + off, p, n);
+    return off;
+  }
+};
+
+static int ens_batch_cap() {
+  const char* e = std::getenv("LOMA_ENS_BATCH");
+  const int v = e ? std::atoi(e) : 0;
+  return v > 0 ? v : 1 << 20;
+}
+
+static void ensemble_plan(loma_run* const* runs, int nruns, const std::vector<int>& icfl,
+                          EnsPlan& plan) {
+  std::vector<std::array<int, 7>> keys;
+  std::vector<std::vector<int>> groups;
+  for (int q = 0; q < nruns; ++q) {
+    loma_run* r = runs[q];
+    const std::array<int, 7> k{r->M, r->fp32 ? 1 : 0, r->mgalz, r->J, r->mgalx, r->L,
+                               r->with_z ? 1 : 0};
+    size_t c = 0;
+    while (c < keys.size() && keys[c] != k) ++c;
+    if (c == keys.size()) { keys.push_back(k); groups.emplace_back(); }
+    groups[c].push_back(q);
+  }
+  // at most `cap` runs per launch: a group larger than that is dealt round
+  // robin to several lanes, which then overlap (LOMA_ENS_BATCH)
+  const int cap = ens_batch_cap();
+  std::vector<std::vector<int>> split;
+  for (const auto& g : groups) {
+    const int nl = ((int)g.size() + cap - 1) / cap;
+    for (int l = 0; l < nl; ++l) {
+      split.emplace_back();
+      for (size_t n = l; n < g.size(); n += nl) split.back().push_back(g[n]);
+    }
+  }
+  groups.swap(split);
+  for (const auto& g : groups) {
+    std::vector<EnsLaunch> lane;
+    loma_run* r0 = runs[g[0]];
+    for (int rk = 1; rk <= 3; ++rk) {
+      std::vector<PhaseArgs> pa;
+      std::vector<FftArgs> fa;
+      for (int q : g) {
+        pa.push_back(phase_args(runs[q], rk));
+        fa.push_back(fft_args(runs[q], rk, (rk == 1 && icfl[q]) ? 1 : 0));
+      }
+      const size_t pa_off = plan.put(pa.data(), pa.size() * sizeof(PhaseArgs));
+      const size_t fa_off = plan.put(fa.data(), fa.size() * sizeof(FftArgs));
+      auto launch = [&](EnsLaunch::Kind kind, int which) {
+        std::vector<int> first;
+        int nb = 0;
+        for (size_t n = 0; n < g.size(); ++n) {
+          first.push_back(nb);
+          loma_run* r = runs[g[n]];
+          nb += kind == EnsLaunch::PHASE ? phase_blocks(which, r->rc)
+                : kind == EnsLaunch::XLINE ? xline_blocks(fa[n]) : zfft_blocks(fa[n]);
+        }
+        EnsLaunch L;
+        L.kind = kind; L.which = which;
+        L.M = r0->M; L.fp32 = r0->fp32;
+        L.proto = fa[0];
+        L.proto.nfields = kind == EnsLaunch::ZINV ? (r0->with_z ? 12 : 8) : (r0->with_z ? 12 : 11);
+        L.args_off = kind == EnsLaunch::PHASE ? pa_off : fa_off;
+        L.first_off = plan.put(first.data(), first.size() * sizeof(int));
+        L.nrun = (int)g.size(); L.nblocks = nb;
+        lane.push_back(L);
+      };
+      launch(EnsLaunch::PHASE, 0);
+      launch(EnsLaunch::ZINV, 0);
+      launch(EnsLaunch::XLINE, 0);
+      launch(EnsLaunch::ZFWD, 0);
+      launch(EnsLaunch::PHASE, 1);
+      launch(EnsLaunch::PHASE, 2);
+      if (rk != 2)
+        for (size_t n = 0; n < g.size(); ++n)
+          if (runs[g[n]]->flags & LOMA_FLAG_HISTORY) {
+            EnsLaunch L;
+            L.kind = EnsLaunch::THICK; L.which = rk == 1 ? 1 : 3;
+            L.run = runs[g[n]]; L.pa = pa[n];
+            lane.push_back(L);
+          }
+    }
+    std::vector<FinishArgs> fin;
+    for (int q : g) {
+      loma_run* r = runs[q];
+      fin.push_back(FinishArgs{r->sc, r->rc, r->cflp, icfl[q], xl_tiles(r->my, r->mgalz, r->L)});
+    }
+    EnsLaunch L;
+    L.kind = EnsLaunch::FINISH;
+    L.args_off = plan.put(fin.data(), fin.size() * sizeof(FinishArgs));
+    L.nrun = (int)g.size();
+    lane.push_back(L);
+    plan.lanes.push_back(std::move(lane));
+  }
+}
+
+static int enqueue_lane(hipStream_t st, const void* desc, const std::vector<EnsLaunch>& lane) {
+  const char* d = (const char*)desc;
+  for (const EnsLaunch& L : lane) {
+    const int* first = (const int*)(d + L.first_off);
+    switch (L.kind) {
+      case EnsLaunch::PHASE:
+        LCHK(launch_phase_batch(L.M, L.fp32, L.which, (const PhaseArgs*)(d + L.args_off), first,
+                                L.nrun, L.nblocks, st));
+        break;
+      case EnsLaunch::ZINV:
+      case EnsLaunch::ZFWD:
+        LCHK(launch_zfft_batch(L.fp32, L.kind == EnsLaunch::ZINV, L.proto,
+                               (const FftArgs*)(d + L.args_off), first, L.nrun, L.nblocks, st));
+        break;
+      case EnsLaunch::XLINE:
+        LCHK(launch_xline_batch(L.fp32, L.proto, (const FftArgs*)(d + L.args_off), first, L.nrun,
+                                L.nblocks, st));
+        break;
+      case EnsLaunch::THICK: {
+        loma_run* r = L.run;
+        LCHK(launch_thickness(r->M, r->fp32, L.pa, r->hy, r->y[r->my - 1] - r->y[0], L.which, st));
+        break;
+      }
+      case EnsLaunch::FINISH:
+        LCHK(launch_finish_batch((const FinishArgs*)(d + L.args_off), L.nrun, st));
+        break;
+    }
+  }
   return LOMA_OK;
 }
 
@@ -1149,7 +1349,9 @@ static int run_steps(loma_run* r, int nsteps) {
   for (int n = 0; n < nsteps; ++n) {
     // cross.f:238: the step size is re-estimated every ncfl steps
     const int icfl = ((r->istep) % r->ncfl == 0) ? 1 : 0;
-    if (r->use_graph && !r->prof) {
+    // steps of an open statistics window run their launches directly: the
+    // captured step graphs hold no statistics kernels
+    if (r->use_graph && !r->prof && !r->st_on) {
       if (!r->gexec[icfl]) {
         int e = build_graph(r, icfl);
         if (e) return e;
@@ -1246,6 +1448,10 @@ int loma_advance_many(loma_run** runs, int nruns, int nsteps) {
     loma_run* r = runs[q];
     if (!r || !r->uploaded) { set_error("loma_advance_many: run not uploaded"); return LOMA_ERR_STATE; }
     if (r->inflight) { set_error("loma_advance_many: a run has a step in flight"); return LOMA_ERR_STATE; }
+    if (r->st_on) {
+      set_error("loma_advance_many: a run has an open statistics window (advance it alone)");
+      return LOMA_ERR_STATE;
+    }
     for (int p = 0; p < q; ++p)
       if (runs[p] == r) { set_error("loma_advance_many: run listed twice"); return LOMA_ERR_ARG; }
     int e = ensure_stream(r);  // every run needs a stream of its own
@@ -1274,24 +1480,45 @@ int loma_advance_many(loma_run** runs, int nruns, int nsteps) {
         EnsembleGraph c;
         c.runs.assign(runs, runs + nruns);
         c.icfl = icfl;
+        // argument blocks of the batched launches, on the device before the
+        // capture starts (nothing is allocated while it records)
+        EnsPlan plan;
+        ensemble_plan(runs, nruns, icfl, plan);
+        HIPCHK(hipMalloc(&c.desc, plan.bytes.size()));
+        if (hipMemcpy(c.desc, plan.bytes.data(), plan.bytes.size(), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+          hipFree(c.desc);
+          set_error("loma_advance_many: descriptor upload failed");
+          return LOMA_ERR_DEVICE;
+        }
         HIPCHK(hipStreamCreate(&c.master));
+        c.lanes.resize(plan.lanes.size(), nullptr);
+        for (auto& l : c.lanes) HIPCHK(hipStreamCreateWithFlags(&l, hipStreamNonBlocking));
         hipEvent_t fork;
-        std::vector<hipEvent_t> join(nruns);
+        std::vector<hipEvent_t> join(plan.lanes.size());
         HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
         for (auto& e : join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        // one level of fork and join: every lane waits on the ensemble stream
+        // and is joined back to it before the capture ends
         HIPCHK(hipStreamBeginCapture(c.master, hipStreamCaptureModeThreadLocal));
         int err = hipEventRecord(fork, c.master);
-        for (int q = 0; q < nruns && !err; ++q) {
-          err = hipStreamWaitEvent(runs[q]->stream, fork, 0);
-          if (!err) err = enqueue_step(runs[q], icfl[q]);
-          if (!err) err = hipEventRecord(join[q], runs[q]->stream);
-          if (!err) err = hipStreamWaitEvent(c.master, join[q], 0);
+        for (size_t k = 0; k < plan.lanes.size() && !err; ++k) {
+          err = hipStreamWaitEvent(c.lanes[k], fork, 0);
+          if (!err) err = enqueue_lane(c.lanes[k], c.desc, plan.lanes[k]);
+          if (!err) err = hipEventRecord(join[k], c.lanes[k]);
+          if (!err) err = hipStreamWaitEvent(c.master, join[k], 0);
         }
         hipGraph_t g;
         hipError_t ce = hipStreamEndCapture(c.master, &g);
         hipEventDestroy(fork);
         for (auto& e : join) hipEventDestroy(e);
-        if (err) { hipStreamDestroy(c.master); set_error("loma_advance_many: capture failed"); return LOMA_ERR_DEVICE; }
+        if (err) {
+          for (auto& l : c.lanes) hipStreamDestroy(l);
+          hipStreamDestroy(c.master);
+          hipFree(c.desc);
+          set_error("loma_advance_many: capture failed");
+          return LOMA_ERR_DEVICE;
+        }
         HIPCHK(ce);
         hipError_t ie = hipGraphInstantiate(&c.exec, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
@@ -1410,6 +1637,126 @@ int loma_history(loma_run* r, double* out) {
   out[5] = s.mass[0];
   out[6] = s.mass[1];
   out[7] = s.mass[2];
+  return LOMA_OK;
+}
+
+// ---- statistics window ----------------------------------------------------
+int loma_stats_config(loma_run* r, int nspec, const int* jspecy) {
+  if (!r || nspec < 0 || (nspec > 0 && !jspecy)) {
+    set_error("loma_stats_config: bad arguments");
+    return LOMA_ERR_ARG;
+  }
+  if (r->inflight) { set_error("loma_stats_config: a step is in flight"); return LOMA_ERR_STATE; }
+  const int my = r->my;
+  // jsptot: the given rows, then the centre row (my+1)/2 (main.f:698-703)
+  std::vector<int> rows;
+  for (int n = 0; n < nspec; ++n) {
+    if (jspecy[n] < 1 || jspecy[n] > my) {
+      set_error("loma_stats_config: spectra row outside 1..my");
+      return LOMA_ERR_ARG;
+    }
+    rows.push_back(jspecy[n] - 1);
+  }
+  rows.push_back((my + 1) / 2 - 1);
+  std::lock_guard<std::recursive_mutex> lk(g_setup_mu);
+  if (r->stream) HIPCHK(hipStreamSynchronize(r->stream));
+  for (void** p : {(void**)&r->st_acc, (void**)&r->st_spe, (void**)&r->st_part,
+                   (void**)&r->st_rho, (void**)&r->st_last, (void**)&r->st_jsp})
+    if (*p) { hipFree(*p); *p = nullptr; }
+  const int nsp = (int)rows.size();
+  const size_t nacc = (size_t)STAT_NCOL * my;
+  const size_t nspe = (size_t)r->nplanes * 12 * nsp * (r->nz + 1);
+  const size_t npart = (size_t)r->nplanes * STAT_NPART * my;
+  const size_t nrho = (size_t)r->mgalz * my;
+  struct { void** p; size_t bytes; } al[] = {
+      {(void**)&r->st_acc, nacc * 8}, {(void**)&r->st_spe, nspe * 8},
+      {(void**)&r->st_part, npart * 8}, {(void**)&r->st_rho, nrho * 8},
+      {(void**)&r->st_last, (size_t)9 * my * 8}, {(void**)&r->st_jsp, (size_t)nsp * 4}};
+  for (auto& a : al) {
+    if (hipMalloc(a.p, a.bytes) != hipSuccess) {
+      *a.p = nullptr;
+      set_error("loma_stats_config: device allocation failed");
+      return LOMA_ERR_OOM;
+    }
+  }
+  r->st_nsp = nsp;
+  HIPCHK(hipMemcpy(r->st_jsp, rows.data(), (size_t)nsp * 4, hipMemcpyHostToDevice));
+  r->st_on = false;
+  return loma_stats_reset(r);
+}
+
+int loma_stats_accumulate(loma_run* r, int on) {
+  if (!r) return LOMA_ERR_ARG;
+  if (on && !r->st_acc) {
+    set_error("loma_stats_accumulate: no statistics window (loma_stats_config)");
+    return LOMA_ERR_STATE;
+  }
+  r->st_on = on != 0;
+  return LOMA_OK;
+}
+
+int loma_stats_reset(loma_run* r) {
+  if (!r) return LOMA_ERR_ARG;
+  if (!r->st_acc) { set_error("loma_stats_reset: no statistics window"); return LOMA_ERR_STATE; }
+  hipStream_t st = r->stream;
+  const int my = r->my;
+  HIPCHK(hipMemsetAsync(r->st_acc, 0, sizeof(double) * STAT_NCOL * my, st));
+  HIPCHK(hipMemsetAsync(r->st_spe, 0,
+                        sizeof(double) * r->nplanes * 12 * r->st_nsp * (r->nz + 1), st));
+  HIPCHK(hipMemsetAsync(r->st_last, 0, sizeof(double) * 9 * my, st));
+  HIPCHK(hipStreamSynchronize(st));
+  r->st_nacum = 0;
+  return LOMA_OK;
+}
+
+int loma_stats_read(loma_run* r, double* sta, float* spe, double* info) {
+  if (!r) return LOMA_ERR_ARG;
+  if (!r->st_acc) { set_error("loma_stats_read: no statistics window"); return LOMA_ERR_STATE; }
+  if (r->inflight) { set_error("loma_stats_read: a step is in flight"); return LOMA_ERR_STATE; }
+  const int my = r->my;
+  const size_t nspe = (size_t)r->nplanes * 12 * r->st_nsp * (r->nz + 1);
+  std::vector<double> acc((size_t)STAT_NCOL * my), last((size_t)9 * my), sp(spe ? nspe : 0);
+  hipStream_t st = r->stream;
+  HIPCHK(hipMemcpyAsync(acc.data(), r->st_acc, acc.size() * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(last.data(), r->st_last, last.size() * 8, hipMemcpyDeviceToHost, st));
+  if (spe) HIPCHK(hipMemcpyAsync(sp.data(), r->st_spe, nspe * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // writestats (escru.f:366-414): columns 22.. are sums over the x-z plane,
+  // written times 1/(mgalx mgalz)
+  const double fac = 1.0 / ((double)r->mgalx * r->mgalz);
+  if (sta)
+    for (int j = 0; j < my; ++j)
+      for (int c = 0; c < STAT_NCOL; ++c)
+        sta[(size_t)j * STAT_NCOL + c] = acc[(size_t)c * my + j] * (c >= 22 ? fac : 1.0);
+  if (spe)
+    for (size_t q = 0; q < nspe; ++q) spe[q] = (float)sp[q];
+  if (info) {
+    std::vector<double> hy(my);
+    const std::vector<double>& y = r->y;
+    for (int j = 1; j < my - 1; ++j) hy[j] = (y[j + 1] - y[j - 1]) / 2.0;  // main.f:689-696
+    hy[0] = (y[1] - y[0]) / 2.0;
+    hy[my - 1] = (y[my - 1] - y[my - 2]) / 2.0;
+    info[0] = r->st_nacum;
+    info[1] = r->st_nacum;
+    // masst (escru.f:372-376) sums wkstats(j,18) hy(j+1) for j = 0..my-1;
+    // wkstats is declared (my,22), so its j = 0 term reads the element before
+    // column 18 in memory, wkstats(my,17): rww of the top row.  Kept as the
+    // reference computes it, so the .cf column is the same.
+    double masst = fac * acc[(size_t)27 * my + (my - 1)] * hy[0];
+    for (int j = 1; j < my; ++j) masst += fac * acc[(size_t)28 * my + (j - 1)] * hy[j];
+    info[2] = masst;
+    // enerdis = sum_j ep(j)/nacum hy(j) (cross.f:1152-1158)
+    double enerdis = 0.0;
+    if (r->st_nacum)
+      for (int j = 0; j < my; ++j) enerdis += acc[(size_t)21 * my + j] / r->st_nacum * hy[j];
+    info[3] = enerdis;
+    // energy(i) = sqrt(|uner(i)|) of the last accumulated step (cross.f:802-816)
+    for (int n = 0; n < 9; ++n) {
+      double e = 0.0;
+      for (int j = 0; j < my; ++j) e += last[(size_t)n * my + j] * hy[j];
+      info[4 + n] = std::sqrt(std::fabs(e));
+    }
+  }
   return LOMA_OK;
 }
 

@@ -33,9 +33,11 @@ constexpr int B5_THREADS = 2 * B5_SYS;     // real and imaginary wave
 constexpr int B5_PB = B5_SYS * B5_RT * 5 / B5_THREADS;  // band doubles per thread per tile
 constexpr int B5_PR = B5_SYS * B5_RT * 2 / B5_THREADS;  // rhs values per thread per tile
 
+// F32: the right-hand side is complex FP32 (else FP64).  rhs and x may be
+// the same buffer (an FP64 solve in place), so neither is restrict.
+template <bool F32>
 __global__ __launch_bounds__(B5_THREADS) void k_band5(const double* __restrict__ band,
-                                                      const void* __restrict__ rhs, int rhs_fp32,
-                                                      double* __restrict__ x,
+                                                      const void* rhs, double* x,
                                                       double* __restrict__ scr, int n, int nsys) {
   __shared__ double tb[2][B5_SYS][B5_RT * 5 + 1];
   __shared__ double tr[2][B5_SYS][B5_RT * 2 + 1];
@@ -62,7 +64,8 @@ __global__ __launch_bounds__(B5_THREADS) void k_band5(const double* __restrict__
       double v = 0.0;
       if (sy < ns && e < rows * 2) {
         const size_t idx = ((size_t)(s0 + sy) * n + r0) * 2 + e;
-        v = rhs_fp32 ? (double)((const float*)rhs)[idx] : ((const double*)rhs)[idx];
+        if constexpr (F32) v = (double)((const float*)rhs)[idx];
+        else v = ((const double*)rhs)[idx];
       }
       pr[q] = v;
     }
@@ -196,8 +199,9 @@ __global__ __launch_bounds__(B5_THREADS) void k_band5(const double* __restrict__
 int launch_band5(const double* band, const void* rhs, double* x, double* scratch, int n,
                  int nsys, int rhs_fp32, hipStream_t st) {
   if (nsys <= 0 || n <= 0) return 0;
-  hipLaunchKernelGGL(k_band5, dim3((nsys + B5_SYS - 1) / B5_SYS), dim3(B5_THREADS), 0, st,
-                     band, rhs, rhs_fp32, x, scratch, n, nsys);
+  const dim3 g((nsys + B5_SYS - 1) / B5_SYS), b(B5_THREADS);
+  if (rhs_fp32) hipLaunchKernelGGL(k_band5<true>, g, b, 0, st, band, rhs, x, scratch, n, nsys);
+  else hipLaunchKernelGGL(k_band5<false>, g, b, 0, st, band, rhs, x, scratch, n, nsys);
   return (int)hipGetLastError();
 }
 

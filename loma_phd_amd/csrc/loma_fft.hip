@@ -536,83 +536,30 @@ __device__ __forceinline__ void zfft_reg(const T2* __restrict__ src, T2* __restr
 // z; one LDS buffer, in-place FFT (J * mgalz <= 256 * E).
 template <typename T, int E, int NN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE))) void k_zfft_inv(FftArgs a) {
-  using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int N = NN ? NN : a.rc.mgalz, J = a.J, my = a.rc.my, myp = a.rc.myp, mz = a.rc.mz,
-            nz = a.rc.nz;
-  constexpr int PAD = zf_pad<T>();
-  const int RS = lds_row_stride(N, PAD);
-  T2* A = (T2*)smem;
-  const int nyb = (my + J - 1) / J;
-  const int i = blockIdx.x / nyb, j0 = (blockIdx.x - i * nyb) * J;
-  const int f = blockIdx.y;
-  const T2* src = (const T2*)a.spec[f];
-  T2* dst = (T2*)a.wbuf[f];
-  if constexpr (NN > 0 && LOMA_ZF_REG) {
-    // first pass from HBM, last pass to HBM (register-resident ends)
-    zfft_reg<T, T2, E, NN, PAD, true>(src, dst, A, i, j0, J, my, myp, mz, nz,
-                                      (const T2*)a.twz);
-    return;
-  }
-  // the dealiased band nz+1 .. N-nz-1 is zero
-  const int nzero = N - mz;
-  for (int it = threadIdx.x; it < nzero * J; it += blockDim.x) {
-    const int q = it / J, jj = it - q * J;
-    A[jj * RS + lds_i<PAD>(nz + 1 + q)] = T2{0, 0};
-  }
-  for (int it = threadIdx.x; it < mz * J; it += blockDim.x) {
-    const int k = it / J, jj = it - k * J, j = j0 + jj;
-    const int z = (k <= nz) ? k : k + N - mz;
-    T2 v = T2{0, 0};
-    if (j < my) v = src[((size_t)i * mz + k) * myp + j];
-    A[jj * RS + lds_i<PAD>(z)] = v;
-  }
-  __syncthreads();
-  fft_any_ip<T, T2, E, NN, 256, PAD>(A, N, J, a.plz, (const T2*)a.twz, +1);
-  for (int it = threadIdx.x; it < N * J; it += blockDim.x) {
-    const int z = it / J, jj = it - z * J, j = j0 + jj;
-    if (j < my) dst[((size_t)i * N + z) * myp + j] = A[jj * RS + lds_i<PAD>(z)];
-  }
+  const int bx = blockIdx.x, by = blockIdx.y;
+#include "loma_zfft_inv_body.inc"
+}
+template <typename T, int E, int NN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE))) void k_zfft_inv_batch(
+    const FftArgs* __restrict__ args, const int* __restrict__ first, int nrun) {
+  const int r = batch_run(first, nrun, blockIdx.x);
+  const FftArgs& a = args[r];
+  const int bx = blockIdx.x - first[r], by = blockIdx.y;
+#include "loma_zfft_inv_body.inc"
 }
 
 template <typename T, int E, int NN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE))) void k_zfft_fwd(FftArgs a) {
-  using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int N = NN ? NN : a.rc.mgalz, J = a.J, my = a.rc.my, myp = a.rc.myp, mz = a.rc.mz,
-            nz = a.rc.nz;
-  constexpr int PAD = zf_pad<T>();
-  const int RS = lds_row_stride(N, PAD);
-  T2* A = (T2*)smem;
-  const int nyb = (my + J - 1) / J;
-  const int i = blockIdx.x / nyb, j0 = (blockIdx.x - i * nyb) * J;
-  const int f = blockIdx.y;
-  const T2* src = (const T2*)a.wout[f];
-  T2* dst = (T2*)a.spec[f];
-  if constexpr (NN > 0 && LOMA_ZF_REG) {
-    zfft_reg<T, T2, E, NN, PAD, false>(src, dst, A, i, j0, J, my, myp, mz, nz,
-                                       (const T2*)a.twz);
-    return;
-  }
-  for (int it = threadIdx.x; it < N * J; it += blockDim.x) {
-    const int z = it / J, jj = it - z * J, j = j0 + jj;
-    T2 v = T2{0, 0};
-    if (j < my) v = src[((size_t)i * N + z) * myp + j];
-    A[jj * RS + lds_i<PAD>(z)] = v;
-  }
-  __syncthreads();
-  fft_any_ip<T, T2, E, NN, 256, PAD>(A, N, J, a.plz, (const T2*)a.twz, -1);
-  const T sc = (T)(1.0 / (double)N);
-  for (int it = threadIdx.x; it < mz * J; it += blockDim.x) {
-    const int k = it / J, jj = it - k * J, j = j0 + jj;
-    if (j < my) {
-      const int z = (k <= nz) ? k : k + N - mz;
-      T2 v = A[jj * RS + lds_i<PAD>(z)];
-      v.x *= sc;
-      v.y *= sc;
-      dst[((size_t)i * mz + k) * myp + j] = v;
-    }
-  }
+  const int bx = blockIdx.x, by = blockIdx.y;
+#include "loma_zfft_fwd_body.inc"
+}
+template <typename T, int E, int NN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LOMA_ZF_WPE))) void k_zfft_fwd_batch(
+    const FftArgs* __restrict__ args, const int* __restrict__ first, int nrun) {
+  const int r = batch_run(first, nrun, blockIdx.x);
+  const FftArgs& a = args[r];
+  const int bx = blockIdx.x - first[r], by = blockIdx.y;
+#include "loma_zfft_fwd_body.inc"
 }
 
 // ------------------------------------------------------------------------
@@ -975,368 +922,17 @@ __device__ __forceinline__ XlRk xl_rk(double dt, int rkstep) {
 template <typename T, bool WZ, int EPT, int NN, bool XREG>
 __global__ __launch_bounds__(XL_THREADS)
 __attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 : LOMA_XL_WPE)))) void k_xline(FftArgs a) {
-  using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double gln[12][XL_MAXL];   // output estimates per line
-  __shared__ double cflw[XL_THREADS / 64][4];  // CFL maxima per wave
-  const RunConst& rc = a.rc;
-  const int N = NN ? NN : rc.mgalx, my = rc.my;
-  const int pairs = a.L / 2;            // a power of two dividing XL_THREADS
-  const int nyb = (my + a.L - 1) / a.L;
-  const int tile = xcd_tile(blockIdx.x, rc.mgalz * nyb);
-  if (tile >= rc.mgalz * nyb) return;
-  const int z = tile / nyb, j0 = (tile - z * nyb) * a.L;
-  T2* A = (T2*)smem;
-  const int RS = xl_row_stride<NN>(N);
-  constexpr int XL_NF = xl_nf<T>();
-  constexpr int XP = xl_pad<T>();
-  static_assert(!XREG || (NN == 512 && EPT == 8 && XL_NF == 1 && XP == 0),
-                "register-resident ends: FP32 rows of 512 points, one field per transform");
-  T2* B = A + lds_entries(XL_NF * pairs * RS, XP);
-  constexpr int NIN = WZ ? 12 : 8;
-  const double ipe = 1.0 / rc.peclet;
-  const XlRk k = xl_rk(a.sc->dt, a.rkstep);
-  const int rnew = a.rkstep & 1, rold = (a.rkstep - 1) & 1;
-  // thread -> one line pair p (lines 2p, 2p+1) and entries x = x0 + TP*m
-  const int TP = XL_THREADS / pairs;
-  const int p = threadIdx.x / TP, x0 = threadIdx.x - p * TP;
-  const int jl[2] = {j0 + 2 * p, j0 + 2 * p + 1};
-  auto prof = [&](int h, int q) __attribute__((always_inline)) -> double {  // y-profile q of line h (0 past my)
-    return (jl[h] < my && q < NIN) ? a.mean_in[(size_t)q * my + jl[h]] : 0.0;
-  };
-  // output estimates per line (one thread per line)
-  if (threadIdx.x < a.L) {
-    const int l = threadIdx.x, j = j0 + l;
-    const bool in = j < my;
-    double m[12];
-#pragma unroll
-    for (int q = 0; q < 12; ++q) m[q] = (in && q < NIN) ? a.mean_in[(size_t)q * my + j] : 0.0;
-    const double mu = m[0], mv = m[1], mw = m[2], mT = in ? m[7] : 1.0;
-    double g[12];
-    g[0] = mu * mT; g[1] = mv * mT; g[2] = mw * mT;
-    g[3] = mu * (mu * mT); g[4] = mv * (mv * mT); g[5] = mw * (mw * mT);
-    g[6] = mu * (mv * mT); g[7] = mu * (mw * mT); g[8] = mv * (mw * mT);
-    // RHS of T on the profiles (cross.f:1631-1637) and what it makes of
-    // dT and drho (cross.f:1669-1673)
-    const double rb = mT * (m[3] * ipe - (mu * m[4] + mv * m[5] + mw * m[6]));
-    const double dTg = k.dtxi * (in ? a.rbar[(size_t)rold * my + j] : 0.0) + k.dtgamma * rb;
-    g[10] = dTg;
-    g[9] = (dTg / (mT * (mT + dTg))) / k.dtbeta - k.ab * rb / (mT * mT);
-    double rzb = 0.0;
-    g[11] = 0.0;
-    if (WZ) {
-      rzb = mT * (m[11] * ipe - (mu * m[8] + mv * m[9] + mw * m[10]));
-      g[11] = k.dtxi * (in ? a.rzbar[(size_t)rold * my + j] : 0.0) + k.dtgamma * rzb;
-    }
-#pragma unroll
-    for (int q = 0; q < 12; ++q) gln[q][l] = g[q];
-    if (z == 0 && in) {
-#pragma unroll
-      for (int q = 0; q < 12; ++q) a.mean_out[(size_t)q * my + j] = g[q];
-      a.rbar[(size_t)rnew * my + j] = rb;
-      if (WZ) a.rzbar[(size_t)rnew * my + j] = rzb;
-    }
-  }
-  __syncthreads();
-  // XR items of the np * pairs input coefficients per thread
-  constexpr int XR = (EPT + 2) / 3;
-  auto ent = [&](int m) __attribute__((always_inline)) { return x0 + TP * m; };  // x of entry m (valid if < N)
-  // this thread's entries of field f's rows (no barrier)
-  auto grab_at = [&](const T2* R, int f, T (&v)[EPT][2]) __attribute__((always_inline)) {
-#pragma unroll
-    for (int m = 0; m < EPT; ++m) {
-      const int x = ent(m);
-      T2 q = T2{0, 0};
-      if (x < N) q = R[lds_i<XP>((f * pairs + p) * RS + x)];
-      v[m][0] = q.x;
-      v[m][1] = q.y;
-    }
-  };
-  auto put_at = [&](int f, const T (&v)[EPT][2]) __attribute__((always_inline)) {
-#pragma unroll
-    for (int m = 0; m < EPT; ++m) {
-      const int x = ent(m);
-      if (x < N) A[lds_i<XP>((f * pairs + p) * RS + x)] = T2{v[m][0], v[m][1]};
-    }
-  };
-  // physical lines of fields s0 and s1 into v0 and v1
-  auto inv2 = [&](const T2* s0, const T2* s1, T (&v0)[EPT][2], T (&v1)[EPT][2]) __attribute__((always_inline)) {
-    if constexpr (XL_NF == 2) {
-      const T2* R = xl_inverse<T, T2, XR, NN, 2>(a, s0, s1, A, B, z, j0, pairs);
-      grab_at(R, 0, v0);
-      grab_at(R, 1, v1);
-      __syncthreads();
-    } else if constexpr (XREG) {
-      xl_inverse_reg<T, T2, XR>(a, s0, A, B, z, j0, pairs, v0);
-      xl_inverse_reg<T, T2, XR>(a, s1, A, B, z, j0, pairs, v1);
-    } else {
-      const T2* R = xl_inverse<T, T2, XR, NN, 1>(a, s0, nullptr, A, B, z, j0, pairs);
-      grab_at(R, 0, v0);
-      __syncthreads();
-      R = xl_inverse<T, T2, XR, NN, 1>(a, s1, nullptr, A, B, z, j0, pairs);
-      grab_at(R, 0, v1);
-      __syncthreads();
-    }
-  };
-  auto inv1 = [&](const T2* s0, T (&v0)[EPT][2]) __attribute__((always_inline)) {
-    if constexpr (XREG) {
-      xl_inverse_reg<T, T2, XR>(a, s0, A, B, z, j0, pairs, v0);
-    } else {
-      const T2* R = xl_inverse<T, T2, XR, NN, 1>(a, s0, nullptr, A, B, z, j0, pairs);
-      grab_at(R, 0, v0);
-      __syncthreads();
-    }
-  };
-  // forward transform of one output held by this thread at its entries
-  auto fwd1 = [&](T2* dst, const T (&v)[EPT][2]) __attribute__((always_inline)) {
-    if constexpr (XREG) {
-      xl_forward_reg<T, T2>(a, dst, A, B, z, j0, pairs, v);
-    } else {
-      put_at(0, v);
-      xl_forward<T, T2, NN, 1>(a, dst, nullptr, A, B, z, j0, pairs);
-    }
-  };
-  T2* const* wb = (T2* const*)a.wbuf;
-  T2* const* wo = (T2* const*)a.wout;
-
-  // ---- pass 1: T, rho u_i -> u_i, rho u_i u_j; CFL maxima ----
-  T tt[EPT][2], ru[EPT][2], rv[EPT][2], rw[EPT][2];  // perturbations
-  inv2(wb[7], wb[0], tt, ru);
-  inv2(wb[1], wb[2], rv, rw);
-  // profiles of rho u, rho v, rho w, T for the two lines
-  double pu[2], pv[2], pw[2], pT[2];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    pu[h] = prof(h, 0); pv[h] = prof(h, 1); pw[h] = prof(h, 2);
-    pT[h] = jl[h] < my ? prof(h, 7) : 1.0;
-  }
-  if (a.do_cfl) {
-    double cx = 0.0, cy = 0.0, cz = 0.0, cd = 0.0;
-    const double hx = rc.hxalp * 0.5, hz = rc.hzbet * 0.5;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      if (jl[h] >= my) continue;
-      const double ihy = 1.0 / a.hy[jl[h]];
-#pragma unroll
-      for (int m = 0; m < EPT; ++m) {
-        if (ent(m) >= N) continue;
-        const double Tt = (double)tt[m][h] + pT[h];
-        cx = max_nan(cx, fabs(((double)ru[m][h] + pu[h]) * Tt));
-        cy = max_nan(cy, fabs(((double)rv[m][h] + pv[h]) * Tt) * ihy);  // cross.f:1705-1707
-        cz = max_nan(cz, fabs(((double)rw[m][h] + pw[h]) * Tt));
-        cd = max_nan(cd, 2.0 * (ipe * Tt) * (hx * hx + hz * hz + ihy * ihy));
-      }
-    }
-    for (int d = 32; d >= 1; d >>= 1) {
-      cx = max_nan(cx, __shfl_xor(cx, d));
-      cy = max_nan(cy, __shfl_xor(cy, d));
-      cz = max_nan(cz, __shfl_xor(cz, d));
-      cd = max_nan(cd, __shfl_xor(cd, d));
-    }
-    // one partial per tile, plain stores (k_finish_step reduces them): the
-    // same-address atomics this replaces serialised on one cache line and
-    // cost 3.1 ms per FP64 launch at 512x513x256
-    if ((threadIdx.x & 63) == 0) {
-      const int w = threadIdx.x >> 6;
-      cflw[w][0] = cx; cflw[w][1] = cy; cflw[w][2] = cz; cflw[w][3] = cd;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-      for (int w = 1; w < XL_THREADS / 64; ++w) {
-        cx = max_nan(cx, cflw[w][0]);
-        cy = max_nan(cy, cflw[w][1]);
-        cz = max_nan(cz, cflw[w][2]);
-        cd = max_nan(cd, cflw[w][3]);
-      }
-      double2* o = (double2*)(a.cflp + 4 * (size_t)tile);
-      o[0] = double2{cx, cy};
-      o[1] = double2{cz, cd};
-    }
-  }
-  // outputs 0 u 1 v 2 w 3 ruu 4 rvv 5 rww 6 ruv 7 ruw 8 rvw  (cross.f:1613-1651)
-  // minus the product of the profiles, expanded exactly so that every term
-  // carries a perturbation factor:  (P+p)(Q+q)(R+r) - PQR
-  //   = p (Q+q)(R+r) + P (q (R+r) + Q r)
-  auto values = [&](int q, T (&out)[EPT][2]) __attribute__((always_inline)) {  // output q at this thread's entries
-#pragma unroll
-    for (int m = 0; m < EPT; ++m) {
-      const int x = ent(m);
-      out[m][0] = out[m][1] = (T)0;
-      if (x >= N) continue;
-      T val[2];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const T A_ = (T)pu[h], B_ = (T)pv[h], C_ = (T)pw[h], P_ = (T)pT[h];
-        const T a_ = ru[m][h], b_ = rv[m][h], c_ = rw[m][h], t_ = tt[m][h];
-        const T TT = P_ + t_;
-        T v;
-        switch (q) {
-          case 0: v = a_ * TT + A_ * t_; break;
-          case 1: v = b_ * TT + B_ * t_; break;
-          case 2: v = c_ * TT + C_ * t_; break;
-          case 3: v = a_ * (A_ + a_) * TT + A_ * (a_ * TT + A_ * t_); break;
-          case 4: v = b_ * (B_ + b_) * TT + B_ * (b_ * TT + B_ * t_); break;
-          case 5: v = c_ * (C_ + c_) * TT + C_ * (c_ * TT + C_ * t_); break;
-          case 6: v = a_ * (B_ + b_) * TT + A_ * (b_ * TT + B_ * t_); break;
-          case 7: v = a_ * (C_ + c_) * TT + A_ * (c_ * TT + C_ * t_); break;
-          default: v = b_ * (C_ + c_) * TT + B_ * (c_ * TT + C_ * t_); break;
-        }
-        val[h] = v;
-      }
-      out[m][0] = val[0];
-      out[m][1] = val[1];
-    }
-  };
-  auto fill = [&](int q, int f) __attribute__((always_inline)) {  // output q into the rows of field f
-    T out[EPT][2];
-    values(q, out);
-    put_at(f, out);
-  };
-#pragma unroll 1
-  for (int q = 0; q < 8; q += 2) {
-    opaque_regs(tt);
-    opaque_regs(ru);
-    opaque_regs(rv);
-    opaque_regs(rw);
-    if constexpr (XL_NF == 2) {
-      fill(q, 0);
-      fill(q + 1, 1);
-      xl_forward<T, T2, NN, 2>(a, wo[q], wo[q + 1], A, B, z, j0, pairs);
-    } else {
-      T out[EPT][2];
-      values(q, out);
-      fwd1(wo[q], out);
-      values(q + 1, out);
-      fwd1(wo[q + 1], out);
-    }
-  }
-  {
-    T out[EPT][2];
-    values(8, out);
-    fwd1(wo[8], out);
-  }
-
-  // ---- pass 2: r = T (lap T / Pe - rho u . grad T)  (cross.f:1652-1673) ----
-  // acc holds the perturbation part; the profile part is per line
-  T acc[EPT][2], accz[EPT][2];
-  // gradient terms: T_x (4), T_y (5), T_z (6); Z_x (8), Z_y (9), Z_z (10):
-  // (R + r)(G + g) - R G = (R + r) g + r G
-  auto grad_term = [&](int c, int gq, const T (&fl)[EPT][2], T (&ac)[EPT][2]) __attribute__((always_inline)) {
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const T R_ = (T)(c == 0 ? pu[h] : (c == 1 ? pv[h] : pw[h])), G_ = (T)prof(h, gq);
-#pragma unroll
-      for (int m = 0; m < EPT; ++m) {
-        const T rc_ = c == 0 ? ru[m][h] : (c == 1 ? rv[m][h] : rw[m][h]);
-        ac[m][h] -= (rc_ + R_) * fl[m][h] + rc_ * G_;
-      }
-    }
-  };
-  if constexpr (XL_NF == 2 && !WZ) {
-    T f[EPT][2], g[EPT][2];
-    inv2(wb[3], wb[4], f, g);  // lap T, T_x
-#pragma unroll
-    for (int m = 0; m < EPT; ++m)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) acc[m][h] = f[m][h] * (T)ipe;
-    grad_term(0, 4, g, acc);
-    opaque_regs(ru);
-    opaque_regs(rv);
-    opaque_regs(rw);
-    inv2(wb[5], wb[6], f, g);  // T_y, T_z
-    grad_term(1, 5, f, acc);
-    grad_term(2, 6, g, acc);
-  } else if constexpr (XL_NF == 2) {
-    T f[EPT][2], g[EPT][2];
-    inv2(wb[3], wb[11], f, g);  // lap T, lap Z
-#pragma unroll
-    for (int m = 0; m < EPT; ++m)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        acc[m][h] = f[m][h] * (T)ipe;
-        accz[m][h] = g[m][h] * (T)ipe;
-      }
-#pragma unroll 1
-    for (int c = 0; c < 3; ++c) {
-      opaque_regs(ru);
-      opaque_regs(rv);
-      opaque_regs(rw);
-      inv2(wb[4 + c], wb[8 + c], f, g);  // T_c, Z_c
-      grad_term(c, 4 + c, f, acc);
-      grad_term(c, 8 + c, g, accz);
-    }
-  } else {
-    T f[EPT][2];
-    inv1(wb[3], f);  // lap T
-#pragma unroll
-    for (int m = 0; m < EPT; ++m)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) acc[m][h] = f[m][h] * (T)ipe;
-    if constexpr (WZ) {
-      inv1(wb[11], f);  // lap Z
-#pragma unroll
-      for (int m = 0; m < EPT; ++m)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) accz[m][h] = f[m][h] * (T)ipe;
-    }
-#pragma unroll 1
-    for (int c = 0; c < 3; ++c) {
-      opaque_regs(ru);
-      opaque_regs(rv);
-      opaque_regs(rw);
-      inv1(wb[4 + c], f);  // T_x, T_y, T_z
-      grad_term(c, 4 + c, f, acc);
-      if constexpr (WZ) {
-        inv1(wb[8 + c], f);  // Z_x, Z_y, Z_z
-        grad_term(c, 8 + c, f, accz);
-      }
-    }
-  }
-  // time advance of T in physical space (rhst history, FP64); outputs minus
-  // their profile estimates.  ru <- DT, rv <- drho, rw <- DZ.
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int j = jl[h];
-    const double mT = pT[h];
-    const double accm = prof(h, 3) * ipe -
-                        (pu[h] * prof(h, 4) + pv[h] * prof(h, 5) + pw[h] * prof(h, 6));
-    const double acczm = WZ ? prof(h, 11) * ipe - (pu[h] * prof(h, 8) + pv[h] * prof(h, 9) +
-                                                   pw[h] * prof(h, 10))
-                            : 0.0;
-    const double g9 = gln[9][2 * p + h], g10 = gln[10][2 * p + h], g11 = gln[11][2 * p + h];
-#pragma unroll
-    for (int m = 0; m < EPT; ++m) {
-      const int x = ent(m);
-      ru[m][h] = rv[m][h] = rw[m][h] = (T)0;
-      if (x >= N || j >= my) continue;
-      const double Tt = (double)tt[m][h] + mT;
-      const double r = Tt * (accm + (double)acc[m][h]);
-      const size_t ri = ((size_t)z * my + j) * N + x;
-      const double rold_ = a.rhst[ri];
-      a.rhst[ri] = r;
-      const double d = k.dtxi * rold_ + k.dtgamma * r;
-      const double tnext = Tt + d;
-      ru[m][h] = (T)(d - g10);
-      rv[m][h] = (T)((d / (Tt * tnext)) / k.dtbeta - k.ab * r / (Tt * Tt) - g9);
-      if constexpr (WZ) {
-        const double rz = Tt * (acczm + (double)accz[m][h]);
-        const double zold = a.rhsz[ri];
-        a.rhsz[ri] = rz;
-        rw[m][h] = (T)(k.dtxi * zold + k.dtgamma * rz - g11);
-      }
-    }
-  }
-  // outputs 9 drho, 10 DT, 11 DZ
-  if constexpr (XL_NF == 2) {
-    put_at(0, rv);
-    put_at(1, ru);
-    xl_forward<T, T2, NN, 2>(a, wo[9], wo[10], A, B, z, j0, pairs);
-  } else {
-    fwd1(wo[9], rv);
-    fwd1(wo[10], ru);
-  }
-  if constexpr (WZ) fwd1(wo[11], rw);
+  const int bx = blockIdx.x;
+#include "loma_xline_body.inc"
+}
+template <typename T, bool WZ, int EPT, int NN, bool XREG>
+__global__ __launch_bounds__(XL_THREADS)
+__attribute__((amdgpu_waves_per_eu(XL_THREADS == 512 ? 4 : (sizeof(T) == 8 ? 2 : LOMA_XL_WPE)))) void k_xline_batch(
+    const FftArgs* __restrict__ args, const int* __restrict__ first, int nrun) {
+  const int r = batch_run(first, nrun, blockIdx.x);
+  const FftArgs& a = args[r];
+  const int bx = blockIdx.x - first[r];
+#include "loma_xline_body.inc"
 }
 
 // fourz on standalone rows (FFTPACK convention), for the per-operator API
@@ -1447,6 +1043,69 @@ int launch_xline(bool fp32, const FftArgs& a, hipStream_t st) {
       LOMA_BY_LENGTH(a.rc.mgalx, (launch_xline_nn<double, true, E64, NN>(a, g, b, lds, st)));
     } else {
       LOMA_BY_LENGTH(a.rc.mgalx, (launch_xline_nn<double, false, E64, NN>(a, g, b, lds, st)));
+    }
+  }
+  return (int)hipGetLastError();
+}
+
+// ---- batched launches (one class of runs: same length, tiles, fields) ----
+int zfft_blocks(const FftArgs& a) { return a.rc.nplanes * ((a.rc.my + a.J - 1) / a.J); }
+int xline_blocks(const FftArgs& a) {
+  return xcd_groups(a.rc.mgalz * ((a.rc.my + a.L - 1) / a.L));
+}
+
+int launch_zfft_batch(bool fp32, bool inv, const FftArgs& proto, const FftArgs* args,
+                      const int* first, int nrun, int nblocks, hipStream_t st) {
+  if (nblocks <= 0 || nrun <= 0) return 0;
+  dim3 g(nblocks, proto.nfields), b(256);
+  const size_t lds = zfft_lds(fp32, proto);
+  if (fp32) {
+    if (inv) {
+      LOMA_BY_LENGTH(proto.rc.mgalz, hipLaunchKernelGGL((k_zfft_inv_batch<float, LOMA_ZF_E32, NN>), g, b, lds, st, args, first, nrun));
+    } else {
+      LOMA_BY_LENGTH(proto.rc.mgalz, hipLaunchKernelGGL((k_zfft_fwd_batch<float, LOMA_ZF_E32, NN>), g, b, lds, st, args, first, nrun));
+    }
+  } else {
+    if (inv) {
+      LOMA_BY_LENGTH(proto.rc.mgalz, hipLaunchKernelGGL((k_zfft_inv_batch<double, LOMA_ZF_E64, NN>), g, b, lds, st, args, first, nrun));
+    } else {
+      LOMA_BY_LENGTH(proto.rc.mgalz, hipLaunchKernelGGL((k_zfft_fwd_batch<double, LOMA_ZF_E64, NN>), g, b, lds, st, args, first, nrun));
+    }
+  }
+  return (int)hipGetLastError();
+}
+
+template <typename T, bool WZ, int E, int NN>
+static void launch_xline_batch_nn(const FftArgs& proto, dim3 g, dim3 b, size_t lds,
+                                  hipStream_t st, const FftArgs* args, const int* first,
+                                  int nrun) {
+  if constexpr (sizeof(T) == 4 && NN == 512 && E == 8 && LOMA_XL_REG32 && xl_pad<T>() == 0 &&
+                xl_nf<T>() == 1) {
+    if (proto.L == 8) {
+      hipLaunchKernelGGL((k_xline_batch<T, WZ, E, NN, true>), g, b, lds, st, args, first, nrun);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_xline_batch<T, WZ, E, NN, false>), g, b, lds, st, args, first, nrun);
+}
+
+int launch_xline_batch(bool fp32, const FftArgs& proto, const FftArgs* args, const int* first,
+                       int nrun, int nblocks, hipStream_t st) {
+  if (nblocks <= 0 || nrun <= 0) return 0;
+  dim3 g(nblocks), b(XL_THREADS);
+  const size_t lds = xline_lds(fp32, proto);
+  constexpr int E64 = 1024 / XL_THREADS, E32 = 2048 / XL_THREADS;
+  if (fp32) {
+    if (proto.with_z) {
+      LOMA_BY_LENGTH(proto.rc.mgalx, (launch_xline_batch_nn<float, true, E32, NN>(proto, g, b, lds, st, args, first, nrun)));
+    } else {
+      LOMA_BY_LENGTH(proto.rc.mgalx, (launch_xline_batch_nn<float, false, E32, NN>(proto, g, b, lds, st, args, first, nrun)));
+    }
+  } else {
+    if (proto.with_z) {
+      LOMA_BY_LENGTH(proto.rc.mgalx, (launch_xline_batch_nn<double, true, E64, NN>(proto, g, b, lds, st, args, first, nrun)));
+    } else {
+      LOMA_BY_LENGTH(proto.rc.mgalx, (launch_xline_batch_nn<double, false, E64, NN>(proto, g, b, lds, st, args, first, nrun)));
     }
   }
   return (int)hipGetLastError();

@@ -109,12 +109,37 @@ struct FftArgs {
   // CFL maxima of each x-line tile, [tile][4]: max |u|, max |v|/hy, max |w|,
   // max diffusive inverse dt (written by the substep that takes the CFL)
   double* cflp;
+  // per-line sums of 1/T over x, [mgalz][my] (statistics steps only; null
+  // otherwise)
+  double* stat_rho;
   int nfields;
   int rkstep;
   int do_cfl;
   int with_z;
   int J;                     // y rows per z-transform block
   int L;                     // lines per x-line block
+};
+
+// statistics window (addstats1/2 and the phys2 sums of cross.f): the
+// accumulated columns, in the order of a .sta record, are
+//   0 um 1 vm 2 wm 3 rum 4 rvm 5 rwm 6 Tm 7 w1m 8 w2m 9 w3m 10 them
+//   11 up 12 vp 13 wp 14 uvr 15 uwr 16 vwr 17 w1p 18 w2p 19 w3p 20 Tp 21 ep
+//   22 ruu 23 ruv 24 ruw 25 rvv 26 rvw 27 rww 28 rhom 29 thep 30 theup
+//   31 mum 32 num
+constexpr int STAT_NCOL = 33;
+constexpr int STAT_NPART = 13;  // per-plane partial sums: Tp, then the 12 of addstats2
+struct StatsArgs {
+  RunConst rc;
+  YTab T;
+  const void* spec[12];
+  const double *mean_in, *mean_out;
+  const int* jsp;            // spectra rows (0-based)
+  int nsp;
+  double* acc;               // [STAT_NCOL][my]
+  double* spe;               // [nplanes][12][nsp][nz+1]
+  double* part;              // [nplanes][STAT_NPART][my], this step's sums per kx plane
+  const double* rho;         // [mgalz][my] line sums of 1/T from the x lines
+  double* last;              // [9][my] this step's plane-summed intensities (uner)
 };
 
 // launchers (return hipError_t as int)
@@ -150,6 +175,43 @@ __host__ __device__ inline int xl_tiles(int my, int mgalz, int L) {
 __host__ __device__ inline int cfl_tiles_max(int my, int mgalz) { return xl_tiles(my, mgalz, 2); }
 int launch_finish_step(DevScalars* sc, RunConst rc, int icfl, const double* cflp, int ntile,
                        hipStream_t st);
+// which = 1: addstats1 on the transform inputs (after phase A); 2: addstats2
+// on the outputs (after phase C2); 3: the reduction of the step's partial
+// sums into the accumulated columns
+int launch_stats(int M, bool fp32, int which, const StatsArgs& a, hipStream_t st);
+
+// ---- batched launches over several runs (ensembles) ----------------------
+// One launch serves a class of runs that share a kernel instantiation: the
+// launch's blocks are the concatenation of each run's own grid, run r
+// owning blocks first[r] .. first[r+1]-1, and the kernel reads run r's
+// argument block from device memory (args[r]).
+struct FinishArgs {
+  DevScalars* sc;
+  RunConst rc;
+  const double* cflp;
+  int icfl;
+  int ntile;
+};
+__device__ __forceinline__ int batch_run(const int* __restrict__ first, int nrun, int b) {
+  int r = 0;
+  for (int q = 1; q < nrun; ++q)
+    if (b >= first[q]) r = q;
+  return r;
+}
+// which: 0 phase A, 1 phase C, 2 phase C2 (all runs of one M and precision)
+int launch_phase_batch(int M, bool fp32, int which, const PhaseArgs* args, const int* first,
+                       int nrun, int nblocks, hipStream_t st);
+// proto: host copy of one run's arguments (length, tiles, fields of the class)
+int launch_zfft_batch(bool fp32, bool inv, const FftArgs& proto, const FftArgs* args,
+                      const int* first, int nrun, int nblocks, hipStream_t st);
+int launch_xline_batch(bool fp32, const FftArgs& proto, const FftArgs* args, const int* first,
+                       int nrun, int nblocks, hipStream_t st);
+int launch_finish_batch(const FinishArgs* args, int nrun, hipStream_t st);
+// blocks of one run's launch of each kernel (the grid the single-run
+// launchers use)
+int phase_blocks(int which, const RunConst& rc);
+int zfft_blocks(const FftArgs& a);
+int xline_blocks(const FftArgs& a);
 
 int launch_zfft_inv(bool fp32, const FftArgs& a, hipStream_t st);
 int launch_xline(bool fp32, const FftArgs& a, hipStream_t st);

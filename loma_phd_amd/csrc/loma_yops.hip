@@ -26,6 +26,10 @@ template <int M>
 int launch_ic_m(const IcArgs&, hipStream_t);
 template <int M>
 int launch_thickness_m(bool, const PhaseArgs&, const double*, double, int, hipStream_t);
+template <int M>
+int launch_stats_m(bool, int, const StatsArgs&, hipStream_t);
+template <int M>
+int launch_phase_batch_m(int, bool, const PhaseArgs*, const int*, int, int, hipStream_t);
 
 RkCoef rk_coefficients() {
   RkCoef r;
@@ -98,6 +102,17 @@ int launch_phaseC2(int M, bool fp32, const PhaseArgs& a, hipStream_t st) {
   if (a.nmodes <= 0) return 0;
   LOMA_DISPATCH_M(M, launch_phase_m<MM>(2, fp32, a, st));
 }
+int launch_phase_batch(int M, bool fp32, int which, const PhaseArgs* args, const int* first,
+                       int nrun, int nblocks, hipStream_t st) {
+  if (nblocks <= 0 || nrun <= 0) return 0;
+  LOMA_DISPATCH_M(M, launch_phase_batch_m<MM>(which, fp32, args, first, nrun, nblocks, st));
+}
+// the grids of launch_phase_m: 4 wavefronts (modes, or Helmholtz classes
+// for phase C2) per 256-thread block
+int phase_blocks(int which, const RunConst& rc) {
+  const int n = which == 2 ? rc.nplanes * (rc.nz + 1) : rc.nplanes * rc.mz;
+  return (n + 3) / 4;
+}
 int launch_ic(int M, const IcArgs& a, hipStream_t st) {
   LOMA_DISPATCH_M(M, launch_ic_m<MM>(a, st));
 }
@@ -164,9 +179,8 @@ int launch_helm_factor(const double* dt21, const double* dt22, int my, int M,
 // 1769-1818) from the maxima of the x-line tiles, reduced here by one
 // workgroup.  NaN flag from the CFL maxima.
 constexpr int FINISH_THREADS = 1024;
-__global__ __launch_bounds__(FINISH_THREADS) void k_finish_step(DevScalars* sc, RunConst rc,
-                                                                 int icfl, const double* cflp,
-                                                                 int ntile) {
+__device__ __forceinline__ void finish_body(DevScalars* sc, const RunConst& rc, int icfl,
+                                            const double* cflp, int ntile) {
   __shared__ double red[FINISH_THREADS / 64][4];
   double c[4] = {0.0, 0.0, 0.0, 0.0};
   if (icfl) {
@@ -200,7 +214,51 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_finish_step(DevScalars* sc, 
     sc->dt = dtr;
   }
 }
+__global__ __launch_bounds__(FINISH_THREADS) void k_finish_step(DevScalars* sc, RunConst rc,
+                                                                 int icfl, const double* cflp,
+                                                                 int ntile) {
+  finish_body(sc, rc, icfl, cflp, ntile);
+}
+// one workgroup per run
+__global__ __launch_bounds__(FINISH_THREADS) void k_finish_batch(const FinishArgs* args) {
+  const FinishArgs& f = args[blockIdx.x];
+  finish_body(f.sc, f.rc, f.icfl, f.cflp, f.ntile);
+}
+int launch_finish_batch(const FinishArgs* args, int nrun, hipStream_t st) {
+  if (nrun <= 0) return 0;
+  hipLaunchKernelGGL(k_finish_batch, dim3(nrun), dim3(FINISH_THREADS), 0, st, args);
+  return (int)hipGetLastError();
+}
 
+
+// The step's per-plane sums into the accumulated columns, planes added in
+// order (one thread per y row): Tp, then up, vp, wp, uvr, uwr, vwr, w1p,
+// w2p, w3p, thep, theup, ep; the line sums of 1/T over z (rhom).  The
+// plane-summed intensities of this step are kept for uner (cross.f:802-816).
+__global__ __launch_bounds__(256) void k_stats_reduce(StatsArgs a) {
+  const int my = a.rc.my;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= my) return;
+  constexpr int col[STAT_NPART] = {20, 11, 12, 13, 14, 15, 16, 17, 18, 19, 29, 30, 21};
+#pragma unroll 1
+  for (int q = 0; q < STAT_NPART; ++q) {
+    double s = 0.0;
+    for (int i = 0; i < a.rc.nplanes; ++i) s += a.part[((size_t)i * STAT_NPART + q) * my + j];
+    a.acc[(size_t)col[q] * my + j] += s;
+    if (q >= 1 && q <= 9) a.last[(size_t)(q - 1) * my + j] = s;
+  }
+  double r = 0.0;
+  for (int z = 0; z < a.rc.mgalz; ++z) r += a.rho[(size_t)z * my + j];
+  a.acc[(size_t)28 * my + j] += r;
+}
+
+int launch_stats(int M, bool fp32, int which, const StatsArgs& a, hipStream_t st) {
+  if (which == 3) {
+    hipLaunchKernelGGL(k_stats_reduce, dim3((a.rc.my + 255) / 256), dim3(256), 0, st, a);
+    return (int)hipGetLastError();
+  }
+  LOMA_DISPATCH_M(M, launch_stats_m<MM>(fp32, which, a, st));
+}
 
 int launch_finish_step(DevScalars* sc, RunConst rc, int icfl, const double* cflp, int ntile,
                        hipStream_t st) {

@@ -457,344 +457,30 @@ __device__ __forceinline__ void scalar_derivs(const PhaseArgs& a, int lane, doub
 
 template <int M, typename TS>
 __global__ __launch_bounds__(256, 2) void k_phaseA(PhaseArgs a) {
-  using T2 = typename std::conditional<sizeof(TS) == 4, float2, double2>::type;
-  LOMA_STAGE2(M);
-  const RunConst& rc = a.rc;
-  const int seq = wave_uniform(a.mode_base + blockIdx.x * 4 + (threadIdx.x >> 6));
-  const int lane = threadIdx.x & 63;
-  if (seq >= a.mode_base + a.nmodes) return;
-  const int w = mode_seq(rc, seq);
-  int i, k;
-  double kx, kz;
-  mode_of(rc, w, i, k, kx, kz);
-  const int my = rc.my, r0 = lane * M;
-  const double rK = kx * kx + kz * kz;
-  const double dt = a.sc->dt;
-  const double dtxi = dt * c_xi[a.rkstep - 1];
-  const size_t o = (size_t)w * my, os = (size_t)w * rc.myp;
-  const double* lu = a.helm_lu + (size_t)helm_class(rc, i, k) * a.lu_stride;
-  const bool m00 = (i == 0 && k == 0);
-  T2* const* sp = (T2* const*)a.spec;
-
-  // (1) phi BCs, phi** = phi + dtxi RHS(phi)^(n-1), Lapvdvhom -> m_y (v,
-  //     parked), dm_y/dy (dv)                         cross.f:328-372, 420-430
-  double dv[2][M];
-  {
-    double f[2][M], v[2][M];
-    load_col<M>(a.phi + o, my, lane, st, f);
-#pragma unroll
-    for (int t = 0; t < M; ++t)
-      if (r0 + t == 0 || r0 + t == my - 1) f[0][t] = f[1][t] = 0.0;
-    stage_in<M>(a.ten13 + o, my, lane, st);
-    stage_axpy<M>(st, lane, f, dtxi);
-    stage_out<M>(st, my, lane, a.phi + o);
-    if (rK == 0.0) {
-#pragma unroll
-      for (int t = 0; t < M; ++t) v[0][t] = v[1][t] = dv[0][t] = dv[1][t] = 0.0;
-    } else {
-      op_helm<M, 2>(a.T, lu, lane, f, v);
-      op_dy1<M, 2>(a.T, lane, v, dv);
-    }
-    stage_put<M>(pk, lane, v);
-  }
-  // (2) vor BCs, m_x, m_z, vor** = vor + dtxi RHS(vor)^(n-1)
-  //                                                   cross.f:315-320, 393-417
-  double mx[2][M], mz[2][M];
-  {
-    double om[2][M];
-    load_col<M>(a.vor + o, my, lane, st, om);
-    const double b0r = col_value<M>(om[0], 1), b0i = col_value<M>(om[1], 1);
-    const double b1r = col_value<M>(om[0], my - 2), b1i = col_value<M>(om[1], my - 2);
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      if (r0 + t == 0) { om[0][t] = b0r; om[1][t] = b0i; }
-      if (r0 + t == my - 1) { om[0][t] = b1r; om[1][t] = b1i; }
-    }
-    stage_in<M>(a.ten12 + o, my, lane, st);
-    stage_axpy<M>(st, lane, om, dtxi);
-    stage_out<M>(st, my, lane, a.vor + o);
-    const double ta = (rK != 0.0) ? kx / rK : 0.0, tb = (rK != 0.0) ? kz / rK : 0.0;
-    // m_x = dv*(i ta) - om*(i tb); m_z = dv*(i tb) + om*(i ta)
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      mx[0][t] = (-dv[1][t] * ta) - (-om[1][t] * tb);
-      mx[1][t] = (dv[0][t] * ta) - (om[0][t] * tb);
-      mz[0][t] = (-dv[1][t] * tb) + (-om[1][t] * ta);
-      mz[1][t] = (dv[0][t] * tb) + (om[0][t] * ta);
-    }
-  }
-  // (3) rho u_i = m_i + grad psi                        cross.f:434-499
-  {
-    double ps[2][M];
-    load_col<M>(a.psi + o, my, lane, st, ps);
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      mx[0][t] = mx[0][t] + (-ps[1][t] * kx);
-      mx[1][t] = mx[1][t] + (ps[0][t] * kx);
-      mz[0][t] = mz[0][t] + (-ps[1][t] * kz);
-      mz[1][t] = mz[1][t] + (ps[0][t] * kz);
-    }
-    if (m00) {
-      // 00 modes of rho u, rho w and the xi part of their update
-      // (cross.f:486-499)
-#pragma unroll
-      for (int t = 0; t < M; ++t) {
-        const int r = r0 + t;
-        if (r < my) {
-          mx[0][t] = a.u00wk[r]; mx[1][t] = 0.0;
-          mz[0][t] = a.w00wk[r]; mz[1][t] = 0.0;
-          if (r >= 1 && r <= my - 2) {
-            a.u00wk[r] = a.u00wk[r] + dtxi * a.rf0u[r];
-            a.w00wk[r] = a.w00wk[r] + dtxi * a.rf0w[r];
-          }
-        }
-      }
-    }
-    store_spec<M>(sp[0] + os, my, lane, st, mx, m00, a.mean_in);
-    store_spec<M>(sp[2] + os, my, lane, st, mz, m00, a.mean_in + 2 * (size_t)my);
-    op_dy1<M, 2>(a.T, lane, ps, dv);
-    // rho v = m_y + d psi/dy
-    stage_add<M>(pk, lane, dv, 1.0);
-    if (m00) {
-      // rho v 00 mode: carried in FP64 beside the transform
-#pragma unroll
-      for (int t = 0; t < M; ++t) {
-        const int r = r0 + t;
-        if (r < my) a.mean_in[(size_t)my + r] = a.v00wk[r];
-        pk[r] = make_double2(0.0, 0.0);
-      }
-    }
-    stage_out<M>(pk, my, lane, sp[1] + os);
-  }
-  // (4) T BCs and derivatives                           cross.f:338-349, 521-554
-  scalar_derivs<M, T2>(a, lane, st, o, os, m00, kx, kz, rK, a.scal, 1.0 / rc.rhobot,
-                       1.0 / rc.rhotop, sp, 7, 4, 5, 6, 3);
-  if (a.with_z) {
-    // passive scalar: BCs and derivatives (lomaHZB cross.f:343-355, 626-666)
-    scalar_derivs<M, T2>(a, lane, st, o, os, m00, kx, kz, rK, a.zf, rc.zbot, rc.ztop, sp, -1, 8,
-                         9, 10, 11);
-  }
+  const int bx = blockIdx.x;
+#include "loma_phaseA_body.inc"
+}
+template <int M, typename TS>
+__global__ __launch_bounds__(256, 2) void k_phaseA_batch(const PhaseArgs* __restrict__ args,
+    const int* __restrict__ first, int nrun) {
+  const int r = wave_uniform(batch_run(first, nrun, blockIdx.x));
+  const PhaseArgs& a = args[r];
+  const int bx = blockIdx.x - first[r];
+#include "loma_phaseA_body.inc"
 }
 
 template <int M, typename TS>
 __global__ __launch_bounds__(256, 2) void k_phaseC(PhaseArgs a) {
-  using T2 = typename std::conditional<sizeof(TS) == 4, float2, double2>::type;
-  LOMA_STAGE2(M);
-  const RunConst& rc = a.rc;
-  const int w = wave_uniform(a.mode_base + blockIdx.x * 4 + (threadIdx.x >> 6));
-  const int lane = threadIdx.x & 63;
-  if (w >= a.mode_base + a.nmodes) return;
-  int i, k;
-  double kx, kz;
-  mode_of(rc, w, i, k, kx, kz);
-  const int my = rc.my, r0 = lane * M;
-  const double rK = kx * kx + kz * kz;
-  const double dt = a.sc->dt;
-  const double dtgamma = dt * c_gama[a.rkstep - 1];
-  const double ire = rc.ire;
-  const size_t o = (size_t)w * my;
-  const bool m00 = (i == 0 && k == 0);
-  // spec slots: 0 u, 1 v, 2 w, 3 ruu, 4 rvv, 5 rww, 6 ruv, 7 ruw, 8 rvw,
-  //             9 drho, 10 DT, 11 DZ
-  const T2* sp[12];
-#pragma unroll
-  for (int q = 0; q < 12; ++q) sp[q] = (const T2*)a.spec[q] + (size_t)w * rc.myp;
-
-  // (6a) T += DT, Z += DZ                               cross.f:619-627
-  {
-    double T[2][M];
-    load_col<M>(a.scal + o, my, lane, st, T);
-    stage_in<M>(sp[10], my, lane, st);
-    stage_add_mean<M>(st, lane, my, m00, a.mean_out + 10 * (size_t)my);
-    stage_axpy<M>(st, lane, T, 1.0);
-    stage_out<M>(st, my, lane, a.scal + o);
-    if (a.with_z) {
-      load_col<M>(a.zf + o, my, lane, st, T);
-      stage_in<M>(sp[11], my, lane, st);
-      stage_add_mean<M>(st, lane, my, m00, a.mean_out + 11 * (size_t)my);
-      stage_axpy<M>(st, lane, T, 1.0);
-      stage_out<M>(st, my, lane, a.zf + o);
-    }
-  }
-  // RHS(Omega_y) = ikz N_x - ikx N_z + lap(omega_y)/Re
-  // RHS(phi)     = -rK N_y - d/dy(ikx N_x + ikz N_z) - lap((curl omega)_y)/Re
-  // (cross.f:651-682, 748-921) with N_i = -div(rho u_i u) and, by linearity
-  // of the y-operators,
-  //   ikz N_x - ikx N_z = kxkz(ruu - rww) + (kz^2 - kx^2) ruw - d/dy G1,
-  //   -rK N_y - d/dy(ikx N_x + ikz N_z) = rK G2 + d/dy(rK rvv - S0 + d/dy G2),
-  //   G1 = ikz ruv - ikx rvw, G2 = ikx ruv + ikz rvw,
-  //   S0 = kx^2 ruu + 2 kxkz ruw + kz^2 rww,
-  //   omega_y = ikz u - ikx w, (curl omega)_y = d/dy(ikx u + ikz w) + rK v.
-  // At most three columns are live in registers; X = ikx u + ikz w and G2
-  // wait in LDS, and RHS(phi) takes X's place for the last two derivatives.
-  double A[2][M], B[2][M];
-  // omega_y -> RHS(Omega_y) viscous part in A; X parked
-  {
-    double u[2][M];
-    load_col<M>(sp[0], my, lane, st, u);
-    load_col<M>(sp[2], my, lane, st, B);  // w
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      const double ur = u[0][t], ui = u[1][t], wr = B[0][t], wi = B[1][t];
-      pk[r0 + t] = make_double2(-ui * kx - wi * kz, ur * kx + wr * kz);
-      B[0][t] = -ui * kz + wi * kx;
-      B[1][t] = ur * kz - wr * kx;
-    }
-  }
-  op_dy2<M, 2>(a.T, lane, B, A);
-#pragma unroll
-  for (int t = 0; t < M; ++t) {
-    A[0][t] = ire * (A[0][t] - rK * B[0][t]);
-    A[1][t] = ire * (A[1][t] - rK * B[1][t]);
-  }
-  // G1 -> A -= d/dy G1; G2 parked in the stage
-  {
-    double uv[2][M];
-    load_col<M>(sp[6], my, lane, st, uv);
-    load_col<M>(sp[8], my, lane, st, B);  // rvw
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      const double g2r = -uv[1][t] * kx - B[1][t] * kz;
-      const double g2i = uv[0][t] * kx + B[0][t] * kz;
-      const double g1r = -uv[1][t] * kz + B[1][t] * kx;
-      const double g1i = uv[0][t] * kz - B[0][t] * kx;
-      st[r0 + t] = make_double2(g2r, g2i);
-      B[0][t] = g1r;
-      B[1][t] = g1i;
-    }
-  }
-  {
-    double dG[2][M];
-    op_dy1<M, 2>(a.T, lane, B, dG);
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      A[0][t] -= dG[0][t];
-      A[1][t] -= dG[1][t];
-    }
-  }
-  stage_get<M>(st, lane, B);  // B <- G2
-  // rho u u, rho u w, rho w w terms of RHS(Omega_y) and S0
-  {
-    double S0[2][M];
-    const double kxx = kx * kx, kzz = kz * kz, kxz = kx * kz;
-    stage_in<M>(sp[3], my, lane, st);  // ruu
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      const double2 q = st[r0 + t];
-      A[0][t] += kxz * q.x;
-      A[1][t] += kxz * q.y;
-      S0[0][t] = kxx * q.x;
-      S0[1][t] = kxx * q.y;
-    }
-    stage_in<M>(sp[7], my, lane, st);  // ruw
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      const double2 q = st[r0 + t];
-      A[0][t] += (kzz - kxx) * q.x;
-      A[1][t] += (kzz - kxx) * q.y;
-      S0[0][t] += 2.0 * kxz * q.x;
-      S0[1][t] += 2.0 * kxz * q.y;
-    }
-    stage_in<M>(sp[5], my, lane, st);  // rww
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      const double2 q = st[r0 + t];
-      A[0][t] -= kxz * q.x;
-      A[1][t] -= kxz * q.y;
-      S0[0][t] += kzz * q.x;
-      S0[1][t] += kzz * q.y;
-    }
-    // Omega_y gamma-update and RHS store                cross.f:966-972
-    stage_in<M>(a.vor + o, my, lane, st);
-    stage_add<M>(st, lane, A, dtgamma);
-    stage_out<M>(st, my, lane, a.vor + o);
-    store_col<M>(a.ten12 + o, my, lane, st, A);
-    // A <- d/dy G2 - S0 ; S0 <- rK G2 (the start of RHS(phi))
-    op_dy1<M, 2>(a.T, lane, B, A);
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      A[0][t] -= S0[0][t];
-      A[1][t] -= S0[1][t];
-      S0[0][t] = rK * B[0][t];
-      S0[1][t] = rK * B[1][t];
-    }
-    // Y = rK rvv + d/dy G2 - S0 into B ; RHS(phi) += d/dy Y
-    stage_in<M>(sp[4], my, lane, st);  // rvv
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      const double2 q = st[r0 + t];
-      B[0][t] = rK * q.x + A[0][t];
-      B[1][t] = rK * q.y + A[1][t];
-    }
-    op_dy1<M, 2>(a.T, lane, B, A);
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      A[0][t] += S0[0][t];
-      A[1][t] += S0[1][t];
-    }
-  }
-  // (curl omega)_y = d/dy X + rK v ; RHS(phi) -= (lap - rK)(curl omega)_y / Re
-  {
-    double X[2][M], C[2][M];
-    stage_get<M>(pk, lane, X);
-    // RHS(phi) so far waits in the parked column while the two derivatives
-    // run with two columns live (three spilled 124 bytes per lane at M = 9);
-    // the fences keep the compiler from forwarding the stored values, which
-    // would hold them in registers again
-    stage_put<M>(pk, lane, A);
-    asm volatile("" ::: "memory");
-    op_dy1<M, 2>(a.T, lane, X, C);
-    stage_in<M>(sp[1], my, lane, st);  // v
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      const double2 q = st[r0 + t];
-      C[0][t] += rK * q.x;
-      C[1][t] += rK * q.y;
-    }
-    op_dy2<M, 2>(a.T, lane, C, X);
-    asm volatile("" ::: "memory");
-    stage_get<M>(pk, lane, A);
-#pragma unroll
-    for (int t = 0; t < M; ++t) {
-      A[0][t] -= ire * (X[0][t] - rK * C[0][t]);
-      A[1][t] -= ire * (X[1][t] - rK * C[1][t]);
-    }
-  }
-  // phi gamma-update and RHS store                      cross.f:974-980
-  stage_in<M>(a.phi + o, my, lane, st);
-  stage_add<M>(st, lane, A, dtgamma);
-  stage_out<M>(st, my, lane, a.phi + o);
-  store_col<M>(a.ten13 + o, my, lane, st, A);
-
-  if (m00) {
-    // 00 modes of u and w: rf0 = -d(rho u_i v)/dy + u_i00''/Re and the gamma
-    // update of the interior rows                     cross.f:686-710, 1006-1011
-#pragma unroll 1
-    for (int c = 0; c < 2; ++c) {
-      double* rf = c == 0 ? a.rf0u : a.rf0w;
-      double* wk = c == 0 ? a.u00wk : a.w00wk;
-      load_col<M>(sp[c == 0 ? 6 : 8], my, lane, st, A);
-      add_mean<M>(A, lane, my, true, a.mean_out + (size_t)(c == 0 ? 6 : 8) * my);
-      op_dy1<M, 2>(a.T, lane, A, B);
-      load_col<M>(sp[c == 0 ? 0 : 2], my, lane, st, A);
-      add_mean<M>(A, lane, my, true, a.mean_out + (size_t)(c == 0 ? 0 : 2) * my);
-#pragma unroll
-      for (int t = 0; t < M; ++t) A[1][t] = 0.0;
-      {
-        double d[2][M];
-        op_dy2<M, 2>(a.T, lane, A, d);
-#pragma unroll
-        for (int t = 0; t < M; ++t) {
-          const int r = r0 + t;
-          if (r < my) {
-            const double fu = -B[0][t] + ire * d[0][t];
-            rf[r] = fu;
-            if (r >= 1 && r <= my - 2) wk[r] = wk[r] + dtgamma * fu;
-          }
-        }
-      }
-    }
-  }
+  const int bx = blockIdx.x;
+#include "loma_phaseC_body.inc"
+}
+template <int M, typename TS>
+__global__ __launch_bounds__(256, 2) void k_phaseC_batch(const PhaseArgs* __restrict__ args,
+    const int* __restrict__ first, int nrun) {
+  const int r = wave_uniform(batch_run(first, nrun, blockIdx.x));
+  const PhaseArgs& a = args[r];
+  const int bx = blockIdx.x - first[r];
+#include "loma_phaseC_body.inc"
 }
 
 // (8) psi from the low-Mach constraint and the 00 mode of rho*v
@@ -828,109 +514,16 @@ __device__ __forceinline__ void bd_dots4(const YTab& T, int lane, const double (
 
 template <int M, typename TS>
 __global__ __launch_bounds__(256) void k_phaseC2(PhaseArgs a) {
-  using T2 = typename std::conditional<sizeof(TS) == 4, float2, double2>::type;
-  LOMA_STAGE(M);
-  const RunConst& rc = a.rc;
-  const int cls = wave_uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
-  const int lane = threadIdx.x & 63;
-  const int nka = rc.nz + 1;
-  if (cls >= rc.nplanes * nka) return;
-  const int i = cls / nka, ka = cls - i * nka;
-  const int k1 = ka, k2 = rc.mz - ka;  // k2 only for ka > 0
-  const int my = rc.my, r0 = lane * M;
-  const int w1 = i * rc.mz + k1, w2 = i * rc.mz + k2;
-  double x[4][M];
-  {
-    double d[2][M];
-    load_col<M>((const T2*)a.spec[9] + (size_t)w1 * rc.myp, my, lane, st, d);
-#pragma unroll
-    for (int t = 0; t < M; ++t) { x[0][t] = d[0][t]; x[1][t] = d[1][t]; }
-  }
-  if (ka == 0) {
-    if (i == 0) {
-      // 00 mode: rho v 00 = integral of drho00 + acoustic split
-      // (cross.f:1027-1072); psi 00 = 0
-      double d[2][M];
-#pragma unroll
-      for (int t = 0; t < M; ++t) { d[0][t] = x[0][t]; d[1][t] = x[1][t]; }
-      add_mean<M>(d, lane, my, true, a.mean_out + 9 * (size_t)my);
-      double x1[1][M];
-#pragma unroll
-      for (int t = 0; t < M; ++t) x1[0][t] = d[0][t];
-      op_inty<M>(a.T, lane, x1);
-      const double trapz = -col_value<M>(x1[0], my - 1);
-      const double add = trapz / (1.0 + sqrt(rc.rhotop / rc.rhobot));
-      // y-integrals of the 00 modes for the .cf history, taken after the
-      // gamma update of rho u00, rho w00 and before that of rho v00
-      // (cross.f:1013-1019)
-      double su = 0.0, sv = 0.0, sw = 0.0;
-#pragma unroll
-      for (int t = 0; t < M; ++t) {
-        const int r = r0 + t;
-        if (r < my) {
-          su += a.u00wk[r] * a.hy[r];
-          sv += a.v00wk[r] * a.hy[r];
-          sw += a.w00wk[r] * a.hy[r];
-        }
-      }
-      su = wave_sum(su);
-      sv = wave_sum(sv);
-      sw = wave_sum(sw);
-#pragma unroll
-      for (int t = 0; t < M; ++t)
-        if (r0 + t < my) a.v00wk[r0 + t] = x1[0][t] + add;
-      if (lane == 0) {
-        a.sc->trapz = trapz;
-        a.sc->mass[0] = su;
-        a.sc->mass[1] = sv;
-        a.sc->mass[2] = sw;
-      }
-      double z[2][M];
-#pragma unroll
-      for (int t = 0; t < M; ++t) z[0][t] = z[1][t] = 0.0;
-      store_col<M>(a.psi + (size_t)w1 * my, my, lane, st, z);
-      return;
-    }
-#pragma unroll
-    for (int t = 0; t < M; ++t) x[2][t] = x[3][t] = 0.0;
-  } else {
-    double d[2][M];
-    wave_sync();  // the stage is reused
-    load_col<M>((const T2*)a.spec[9] + (size_t)w2 * rc.myp, my, lane, st, d);
-#pragma unroll
-    for (int t = 0; t < M; ++t) { x[2][t] = d[0][t]; x[3][t] = d[1][t]; }
-  }
-  double v[4][M];
-  op_helm<M, 4>(a.T, a.helm_lu + (size_t)cls * a.lu_stride, lane, x, v);
-  double dp[8];
-  bd_dots4<M>(a.T, lane, v, dp);
-  const double* bc = a.helm_bc + (size_t)cls * 4;
-  const double a11 = bc[0], a12 = bc[1], a21 = bc[2], a22 = bc[3];
-  const double det = a11 * a22 - a12 * a21;
-  double c1[4], c2[4];
-#pragma unroll
-  for (int n = 0; n < 4; ++n) {
-    c1[n] = (-dp[n] * a22 + a12 * dp[4 + n]) / det;
-    c2[n] = (-a11 * dp[4 + n] + a21 * dp[n]) / det;
-  }
-  const TabRef H = tab(a.helm_h + (size_t)cls * 2 * WAVE * M, 2, 0, lane);
-#pragma unroll
-  for (int t = 0; t < M; ++t) {
-    const double h1 = tget(H, t, 0), h2 = tget(H, t, 1);
-#pragma unroll
-    for (int n = 0; n < 4; ++n) v[n][t] += c1[n] * h1 + c2[n] * h2;
-  }
-  {
-    double o[2][M];
-#pragma unroll
-    for (int t = 0; t < M; ++t) { o[0][t] = v[0][t]; o[1][t] = v[1][t]; }
-    store_col<M>(a.psi + (size_t)w1 * my, my, lane, st, o);
-    if (ka > 0) {
-#pragma unroll
-      for (int t = 0; t < M; ++t) { o[0][t] = v[2][t]; o[1][t] = v[3][t]; }
-      store_col<M>(a.psi + (size_t)w2 * my, my, lane, st, o);
-    }
-  }
+  const int bx = blockIdx.x;
+#include "loma_phaseC2_body.inc"
+}
+template <int M, typename TS>
+__global__ __launch_bounds__(256) void k_phaseC2_batch(const PhaseArgs* __restrict__ args,
+    const int* __restrict__ first, int nrun) {
+  const int r = wave_uniform(batch_run(first, nrun, blockIdx.x));
+  const PhaseArgs& a = args[r];
+  const int bx = blockIdx.x - first[r];
+#include "loma_phaseC2_body.inc"
 }
 
 // Seeded mixing-layer initial condition on the device, one wavefront per
@@ -1217,6 +810,23 @@ int launch_phase_m(int which, bool fp32, const PhaseArgs& a, hipStream_t st) {
 }
 
 template <int M>
+int launch_phase_batch_m(int which, bool fp32, const PhaseArgs* args, const int* first,
+                         int nrun, int nblocks, hipStream_t st) {
+  dim3 g(nblocks), b(256);
+  if (which == 0) {
+    if (fp32) hipLaunchKernelGGL((k_phaseA_batch<M, float>), g, b, 0, st, args, first, nrun);
+    else hipLaunchKernelGGL((k_phaseA_batch<M, double>), g, b, 0, st, args, first, nrun);
+  } else if (which == 1) {
+    if (fp32) hipLaunchKernelGGL((k_phaseC_batch<M, float>), g, b, 0, st, args, first, nrun);
+    else hipLaunchKernelGGL((k_phaseC_batch<M, double>), g, b, 0, st, args, first, nrun);
+  } else {
+    if (fp32) hipLaunchKernelGGL((k_phaseC2_batch<M, float>), g, b, 0, st, args, first, nrun);
+    else hipLaunchKernelGGL((k_phaseC2_batch<M, double>), g, b, 0, st, args, first, nrun);
+  }
+  return (int)hipGetLastError();
+}
+
+template <int M>
 int launch_thickness_m(bool fp32, const PhaseArgs& a, const double* hy, double ly, int which,
                        hipStream_t st) {
   if (fp32) hipLaunchKernelGGL((k_thickness<M, float>), dim3(1), dim3(64), 0, st, a, hy, ly, which);
@@ -1229,6 +839,10 @@ int launch_ic_m(const IcArgs& a, hipStream_t st) {
   hipLaunchKernelGGL((k_ic<M>), dim3((nmodes + 3) / 4), dim3(256), 0, st, a);
   return (int)hipGetLastError();
 }
+
+}  // namespace loma
+#include "loma_stats_impl.h"
+namespace loma {
 
 #define LOMA_INSTANTIATE_M(MV)                                                        \
   template int launch_dy_m<MV>(int, const YTab&, const double*, double*, int, hipStream_t); \
@@ -1243,8 +857,11 @@ int launch_ic_m(const IcArgs& a, hipStream_t st) {
                                  double*, double*, const double*, double, double, double, \
                                  int, hipStream_t);                                       \
   template int launch_phase_m<MV>(int, bool, const PhaseArgs&, hipStream_t);              \
+  template int launch_phase_batch_m<MV>(int, bool, const PhaseArgs*, const int*, int, int,  \
+                                        hipStream_t);                                     \
   template int launch_ic_m<MV>(const IcArgs&, hipStream_t);                              \
   template int launch_thickness_m<MV>(bool, const PhaseArgs&, const double*, double, int,  \
-                                      hipStream_t);
+                                      hipStream_t);                                       \
+  template int launch_stats_m<MV>(bool, int, const StatsArgs&, hipStream_t);
 
 }  // namespace loma
