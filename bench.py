@@ -258,6 +258,18 @@ def bench_ensemble(nruns, steps, warmup, seed=7, workers=8):
         capi.advance_many(runs, 1)
         lat.append((time.perf_counter() - t1) * 1e3)
     pts = sum(c.points for c in cfgs)
+    # each grid of the ensemble alone (one run, same precision, flags and
+    # tiles): the ensemble step against its runs stepped one after another
+    single_ms = {}
+    for c, r in zip(cfgs, runs):
+        key = (c.mgalx, c.my, c.mgalz)
+        if key in single_ms:
+            continue
+        r.advance(1)
+        t1 = time.perf_counter()
+        r.advance(3)
+        single_ms[key] = (time.perf_counter() - t1) * 1e3 / 3
+    seq_ms = sum(single_ms[(c.mgalx, c.my, c.mgalz)] for c in cfgs)
     out = dict(runs=nruns, steps=steps, grid_points=pts,
                gpts_per_s=round(pts * steps / wall, 1),
                ms_per_ensemble_step=round(wall * 1e3 / steps, 3),
@@ -268,10 +280,49 @@ def bench_ensemble(nruns, steps, warmup, seed=7, workers=8):
                                                   if (c.mgalx, c.my, c.mgalz) == g)
                       for g in grid.ENSEMBLE_GRIDS},
                tiles=sorted({(c.mgalx, c.my, c.mgalz, *r.tiles()) for c, r in zip(cfgs, runs)}),
-               precision="mixed", launch="one hipGraph launch per RK3 step")
+               precision="mixed",
+               launch="one hipGraph launch per RK3 step; one launch per kernel, substage and "
+                      "group of runs with the same instantiation, one capture lane per group",
+               single_ms_per_step={f"{k[0]}x{k[1]}x{k[2]}": round(v, 3)
+                                   for k, v in single_ms.items()},
+               runs_in_sequence_ms=round(seq_ms, 3),
+               vs_runs_in_sequence=round(seq_ms / (wall * 1e3 / steps), 3))
     for r in runs:
         r.close()
     return out
+
+
+def bench_stats(cfg, st, steps=3):
+    """Cost of the statistics window (addstats1/2, the phys2 sums) at the
+    bench workload, mixed: ms per step over `steps` steps with the window
+    closed (captured step graph) and open (direct launches plus the
+    statistics kernels at rkstep 1)."""
+    import torch
+    from loma_phd_amd import capi
+    run = capi.Run(cfg.mgalx, cfg.my, cfg.mgalz, st["y"], cfg.Re, cfg.alp, cfg.bet, cfg.pr,
+                   cfg.cfl, cfg.ncfl, cfg.dt0, capi.LOMA_MIXED, 0)
+    run.upload(st["vor"], st["phi"], st["psi"], st["scal"], st["u00"], st["v00"], st["w00"])
+    run.use_graph(True)
+    rows = np.unique(np.linspace(cfg.my // 8, cfg.my - cfg.my // 8, 30).astype(int))
+    run.stats_config(rows)
+    run.advance(2)
+
+    def timed(n):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        run.advance(n)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3 / n
+
+    closed = timed(steps)
+    run.stats_accumulate(True)
+    opened = timed(steps)
+    nacum = run.stats_read()["nacum"]
+    run.close()
+    return dict(steps=steps, spectra_rows=int(rows.size) + 1,
+                ms_per_step_window_closed=round(closed, 3),
+                ms_per_step_window_open=round(opened, 3),
+                extra_ms_per_step=round(opened - closed, 3), nacum=nacum)
 
 
 def bench_served(window, rate=2.0, seed=11, max_slots=32, workers=8):
@@ -325,6 +376,8 @@ def main():
     ap.add_argument("--ensemble-runs", type=int, default=32,
                     help="runs in the ensemble-batch leg (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=30)
+    ap.add_argument("--stats-steps", type=int, default=3,
+                    help="steps timed with the statistics window closed and open (0: skip)")
     ap.add_argument("--served-window", type=float, default=30.0,
                     help="seconds of Poisson arrivals in the served leg (0 skips it)")
     args = ap.parse_args()
@@ -399,6 +452,9 @@ def main():
                                         ic.HostOps(st["y"]), cfg.alp, cfg.bet)
         line["mixed_vs_fp64_field_errors"] = dict(
             steps=res["mixed"]["nsteps_total"], **{k: float(f"{v:.3e}") for k, v in errs.items()})
+    if args.stats_steps > 0:
+        print("[bench] statistics window", file=sys.stderr, flush=True)
+        line["stats_window"] = bench_stats(cfg, st, args.stats_steps)
     if args.band:
         print("[bench] batched banded solves", file=sys.stderr, flush=True)
         line["band"] = bench_band()

@@ -259,7 +259,21 @@ struct EnsembleGraph {
   std::vector<hipEvent_t> ready;  // per run: work already queued on its stream
   void* desc = nullptr;           // device argument blocks of the batched launches
   std::vector<hipStream_t> lanes; // one capture stream per group of runs
+  int cap = 0;                    // most runs per batched launch
 };
+// Batch shape of an ensemble whose runs all carry LOMA_FLAG_AUTOTUNE: its
+// first steps alternate between one launch per group and at most two runs
+// per launch, timed on the device, and the faster is kept.  Every run's
+// result is the same either way, so the trial steps are ordinary steps.
+struct EnsShape {
+  std::vector<loma_run*> runs;
+  int trial = 0;          // trial steps done
+  float best[2] = {1e30f, 1e30f};
+  int cap = 0;            // chosen (0 while trying)
+};
+static std::vector<EnsShape> g_shape;
+constexpr int ENS_CAPS[2] = {1 << 20, 2};
+constexpr int ENS_TRIALS = 4;   // alternating, two per shape
 static std::mutex g_ens_mu;
 static std::vector<EnsembleGraph> g_ens;
 
@@ -280,6 +294,12 @@ static void ensemble_forget(loma_run* r) {
     } else {
       ++q;
     }
+  }
+  for (size_t q = 0; q < g_shape.size();) {
+    bool has = false;
+    for (loma_run* x : g_shape[q].runs) has = has || x == r;
+    if (has) g_shape.erase(g_shape.begin() + q);
+    else ++q;
   }
 }
 
@@ -929,6 +949,9 @@ static StatsArgs stats_args(loma_run* r) {
   a.mean_out = r->mean_out;
   a.jsp = r->st_jsp;
   a.nsp = r->st_nsp;
+  a.kach = STAT_KACH;
+  a.nch = (r->nz + STAT_KACH) / STAT_KACH;
+  a.nw = r->nplanes * a.nch;
   a.acc = r->st_acc;
   a.spe = r->st_spe;
   a.part = r->st_part;
@@ -1081,14 +1104,15 @@ struct EnsPlan {
   }
 };
 
-static int ens_batch_cap() {
+// LOMA_ENS_BATCH=n fixes the most runs per batched launch (0: not set)
+static int ens_batch_env() {
   const char* e = std::getenv("LOMA_ENS_BATCH");
   const int v = e ? std::atoi(e) : 0;
-  return v > 0 ? v : 1 << 20;
+  return v > 0 ? v : 0;
 }
 
 static void ensemble_plan(loma_run* const* runs, int nruns, const std::vector<int>& icfl,
-                          EnsPlan& plan) {
+                          int cap, EnsPlan& plan) {
   std::vector<std::array<int, 7>> keys;
   std::vector<std::vector<int>> groups;
   for (int q = 0; q < nruns; ++q) {
@@ -1101,8 +1125,7 @@ static void ensemble_plan(loma_run* const* runs, int nruns, const std::vector<in
     groups[c].push_back(q);
   }
   // at most `cap` runs per launch: a group larger than that is dealt round
-  // robin to several lanes, which then overlap (LOMA_ENS_BATCH)
-  const int cap = ens_batch_cap();
+  // robin to several lanes, which then overlap
   std::vector<std::vector<int>> split;
   for (const auto& g : groups) {
     const int nl = ((int)g.size() + cap - 1) / cap;
@@ -1443,7 +1466,7 @@ int loma_stream(loma_run* r, void** stream) {
 
 int loma_advance_many(loma_run** runs, int nruns, int nsteps) {
   if (!runs || nruns <= 0 || nsteps < 0) return LOMA_ERR_ARG;
-  bool graph = true;
+  bool graph = true, all_tune = true;
   for (int q = 0; q < nruns; ++q) {
     loma_run* r = runs[q];
     if (!r || !r->uploaded) { set_error("loma_advance_many: run not uploaded"); return LOMA_ERR_STATE; }
@@ -1457,6 +1480,7 @@ int loma_advance_many(loma_run** runs, int nruns, int nsteps) {
     int e = ensure_stream(r);  // every run needs a stream of its own
     if (e) return e;
     graph = graph && r->use_graph && !r->prof;
+    all_tune = all_tune && (r->flags & LOMA_FLAG_AUTOTUNE);
   }
   for (int n = 0; n < nsteps; ++n) {
     std::vector<int> icfl(nruns);
@@ -1471,19 +1495,37 @@ int loma_advance_many(loma_run** runs, int nruns, int nsteps) {
       // lock order everywhere: set-up, then ensembles
       std::lock_guard<std::recursive_mutex> lk2(g_setup_mu);
       std::lock_guard<std::mutex> lk(g_ens_mu);
+      // batch shape: fixed by the environment, tried and kept (autotune), or
+      // one launch per group
+      int cap = ens_batch_env();
+      EnsShape* shape = nullptr;
+      if (!cap && all_tune) {
+        for (auto& e : g_shape)
+          if ((int)e.runs.size() == nruns && std::equal(e.runs.begin(), e.runs.end(), runs))
+            shape = &e;
+        if (!shape) {
+          g_shape.push_back(EnsShape{});
+          shape = &g_shape.back();
+          shape->runs.assign(runs, runs + nruns);
+        }
+        cap = shape->cap ? shape->cap : ENS_CAPS[shape->trial & 1];
+        if (shape->cap) shape = nullptr;  // decided: no timing
+      }
+      if (!cap) cap = ENS_CAPS[0];
       EnsembleGraph* eg = nullptr;
       for (auto& c : g_ens)
-        if ((int)c.runs.size() == nruns && c.icfl == icfl &&
+        if ((int)c.runs.size() == nruns && c.icfl == icfl && c.cap == cap &&
             std::equal(c.runs.begin(), c.runs.end(), runs))
           eg = &c;
       if (!eg) {
         EnsembleGraph c;
         c.runs.assign(runs, runs + nruns);
         c.icfl = icfl;
+        c.cap = cap;
         // argument blocks of the batched launches, on the device before the
         // capture starts (nothing is allocated while it records)
         EnsPlan plan;
-        ensemble_plan(runs, nruns, icfl, plan);
+        ensemble_plan(runs, nruns, icfl, cap, plan);
         HIPCHK(hipMalloc(&c.desc, plan.bytes.size()));
         if (hipMemcpy(c.desc, plan.bytes.data(), plan.bytes.size(), hipMemcpyHostToDevice) !=
             hipSuccess) {
@@ -1534,7 +1576,26 @@ int loma_advance_many(loma_run** runs, int nruns, int nsteps) {
         HIPCHK(hipEventRecord(eg->ready[q], runs[q]->stream));
         HIPCHK(hipStreamWaitEvent(eg->master, eg->ready[q], 0));
       }
-      HIPCHK(hipGraphLaunch(eg->exec, eg->master));
+      if (shape) {
+        // a trial step of the batch shape: timed on the device
+        hipEvent_t t0, t1;
+        HIPCHK(hipEventCreate(&t0));
+        HIPCHK(hipEventCreate(&t1));
+        HIPCHK(hipEventRecord(t0, eg->master));
+        HIPCHK(hipGraphLaunch(eg->exec, eg->master));
+        HIPCHK(hipEventRecord(t1, eg->master));
+        HIPCHK(hipEventSynchronize(t1));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+        hipEventDestroy(t0);
+        hipEventDestroy(t1);
+        const int v = shape->trial & 1;
+        shape->best[v] = std::min(shape->best[v], ms);
+        if (++shape->trial >= ENS_TRIALS)
+          shape->cap = ENS_CAPS[shape->best[1] < shape->best[0] ? 1 : 0];
+      } else {
+        HIPCHK(hipGraphLaunch(eg->exec, eg->master));
+      }
     }
     for (int q = 0; q < nruns; ++q) runs[q]->istep++;
   }
@@ -1666,7 +1727,7 @@ int loma_stats_config(loma_run* r, int nspec, const int* jspecy) {
   const int nsp = (int)rows.size();
   const size_t nacc = (size_t)STAT_NCOL * my;
   const size_t nspe = (size_t)r->nplanes * 12 * nsp * (r->nz + 1);
-  const size_t npart = (size_t)r->nplanes * STAT_NPART * my;
+  const size_t npart = (size_t)r->nplanes * ((r->nz + STAT_KACH) / STAT_KACH) * STAT_NPART * my;
   const size_t nrho = (size_t)r->mgalz * my;
   struct { void** p; size_t bytes; } al[] = {
       {(void**)&r->st_acc, nacc * 8}, {(void**)&r->st_spe, nspe * 8},

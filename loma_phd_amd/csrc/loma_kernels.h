@@ -127,7 +127,8 @@ struct FftArgs {
 //   22 ruu 23 ruv 24 ruw 25 rvv 26 rvw 27 rww 28 rhom 29 thep 30 theup
 //   31 mum 32 num
 constexpr int STAT_NCOL = 33;
-constexpr int STAT_NPART = 13;  // per-plane partial sums: Tp, then the 12 of addstats2
+constexpr int STAT_NPART = 13;  // per-chunk partial sums: Tp, then the 12 of addstats2
+constexpr int STAT_KACH = 4;    // |kz| classes per statistics wavefront
 struct StatsArgs {
   RunConst rc;
   YTab T;
@@ -135,9 +136,12 @@ struct StatsArgs {
   const double *mean_in, *mean_out;
   const int* jsp;            // spectra rows (0-based)
   int nsp;
+  int kach;                  // |kz| classes per wavefront (a chunk of one kx plane)
+  int nch;                   // chunks per kx plane: ceil((nz+1) / kach)
+  int nw;                    // statistics wavefronts: nplanes * nch
   double* acc;               // [STAT_NCOL][my]
   double* spe;               // [nplanes][12][nsp][nz+1]
-  double* part;              // [nplanes][STAT_NPART][my], this step's sums per kx plane
+  double* part;              // [STAT_NPART][my][nw], this step's sums per chunk
   const double* rho;         // [mgalz][my] line sums of 1/T from the x lines
   double* last;              // [9][my] this step's plane-summed intensities (uner)
 };

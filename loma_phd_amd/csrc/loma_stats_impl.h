@@ -1,13 +1,15 @@
 // loma_stats_impl.h -- the statistics window on the device: the y-profile
 // sums and spectra of addstats1 (trunk/lomacte3/src/cross.f:1981-2063) and
 // addstats2 (cross.f:2070-2287), accumulated at rkstep 1 of the steps of a
-// window, one wavefront per kx plane.
+// window.
 //
-// A plane's wavefront walks its kz modes in order, so every spectrum entry
-// (kx plane, |kz|, row) and every per-y sum has one writer and one summation
-// order: the results do not depend on scheduling.  The per-y sums of a step
-// go to a per-plane partial; k_stats_reduce (loma_yops.hip) adds the planes
-// in order.  The means of the 00 mode are added by the plane-0 wavefront.
+// One wavefront per chunk of a kx plane: kach consecutive |kz| classes, each
+// the mode pair kz = +-ka (ka = 0 alone), walked in order.  Every spectrum
+// entry (kx plane, |kz|, row) therefore has one writer, and every per-y sum
+// one summation order: the results do not depend on scheduling.  The per-y
+// sums of a step go to a per-chunk partial; k_stats_reduce (loma_yops.hip)
+// adds the chunks and planes in order.  The means of the 00 mode are added
+// by the wavefront that holds it (plane 0, chunk 0).
 // Included by loma_yops_impl.h (it needs the y operators and staging).
 #pragma once
 
@@ -17,9 +19,27 @@ __device__ __forceinline__ double* spe_at(const StatsArgs& a, int i, int q, int 
   return a.spe + (((size_t)i * 12 + q) * a.nsp + n) * (a.rc.nz + 1) + ka;
 }
 
+// Lane n of the wavefront gets the value of a column (held M rows per lane)
+// at the n-th spectra row of the block n0.. (zero for lanes past the rows):
+// M shuffles, one per row slot, each lane keeping the one its row sits in.
+template <int M>
+__device__ __forceinline__ double2 spectra_row(const StatsArgs& a, const double (&x)[2][M],
+                                               int lane, int n0) {
+  const int n = n0 + lane;
+  const int row = n < a.nsp ? a.jsp[n] : 0;
+  const int l = row / M, t = row - l * M;
+  double re = 0.0, im = 0.0;
+#pragma unroll
+  for (int q = 0; q < M; ++q) {
+    const double vr = __shfl(x[0][q], l), vi = __shfl(x[1][q], l);
+    if (q == t) { re = vr; im = vi; }
+  }
+  return n < a.nsp ? make_double2(re, im) : make_double2(0.0, 0.0);
+}
+
 // addstats1: rho u, rho v, rho w and T as they enter the transforms (spec
 // slots 0, 1, 2 and 7 after phase A, the 00 mode's FP64 profile included),
-// cross.f:578-586.  Spectra 9..12, T'^2 per y (into the plane partial), the
+// cross.f:578-586.  Spectra 9..12, T'^2 per y (into the chunk partial), the
 // means of rho u_i and T, and the phys2 sums that need no physical field
 // (sigma = 0: mum = sum T^0, num = sum T over the plane, cross.f:1604-1606).
 template <int M, typename TS>
@@ -27,29 +47,32 @@ __global__ __launch_bounds__(256) void k_stats_in(StatsArgs a) {
   using T2 = typename std::conditional<sizeof(TS) == 4, float2, double2>::type;
   LOMA_STAGE(M);
   const RunConst& rc = a.rc;
-  const int i = wave_uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int wv = wave_uniform(blockIdx.x * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
-  if (i >= rc.nplanes) return;
+  if (wv >= rc.nplanes * a.nch) return;
+  const int i = wv / a.nch, ch = wv - i * a.nch;
+  const int ka0 = ch * a.kach, ka1 = min(rc.nz, ka0 + a.kach - 1);
   const int my = rc.my, r0 = lane * M;
   const double f = i == 0 ? 1.0 : 2.0;  // kx > 0 planes stand for their conjugates
   const double npts = (double)rc.mgalx * (double)rc.mgalz;
   double tp[M];
 #pragma unroll
   for (int t = 0; t < M; ++t) tp[t] = 0.0;
-  for (int k = 0; k < rc.mz; ++k) {
+  for (int km = 2 * ka0; km <= 2 * ka1 + 1; ++km) {
+    const int ka = km >> 1;
+    if ((km & 1) && ka == 0) continue;  // kz = 0 has no pair
+    const int k = (km & 1) ? rc.mz - ka : ka;
     const int w = i * rc.mz + k;
     const bool m00 = (i == 0 && k == 0);
-    const int ka = k <= rc.nz ? k : rc.mz - k;
 #pragma unroll 1
     for (int c = 0; c < 4; ++c) {
       const int slot = c < 3 ? c : 7;
       double x[2][M];
       load_col<M>((const T2*)a.spec[slot] + (size_t)w * rc.myp, my, lane, st, x);
       add_mean<M>(x, lane, my, m00, a.mean_in + (size_t)slot * my);
-      for (int n = 0; n < a.nsp; ++n) {
-        const int row = a.jsp[n];
-        const double re = col_value<M>(x[0], row), im = col_value<M>(x[1], row);
-        if (lane == 0) *spe_at(a, i, 8 + c, n, ka) += f * (re * re + im * im);
+      for (int n0 = 0; n0 < a.nsp; n0 += WAVE) {
+        const double2 v = spectra_row<M>(a, x, lane, n0);
+        if (n0 + lane < a.nsp) *spe_at(a, i, 8 + c, n0 + lane, ka) += f * (v.x * v.x + v.y * v.y);
       }
       if (c == 3) {
 #pragma unroll
@@ -72,17 +95,17 @@ __global__ __launch_bounds__(256) void k_stats_in(StatsArgs a) {
   }
 #pragma unroll
   for (int t = 0; t < M; ++t)
-    if (r0 + t < my) a.part[((size_t)i * STAT_NPART + 0) * my + r0 + t] = tp[t];
+    if (r0 + t < my) a.part[((size_t)0 * my + r0 + t) * a.nw + wv] = tp[t];
 }
 
 // addstats2: u, v, w as foup2 returns them (spec slots 0, 1, 2 after the z
 // transforms, the 00 mode's FP64 estimate added back), their y derivatives
 // and the vorticities of cross.f:748-767 (o1 = dw/dy - ikz v, o2 = ikz u -
 // ikx w, o3 = ikx v - du/dy).  The twelve per-y sums of a mode go to LDS
-// (one column each, row t*64 + lane of the lane's own rows), the plane's
-// totals to its partial.  The plane-0 wavefront also adds the 00 means and
-// the phys2 sums of rho u_i u_j, which are the 00 mode of the transformed
-// products times mgalx*mgalz (cross.f:1586-1601).
+// (one column each, row t*64 + lane of the lane's own rows), the chunk's
+// totals to its partial.  The wavefront of plane 0, chunk 0 also adds the
+// 00 means and the phys2 sums of rho u_i u_j, which are the 00 mode of the
+// transformed products times mgalx*mgalz (cross.f:1586-1601).
 template <int M, typename TS>
 __global__ __launch_bounds__(64) void k_stats_out(StatsArgs a) {
   using T2 = typename std::conditional<sizeof(TS) == 4, float2, double2>::type;
@@ -90,7 +113,9 @@ __global__ __launch_bounds__(64) void k_stats_out(StatsArgs a) {
   __shared__ double sacc[12][WAVE * M];
   double2* st = stage_;
   const RunConst& rc = a.rc;
-  const int i = blockIdx.x;
+  const int wv = blockIdx.x;
+  const int i = wv / a.nch, ch = wv - i * a.nch;
+  const int ka0 = ch * a.kach, ka1 = min(rc.nz, ka0 + a.kach - 1);
   const int lane = threadIdx.x;
   const int my = rc.my, r0 = lane * M;
   const double f = i == 0 ? 1.0 : 2.0;
@@ -100,12 +125,11 @@ __global__ __launch_bounds__(64) void k_stats_out(StatsArgs a) {
 #pragma unroll
     for (int t = 0; t < M; ++t) sacc[q][t * WAVE + lane] = 0.0;
   // spectrum q of this mode at the spectra rows, from a column held in
-  // registers (every lane takes part in the shuffles; lane 0 writes)
+  // registers: lane n updates row n's entry (one writer per entry)
   auto spec_add = [&](int q, int ka, const double (&x)[2][M]) __attribute__((always_inline)) {
-    for (int n = 0; n < a.nsp; ++n) {
-      const int row = a.jsp[n];
-      const double re = col_value<M>(x[0], row), im = col_value<M>(x[1], row);
-      if (lane == 0) *spe_at(a, i, q, n, ka) += f * (re * re + im * im);
+    for (int n0 = 0; n0 < a.nsp; n0 += WAVE) {
+      const double2 v = spectra_row<M>(a, x, lane, n0);
+      if (n0 + lane < a.nsp) *spe_at(a, i, q, n0 + lane, ka) += f * (v.x * v.x + v.y * v.y);
     }
   };
   auto mean_add = [&](int c, const double (&x)[2][M]) __attribute__((always_inline)) {
@@ -113,10 +137,12 @@ __global__ __launch_bounds__(64) void k_stats_out(StatsArgs a) {
     for (int t = 0; t < M; ++t)
       if (r0 + t < my) a.acc[(size_t)c * my + r0 + t] += x[0][t];
   };
-  for (int k = 0; k < rc.mz; ++k) {
+  for (int km = 2 * ka0; km <= 2 * ka1 + 1; ++km) {
+    const int ka = km >> 1;
+    if ((km & 1) && ka == 0) continue;  // kz = 0 has no pair
+    const int k = (km & 1) ? rc.mz - ka : ka;
     const int w = i * rc.mz + k;
     const bool m00 = (i == 0 && k == 0);
-    const int ka = k <= rc.nz ? k : rc.mz - k;
     const double kz = (k <= rc.nz) ? rc.bet * k : -rc.bet * (rc.mz - k);
     const double rK = kx * kx + kz * kz;
     double u[2][M], v[2][M], ww[2][M];
@@ -151,13 +177,11 @@ __global__ __launch_bounds__(64) void k_stats_out(StatsArgs a) {
       spec_add(0, ka, u);
       spec_add(1, ka, v);
       spec_add(2, ka, ww);
-      for (int n = 0; n < a.nsp; ++n) {  // Re and Im of u conj v
-        const int row = a.jsp[n];
-        const double ur = col_value<M>(u[0], row), ui = col_value<M>(u[1], row);
-        const double vr = col_value<M>(v[0], row), vi = col_value<M>(v[1], row);
-        if (lane == 0) {
-          *spe_at(a, i, 3, n, ka) += f * (ur * vr + ui * vi);
-          *spe_at(a, i, 4, n, ka) += f * (ui * vr - ur * vi);
+      for (int n0 = 0; n0 < a.nsp; n0 += WAVE) {  // Re and Im of u conj v
+        const double2 uu = spectra_row<M>(a, u, lane, n0), vv = spectra_row<M>(a, v, lane, n0);
+        if (n0 + lane < a.nsp) {
+          *spe_at(a, i, 3, n0 + lane, ka) += f * (uu.x * vv.x + uu.y * vv.y);
+          *spe_at(a, i, 4, n0 + lane, ka) += f * (uu.y * vv.x - uu.x * vv.y);
         }
       }
       spec_add(6, ka, o2);
@@ -216,8 +240,8 @@ __global__ __launch_bounds__(64) void k_stats_out(StatsArgs a) {
 #pragma unroll
     for (int t = 0; t < M; ++t)
       if (r0 + t < my)
-        a.part[((size_t)i * STAT_NPART + 1 + q) * my + r0 + t] = sacc[q][t * WAVE + lane];
-  if (i == 0) {
+        a.part[((size_t)(1 + q) * my + r0 + t) * a.nw + wv] = sacc[q][t * WAVE + lane];
+  if (wv == 0) {
     // sum over the plane of rho u_i u_j = mgalx*mgalz times the 00 mode of
     // the transformed product (slot -> column: 3 ruu, 6 ruv, 7 ruw, 4 rvv,
     // 8 rvw, 5 rww)
@@ -239,11 +263,11 @@ __global__ __launch_bounds__(64) void k_stats_out(StatsArgs a) {
 template <int M>
 int launch_stats_m(bool fp32, int which, const StatsArgs& a, hipStream_t st) {
   if (which == 1) {
-    dim3 g((a.rc.nplanes + 3) / 4), b(256);
+    dim3 g((a.rc.nplanes * a.nch + 3) / 4), b(256);
     if (fp32) hipLaunchKernelGGL((k_stats_in<M, float>), g, b, 0, st, a);
     else hipLaunchKernelGGL((k_stats_in<M, double>), g, b, 0, st, a);
   } else {
-    dim3 g(a.rc.nplanes), b(64);
+    dim3 g(a.rc.nplanes * a.nch), b(64);
     if (fp32) hipLaunchKernelGGL((k_stats_out<M, float>), g, b, 0, st, a);
     else hipLaunchKernelGGL((k_stats_out<M, double>), g, b, 0, st, a);
   }

@@ -231,30 +231,35 @@ int launch_finish_batch(const FinishArgs* args, int nrun, hipStream_t st) {
 }
 
 
-// The step's per-plane sums into the accumulated columns, planes added in
-// order (one thread per y row): Tp, then up, vp, wp, uvr, uwr, vwr, w1p,
-// w2p, w3p, thep, theup, ep; the line sums of 1/T over z (rhom).  The
-// plane-summed intensities of this step are kept for uner (cross.f:802-816).
-__global__ __launch_bounds__(256) void k_stats_reduce(StatsArgs a) {
+// The step's per-chunk sums into the accumulated columns: one wavefront per
+// (y row, quantity) adds the row's chunk partials (contiguous in memory),
+// each lane a fixed strided subset in order and then a fixed shuffle tree,
+// so the sum does not depend on scheduling.  Quantities: Tp, then up, vp,
+// wp, uvr, uwr, vwr, w1p, w2p, w3p, thep, theup, ep; the last index takes
+// the line sums of 1/T over z (rhom).  The plane-summed intensities of this
+// step are kept for uner (cross.f:802-816).
+__global__ __launch_bounds__(64) void k_stats_reduce(StatsArgs a) {
   const int my = a.rc.my;
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= my) return;
-  constexpr int col[STAT_NPART] = {20, 11, 12, 13, 14, 15, 16, 17, 18, 19, 29, 30, 21};
-#pragma unroll 1
-  for (int q = 0; q < STAT_NPART; ++q) {
-    double s = 0.0;
-    for (int i = 0; i < a.rc.nplanes; ++i) s += a.part[((size_t)i * STAT_NPART + q) * my + j];
+  const int j = blockIdx.x, q = blockIdx.y, lane = threadIdx.x;
+  constexpr int col[STAT_NPART + 1] = {20, 11, 12, 13, 14, 15, 16, 17, 18, 19, 29, 30, 21, 28};
+  double s = 0.0;
+  if (q < STAT_NPART) {
+    const double* p = a.part + ((size_t)q * my + j) * a.nw;
+    for (int w = lane; w < a.nw; w += 64) s += p[w];
+  } else {
+    for (int z = lane; z < a.rc.mgalz; z += 64) s += a.rho[(size_t)z * my + j];
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
+  if (lane == 0) {
     a.acc[(size_t)col[q] * my + j] += s;
     if (q >= 1 && q <= 9) a.last[(size_t)(q - 1) * my + j] = s;
   }
-  double r = 0.0;
-  for (int z = 0; z < a.rc.mgalz; ++z) r += a.rho[(size_t)z * my + j];
-  a.acc[(size_t)28 * my + j] += r;
 }
 
 int launch_stats(int M, bool fp32, int which, const StatsArgs& a, hipStream_t st) {
   if (which == 3) {
-    hipLaunchKernelGGL(k_stats_reduce, dim3((a.rc.my + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_stats_reduce, dim3(a.rc.my, STAT_NPART + 1), dim3(64), 0, st, a);
     return (int)hipGetLastError();
   }
   LOMA_DISPATCH_M(M, launch_stats_m<MM>(fp32, which, a, st));

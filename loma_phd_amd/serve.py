@@ -125,11 +125,16 @@ class HipBackend:
     request of that grid (loma_set_physics + upload)."""
 
     def __init__(self, keep_state: bool = False, use_graph: bool = True, reuse: bool = True,
-                 device_ic: bool = True):
+                 device_ic: bool = True, autotune: bool = True):
         from . import capi  # loads libloma_hip.so; fails loudly if it is missing
         self.capi = capi
         self.keep_state = keep_state
         self.use_graph = use_graph
+        # autotune: every run is made with LOMA_FLAG_AUTOTUNE, so the first run
+        # of each grid times its transform tiles and later runs of that grid
+        # take the cached choice (the library keeps it per grid, precision
+        # and Z for the life of the process)
+        self.autotune = autotune
         # device_ic: only the seeded draws are made on the host and the state
         # is built on the device (loma_seed).  The host initial condition is
         # numpy work that holds the GIL for seconds per large grid and would
@@ -164,7 +169,8 @@ class HipBackend:
             self.pool.close_all()
 
     def flags(self, req: Request) -> int:
-        return self.capi.LOMA_FLAG_SCALAR_Z if req.cfg.scalar_z else 0
+        f = self.capi.LOMA_FLAG_SCALAR_Z if req.cfg.scalar_z else 0
+        return f | (self.capi.LOMA_FLAG_AUTOTUNE if self.autotune else 0)
 
     def footprint(self, req: Request) -> int:
         c = req.cfg

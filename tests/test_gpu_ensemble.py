@@ -62,3 +62,29 @@ def test_autotune_same_result(gpu):
     da, db = a.download(), b.download()
     for f in FIELDS:
         assert rel_l2(db[f], da[f]) < 1e-14, f
+
+
+def test_ensemble_batch_shape_trials(gpu):
+    """Runs made with LOMA_FLAG_AUTOTUNE: the ensemble's first four steps
+    try two batch shapes (one launch per group, at most two runs per launch)
+    and keep the faster; every step, trial or not, leaves each run where it
+    would be alone, bit for bit."""
+    from loma_phd_amd import capi
+    print("ensemble batch-shape trials", flush=True)
+    flags = capi.LOMA_FLAG_AUTOTUNE
+    alone = []
+    for g in GRIDS:
+        r = _make(g, 1, flags)
+        r.advance(6)
+        alone.append(r.download())
+        r.close()
+    runs = [_make(g, 1, flags) for g in GRIDS]
+    for r in runs:
+        r.use_graph(True)
+    for _ in range(6):
+        capi.advance_many(runs, 1)
+    for q, r in enumerate(runs):
+        got = r.download()
+        for f in FIELDS:
+            assert np.array_equal(got[f], alone[q][f]), (q, f)
+        r.close()

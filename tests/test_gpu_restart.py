@@ -73,3 +73,50 @@ def test_restart_on_nan(gpu):
     assert sc["nan"] == 0 and np.isfinite(sc["dt"])
     for f in FIELDS:
         assert np.array_equal(got[f], want[f]), f
+
+
+def test_restart_file_continues_the_run(gpu, tmp_path):
+    """A restart file written from a device run (the escru layout with its
+    FP64 sidecar, loma_phd_amd.restart) holds the state exactly: read back,
+    it equals the downloaded state, and a run started from the file steps
+    bit for bit like a run started from the downloaded state itself.  (Both
+    restarted runs take the boundary densities from the 00 mode of T as
+    cross1 does on entry, cross.f:208-213; T's end rows carry the last
+    substep's increment, so a restart is not the uninterrupted run, in the
+    reference either.)  Without the sidecar the file's FP32 fields are the
+    state to FP32 rounding."""
+    from loma_phd_amd import capi, ic, restart
+    from loma_phd_amd.grid import MixingLayerConfig
+    print("restart file round trip 48x97x36", flush=True)
+    cfg = MixingLayerConfig(mgalx=48, my=97, mgalz=36, s=3.0, Re=300.0, seed=8,
+                            scalar_z=False)
+    st = ic.initial_condition(cfg)
+    a = _run(cfg, st, cfg.cfl)
+    a.advance(3)
+    sa = a.download()
+    sc = a.scalars()
+    a.close()
+    path = str(tmp_path / "run.004")
+    restart.write(path, dict(sa, y=st["y"]), time=sc["time"], Re=cfg.Re, alp=cfg.alp,
+                  bet=cfg.bet, with_z=False)
+    back = restart.read(path)
+    assert back["time"] == sc["time"]
+    for f in FIELDS:
+        assert np.array_equal(back[f], sa[f]), f
+    raw = restart.read(path, prefer_sidecar=False)
+    for f in ("vor", "phi", "psi", "scal"):
+        e = np.abs(raw[f] - sa[f]).max() / np.abs(sa[f]).max()
+        assert e <= 2 ** -23, (f, e)
+
+    def start(s):
+        r = capi.Run(cfg.mgalx, cfg.my, cfg.mgalz, st["y"], cfg.Re, cfg.alp, cfg.bet, cfg.pr,
+                     cfg.cfl, cfg.ncfl, sc["dt"], 0, 0)
+        r.upload(s["vor"], s["phi"], s["psi"], s["scal"], s["u00"], s["v00"], s["w00"])
+        r.advance(2)
+        out = r.download()
+        r.close()
+        return out
+
+    da, db = start(sa), start(back)
+    for f in FIELDS:
+        assert np.array_equal(da[f], db[f]), f
