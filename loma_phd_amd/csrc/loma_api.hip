@@ -179,7 +179,7 @@ static size_t plan_bytes(int mgalx, int my, int mgalz, bool fp32, bool with_z) {
   const size_t P = (size_t)mgalx * my * mgalz, es = fp32 ? 8 : 16;
   const size_t nsp = with_z ? 12 : 11, nst = with_z ? 7 : 6;
   const size_t tw = fp32 ? 8 : 16;
-  size_t b = align256(8 * (size_t)ytab_offset(YT_COUNT, M));  // per-row and scan tables
+  size_t b = align256(12 * (size_t)ytab_offset(YT_COUNT, M));  // y tables, FP64 and FP32
   b += 2 * align256(8 * 5 * (size_t)my);                       // band copies
   b += align256(8 * nclass) + align256(8 * 5 * R * nclass) + align256(32 * nclass);
   b += align256(8 * 2 * R * nclass);                           // homogeneous solutions
@@ -450,13 +450,22 @@ int loma_init(loma_run** out, int mgalx, int my, int mgalz, const double* y, dou
   lev[3] = scan_table(my, M, 2, true, nullptr, f.up2.data(), f.inv2.data(), H(YT_HU2));
   lev[4] = scan_table(my, M, 3, false, f.loi.data(), nullptr, nullptr, H(YT_HLI));
   lev[5] = scan_table(my, M, 3, true, nullptr, f.upi.data(), f.invi.data(), H(YT_HUI));
-  if ((rc = dalloc(r, &r->tab, tot * sizeof(double)))) { loma_free(r); return rc; }
+  // the FP64 tables, then their FP32 rounding in the same layout (the
+  // mixed form's FP32 y-operators read it)
+  std::vector<float> h32(h.begin(), h.end());
+  if ((rc = dalloc(r, &r->tab, tot * (sizeof(double) + sizeof(float))))) {
+    loma_free(r); return rc;
+  }
   if (hipMemcpyAsync(r->tab, h.data(), tot * sizeof(double), hipMemcpyHostToDevice, sst) !=
-      hipSuccess) {
+          hipSuccess ||
+      hipMemcpyAsync(r->tab + tot, h32.data(), tot * sizeof(float), hipMemcpyHostToDevice,
+                     sst) != hipSuccess ||
+      hipStreamSynchronize(sst) != hipSuccess) {
     set_error("loma_init: table upload failed"); loma_free(r); return LOMA_ERR_DEVICE;
   }
   YTab& T = r->T;
   T.base = r->tab;
+  T.base32 = reinterpret_cast<const float*>(r->tab + tot);
   T.kL1 = lev[0]; T.kU1 = lev[1]; T.kL2 = lev[2]; T.kU2 = lev[3]; T.kLi = lev[4];
   T.kUi = lev[5];
   std::memcpy(r->scan_levels, lev, sizeof(lev));

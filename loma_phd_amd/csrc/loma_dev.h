@@ -55,7 +55,7 @@ struct TabRef {
   __amdgpu_buffer_rsrc_t r;
   int ncol;
   int off;
-  int lane8;
+  int lane_off;
   int sbias;
 };
 __device__ __forceinline__ TabRef tab(const double* base, int ncol, int off, int lane) {
@@ -63,8 +63,20 @@ __device__ __forceinline__ TabRef tab(const double* base, int ncol, int off, int
 }
 __device__ __forceinline__ double tget(const TabRef& b, int t, int idx) {
   const auto v = __builtin_amdgcn_raw_buffer_load_b64(
-      b.r, b.lane8, b.sbias + (t * b.ncol + b.off + idx) * (WAVE * 8), 0);
+      b.r, b.lane_off, b.sbias + (t * b.ncol + b.off + idx) * (WAVE * 8), 0);
   return __builtin_bit_cast(double, v);
+}
+// entry of a table in the arithmetic type V: the FP64 table, or its FP32
+// copy (a view made by ytab_v<M, float>, whose lane offset counts 4 bytes)
+template <typename V>
+__device__ __forceinline__ V tget_v(const TabRef& b, int t, int idx) {
+  if constexpr (sizeof(V) == 8) {
+    return tget(b, t, idx);
+  } else {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b32(
+        b.r, b.lane_off, b.sbias + (t * b.ncol + b.off + idx) * (WAVE * 4), 0);
+    return __builtin_bit_cast(float, v);
+  }
 }
 // The same view, opaque to the optimiser: loads through it are issued again
 // instead of keeping coefficients loaded earlier live across a scan (table
@@ -72,16 +84,17 @@ __device__ __forceinline__ double tget(const TabRef& b, int t, int idx) {
 __device__ __forceinline__ TabRef relaunder(const TabRef& b) {
   int z = b.sbias;
   asm volatile("" : "+s"(z));
-  return TabRef{b.r, b.ncol, b.off, b.lane8, z};
+  return TabRef{b.r, b.ncol, b.off, b.lane_off, z};
 }
 // The same, additionally ordered after `dep` is computed: a level of a scan
 // loads its table entries once the previous level's carry exists, all
 // together, instead of the compiler hoisting every level's loads to the
 // start (and spilling them) or serialising them one by one.
-__device__ __forceinline__ TabRef relaunder_after(const TabRef& b, double dep) {
+template <typename V>
+__device__ __forceinline__ TabRef relaunder_after(const TabRef& b, V dep) {
   int z = b.sbias;
   asm volatile("" : "+s"(z) : "v"(dep));
-  return TabRef{b.r, b.ncol, b.off, b.lane8, z};
+  return TabRef{b.r, b.ncol, b.off, b.lane_off, z};
 }
 // index of entry `idx` of grid row `row` in a lane-major table
 template <int M>
@@ -344,20 +357,20 @@ __device__ __forceinline__ void usolve(const TabRef up, const TabRef inv, int la
 // Only the first `levels` levels are applied: the compositions beyond them
 // are below 1e-22 in max norm (the factors' inverses decay geometrically),
 // checked when the tables are built.
-template <int M, int P, int NC>
+template <int M, int P, int NC, typename V>
 __device__ __forceinline__ void lsolve_tab(const TabRef lo, const TabRef H, int levels,
-                                           int lane, double (&x)[NC][M]) {
+                                           int lane, V (&x)[NC][M]) {
 #pragma unroll
   for (int t = 1; t < M; ++t)
 #pragma unroll
     for (int i = 1; i <= P; ++i) {
       if (t - i < 0) continue;
-      const double l = tget(lo, t, i - 1);
+      const V l = tget_v<V>(lo, t, i - 1);
 #pragma unroll
       for (int n = 0; n < NC; ++n) x[n][t] -= l * x[n][t - i];
     }
   op_fence();
-  double c[NC][P];
+  V c[NC][P];
 #pragma unroll
   for (int n = 0; n < NC; ++n)
 #pragma unroll
@@ -369,12 +382,12 @@ __device__ __forceinline__ void lsolve_tab(const TabRef lo, const TabRef H, int 
       // the level's composed map is loaded before the shuffles and outside
       // the lane test, so its loads are in flight together
       const TabRef Hk = relaunder_after(H, c[0][0]);
-      double h[P][P];
+      V h[P][P];
 #pragma unroll
       for (int q = 0; q < P; ++q)
 #pragma unroll
-        for (int b = 0; b < P; ++b) h[q][b] = tget(Hk, k * P + q, b);
-      double cp[NC][P];
+        for (int b = 0; b < P; ++b) h[q][b] = tget_v<V>(Hk, k * P + q, b);
+      V cp[NC][P];
 #pragma unroll
       for (int n = 0; n < NC; ++n)
 #pragma unroll
@@ -390,26 +403,26 @@ __device__ __forceinline__ void lsolve_tab(const TabRef lo, const TabRef H, int 
     }
   }
   op_fence();
-  double s[NC][P];
+  V s[NC][P];
 #pragma unroll
   for (int n = 0; n < NC; ++n)
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-      const double v = __shfl_up(c[n][q], 1);
-      s[n][q] = lane == 0 ? 0.0 : v;
+      const V v = __shfl_up(c[n][q], 1);
+      s[n][q] = lane == 0 ? V(0) : v;
     }
   const TabRef lo2 = relaunder(lo);
-  double dl[NC][M];
+  V dl[NC][M];
 #pragma unroll
   for (int t = 0; t < M; ++t) {
 #pragma unroll
-    for (int n = 0; n < NC; ++n) dl[n][t] = 0.0;
+    for (int n = 0; n < NC; ++n) dl[n][t] = V(0);
 #pragma unroll
     for (int i = 1; i <= P; ++i) {
-      const double l = tget(lo2, t, i - 1);
+      const V l = tget_v<V>(lo2, t, i - 1);
 #pragma unroll
       for (int n = 0; n < NC; ++n) {
-        const double pv = (t - i >= 0) ? dl[n][t - i] : s[n][i - t - 1];
+        const V pv = (t - i >= 0) ? dl[n][t - i] : s[n][i - t - 1];
         dl[n][t] -= l * pv;
       }
     }
@@ -418,24 +431,24 @@ __device__ __forceinline__ void lsolve_tab(const TabRef lo, const TabRef H, int 
   }
 }
 
-template <int M, int P, int NC>
+template <int M, int P, int NC, typename V>
 __device__ __forceinline__ void usolve_tab(const TabRef up, const TabRef inv, const TabRef H,
-                                           int levels, int lane, double (&x)[NC][M]) {
+                                           int levels, int lane, V (&x)[NC][M]) {
 #pragma unroll
   for (int t = M - 1; t >= 0; --t) {
 #pragma unroll
     for (int d = 1; d <= P; ++d) {
       if (t + d >= M) continue;
-      const double u = tget(up, t, d - 1);
+      const V u = tget_v<V>(up, t, d - 1);
 #pragma unroll
       for (int n = 0; n < NC; ++n) x[n][t] -= u * x[n][t + d];
     }
-    const double iv = tget(inv, t, 0);
+    const V iv = tget_v<V>(inv, t, 0);
 #pragma unroll
     for (int n = 0; n < NC; ++n) x[n][t] *= iv;
   }
   op_fence();
-  double c[NC][P];
+  V c[NC][P];
 #pragma unroll
   for (int n = 0; n < NC; ++n)
 #pragma unroll
@@ -445,12 +458,12 @@ __device__ __forceinline__ void usolve_tab(const TabRef up, const TabRef inv, co
     if (k < levels) {
       const int d = 1 << k;
       const TabRef Hk = relaunder_after(H, c[0][0]);
-      double h[P][P];
+      V h[P][P];
 #pragma unroll
       for (int q = 0; q < P; ++q)
 #pragma unroll
-        for (int b = 0; b < P; ++b) h[q][b] = tget(Hk, k * P + q, b);
-      double cp[NC][P];
+        for (int b = 0; b < P; ++b) h[q][b] = tget_v<V>(Hk, k * P + q, b);
+      V cp[NC][P];
 #pragma unroll
       for (int n = 0; n < NC; ++n)
 #pragma unroll
@@ -466,30 +479,30 @@ __device__ __forceinline__ void usolve_tab(const TabRef up, const TabRef inv, co
     }
   }
   op_fence();
-  double s[NC][P];
+  V s[NC][P];
 #pragma unroll
   for (int n = 0; n < NC; ++n)
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-      const double v = __shfl_down(c[n][q], 1);
-      s[n][q] = lane == WAVE - 1 ? 0.0 : v;
+      const V v = __shfl_down(c[n][q], 1);
+      s[n][q] = lane == WAVE - 1 ? V(0) : v;
     }
   const TabRef up2 = relaunder(up), inv2 = relaunder(inv);
-  double dl[NC][M];
+  V dl[NC][M];
 #pragma unroll
   for (int t = M - 1; t >= 0; --t) {
 #pragma unroll
-    for (int n = 0; n < NC; ++n) dl[n][t] = 0.0;
+    for (int n = 0; n < NC; ++n) dl[n][t] = V(0);
 #pragma unroll
     for (int d = 1; d <= P; ++d) {
-      const double u = tget(up2, t, d - 1);
+      const V u = tget_v<V>(up2, t, d - 1);
 #pragma unroll
       for (int n = 0; n < NC; ++n) {
-        const double pv = (t + d < M) ? dl[n][t + d] : s[n][t + d - M];
+        const V pv = (t + d < M) ? dl[n][t + d] : s[n][t + d - M];
         dl[n][t] -= u * pv;
       }
     }
-    const double iv = tget(inv2, t, 0);
+    const V iv = tget_v<V>(inv2, t, 0);
 #pragma unroll
     for (int n = 0; n < NC; ++n) {
       dl[n][t] *= iv;
@@ -499,31 +512,31 @@ __device__ __forceinline__ void usolve_tab(const TabRef up, const TabRef inv, co
 }
 
 // ---- banded matvec y_r = sum_d B(r,d) u_{r-P+d} ---------------------------
-template <int M, int P, int NC>
-__device__ __forceinline__ void bmatvec(const TabRef B, int lane, const double (&u)[NC][M],
-                                        double (&y)[NC][M]) {
-  double hl[NC][P], hr[NC][P];
+template <int M, int P, int NC, typename V>
+__device__ __forceinline__ void bmatvec(const TabRef B, int lane, const V (&u)[NC][M],
+                                        V (&y)[NC][M]) {
+  V hl[NC][P], hr[NC][P];
 #pragma unroll
   for (int n = 0; n < NC; ++n)
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-      const double a = __shfl_up(u[n][M - P + q], 1);
-      const double b = __shfl_down(u[n][q], 1);
-      hl[n][q] = lane == 0 ? 0.0 : a;
-      hr[n][q] = lane == WAVE - 1 ? 0.0 : b;
+      const V a = __shfl_up(u[n][M - P + q], 1);
+      const V b = __shfl_down(u[n][q], 1);
+      hl[n][q] = lane == 0 ? V(0) : a;
+      hr[n][q] = lane == WAVE - 1 ? V(0) : b;
     }
 #pragma unroll
   for (int t = 0; t < M; ++t) {
-    double acc[NC];
+    V acc[NC];
 #pragma unroll
-    for (int n = 0; n < NC; ++n) acc[n] = 0.0;
+    for (int n = 0; n < NC; ++n) acc[n] = V(0);
 #pragma unroll
     for (int d = 0; d <= 2 * P; ++d) {
       const int col = t - P + d;
-      const double bv = tget(B, t, d);
+      const V bv = tget_v<V>(B, t, d);
 #pragma unroll
       for (int n = 0; n < NC; ++n) {
-        const double v = col < 0 ? hl[n][col + P] : (col >= M ? hr[n][col - M] : u[n][col]);
+        const V v = col < 0 ? hl[n][col + P] : (col >= M ? hr[n][col - M] : u[n][col]);
         acc[n] += bv * v;
       }
     }
@@ -579,6 +592,7 @@ __host__ __device__ constexpr int ytab_offset(int id, int M) {
 // table access goes through one buffer resource with a constant offset.
 struct YTab {
   const double* base;
+  const float* base32;  // the same tables rounded to FP32, same layout
   // scan level counts of the fixed factors (see lsolve_tab)
   int kL1, kU1, kL2, kU2, kLi, kUi;
   int my;
@@ -589,23 +603,31 @@ template <int M>
 __device__ __forceinline__ TabRef ytab(const YTab& T, int id, int ncol, int lane) {
   return TabRef{make_rsrc(T.base, 0x7fffffff), ncol, 0, lane * 8, ytab_offset(id, M) * 8};
 }
+// ... in the arithmetic type V (read with tget_v<V>)
+template <int M, typename V>
+__device__ __forceinline__ TabRef ytab_v(const YTab& T, int id, int ncol, int lane) {
+  if constexpr (sizeof(V) == 8)
+    return ytab<M>(T, id, ncol, lane);
+  else
+    return TabRef{make_rsrc(T.base32, 0x7fffffff), ncol, 0, lane * 4, ytab_offset(id, M) * 4};
+}
 
 // first derivative du = fmap * A1^{-1} B1 u   (deryr2, cfdiff.f:518-601)
-template <int M, int NC>
-__device__ __forceinline__ void op_dy1(const YTab& T, int lane, const double (&u)[NC][M],
-                                       double (&du)[NC][M], bool apply_fmap = true) {
-  bmatvec<M, 3, NC>(ytab<M>(T, YT_B1, 7, lane), lane, u, du);
+template <int M, int NC, typename V>
+__device__ __forceinline__ void op_dy1(const YTab& T, int lane, const V (&u)[NC][M],
+                                       V (&du)[NC][M], bool apply_fmap = true) {
+  bmatvec<M, 3, NC>(ytab_v<M, V>(T, YT_B1, 7, lane), lane, u, du);
   op_fence();
-  lsolve_tab<M, 3, NC>(ytab<M>(T, YT_LO1, 3, lane), ytab<M>(T, YT_HL1, 3, lane), T.kL1, lane, du);
+  lsolve_tab<M, 3, NC>(ytab_v<M, V>(T, YT_LO1, 3, lane), ytab_v<M, V>(T, YT_HL1, 3, lane), T.kL1, lane, du);
   op_fence();
-  usolve_tab<M, 3, NC>(ytab<M>(T, YT_UP1, 3, lane), ytab<M>(T, YT_INV1, 1, lane),
-                       ytab<M>(T, YT_HU1, 3, lane), T.kU1, lane, du);
+  usolve_tab<M, 3, NC>(ytab_v<M, V>(T, YT_UP1, 3, lane), ytab_v<M, V>(T, YT_INV1, 1, lane),
+                       ytab_v<M, V>(T, YT_HU1, 3, lane), T.kU1, lane, du);
   op_fence();
   if (apply_fmap) {
-    const TabRef fm = ytab<M>(T, YT_FMAP, 1, lane);
+    const TabRef fm = ytab_v<M, V>(T, YT_FMAP, 1, lane);
 #pragma unroll
     for (int t = 0; t < M; ++t) {
-      const double f = tget(fm, t, 0);
+      const V f = tget_v<V>(fm, t, 0);
 #pragma unroll
       for (int n = 0; n < NC; ++n) du[n][t] *= f;
     }
@@ -614,15 +636,15 @@ __device__ __forceinline__ void op_dy1(const YTab& T, int lane, const double (&u
 }
 
 // second derivative d2u = A2^{-1} B2 u   (deryyr2, cfdiff.f:735-797)
-template <int M, int NC>
-__device__ __forceinline__ void op_dy2(const YTab& T, int lane, const double (&u)[NC][M],
-                                       double (&d2u)[NC][M]) {
-  bmatvec<M, 2, NC>(ytab<M>(T, YT_B2, 5, lane), lane, u, d2u);
+template <int M, int NC, typename V>
+__device__ __forceinline__ void op_dy2(const YTab& T, int lane, const V (&u)[NC][M],
+                                       V (&d2u)[NC][M]) {
+  bmatvec<M, 2, NC>(ytab_v<M, V>(T, YT_B2, 5, lane), lane, u, d2u);
   op_fence();
-  lsolve_tab<M, 2, NC>(ytab<M>(T, YT_LO2, 2, lane), ytab<M>(T, YT_HL2, 2, lane), T.kL2, lane, d2u);
+  lsolve_tab<M, 2, NC>(ytab_v<M, V>(T, YT_LO2, 2, lane), ytab_v<M, V>(T, YT_HL2, 2, lane), T.kL2, lane, d2u);
   op_fence();
-  usolve_tab<M, 2, NC>(ytab<M>(T, YT_UP2, 2, lane), ytab<M>(T, YT_INV2, 1, lane),
-                       ytab<M>(T, YT_HU2, 2, lane), T.kU2, lane, d2u);
+  usolve_tab<M, 2, NC>(ytab_v<M, V>(T, YT_UP2, 2, lane), ytab_v<M, V>(T, YT_INV2, 1, lane),
+                       ytab_v<M, V>(T, YT_HU2, 2, lane), T.kU2, lane, d2u);
   op_fence();
 }
 
@@ -762,6 +784,102 @@ __device__ __forceinline__ void store_col(T2* __restrict__ col, int my, int lane
   wave_sync();
   stage_put<M>(st, lane, x);
   stage_out<M>(st, my, lane, col);
+}
+
+// ---- FP32 staging (the mixed form's phase C) ------------------------------
+// The same movements with a float2 stage (half the LDS of a double2 stage)
+// for FP32 spectral columns, and FP64 state updates done in the coalesced
+// order straight from HBM: the state never passes through registers in
+// chunk order, so it needs no FP64 stage.
+__device__ __forceinline__ float2 buf_ld32(__amdgpu_buffer_rsrc_t r, int voff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  return make_float2(__builtin_bit_cast(float, (unsigned)u),
+                     __builtin_bit_cast(float, (unsigned)(u >> 32)));
+}
+template <int M>
+__device__ __forceinline__ void stage_in32(const float2* __restrict__ col, int my, int lane,
+                                           float2* st) {
+  const auto r = make_rsrc(col, my * 8);
+  float2 q[M];
+#pragma unroll
+  for (int t = 0; t < M; ++t) q[t] = buf_ld32(r, col_voff<float2>(lane, t));
+  wave_sync();
+#pragma unroll
+  for (int t = 0; t < M; ++t) st[t * WAVE + lane] = q[t];
+  wave_sync();
+}
+template <int M>
+__device__ __forceinline__ void stage_get32(const float2* st, int lane, float (&x)[2][M]) {
+  const int r0 = lane * M;
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    const float2 v = st[r0 + t];
+    x[0][t] = v.x;
+    x[1][t] = v.y;
+  }
+}
+template <int M>
+__device__ __forceinline__ void stage_put32(float2* st, int lane, const float (&x)[2][M]) {
+  const int r0 = lane * M;
+#pragma unroll
+  for (int t = 0; t < M; ++t) st[r0 + t] = make_float2(x[0][t], x[1][t]);
+}
+template <int M>
+__device__ __forceinline__ void load_col32(const float2* __restrict__ col, int my, int lane,
+                                           float2* st, float (&x)[2][M]) {
+  stage_in32<M>(col, my, lane, st);
+  stage_get32<M>(st, lane, x);
+}
+// FP64 state column += s * x and the FP64 history column = x, for an FP32
+// column x (own chunk): staged to the coalesced order, then each lane
+// updates its rows p = t*64 + lane in FP64 (rows >= my dropped by the range
+// checks)
+template <int M>
+__device__ __forceinline__ void update_col32(double2* __restrict__ state,
+                                             double2* __restrict__ hist, int my, int lane,
+                                             float2* st, const float (&x)[2][M], double s) {
+  wave_sync();
+  stage_put32<M>(st, lane, x);
+  wave_sync();
+  float2 v[M];
+#pragma unroll
+  for (int t = 0; t < M; ++t) v[t] = st[t * WAVE + lane];
+  wave_sync();
+  const auto rs = make_rsrc(state, my * 16), rh = make_rsrc(hist, my * 16);
+  double2 h[M];
+#pragma unroll
+  for (int t = 0; t < M; ++t) h[t] = buf_ld(rs, col_voff<double2>(lane, t), (double2*)nullptr);
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    const double2 a = make_double2((double)v[t].x, (double)v[t].y);
+    buf_st(rs, col_voff<double2>(lane, t), make_double2(h[t].x + s * a.x, h[t].y + s * a.y),
+           (double2*)nullptr);
+    buf_st(rh, col_voff<double2>(lane, t), a, (double2*)nullptr);
+  }
+}
+// FP64 state column += an FP32 spectral increment (plus, for the 00 mode,
+// its FP64 mean profile `mean`; null otherwise), in the coalesced order
+template <int M>
+__device__ __forceinline__ void add_col32(double2* __restrict__ state,
+                                          const float2* __restrict__ inc, int my, int lane,
+                                          const double* __restrict__ mean) {
+  const auto rs = make_rsrc(state, my * 16), ri = make_rsrc(inc, my * 8);
+  double2 h[M];
+  float2 d[M];
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    h[t] = buf_ld(rs, col_voff<double2>(lane, t), (double2*)nullptr);
+    d[t] = buf_ld32(ri, col_voff<float2>(lane, t));
+  }
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    const int p = t * WAVE + lane;
+    double dr = (double)d[t].x;
+    if (mean != nullptr && p < my) dr += mean[p];
+    buf_st(rs, col_voff<double2>(lane, t), make_double2(h[t].x + dr, h[t].y + (double)d[t].y),
+           (double2*)nullptr);
+  }
 }
 
 }  // namespace loma

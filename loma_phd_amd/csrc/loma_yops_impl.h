@@ -31,6 +31,11 @@ static __constant__ double c_xi[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
   __shared__ double2 stage_buf_[4][2][WAVE * (M)];    \
   double2* st = stage_buf_[threadIdx.x >> 6][0];      \
   double2* pk = stage_buf_[threadIdx.x >> 6][1]
+// the same with float2 columns (FP32 phase C)
+#define LOMA_STAGE2_F32(M)                            \
+  __shared__ float2 stage_buf_[4][2][WAVE * (M)];     \
+  float2* st = stage_buf_[threadIdx.x >> 6][0];       \
+  float2* pk = stage_buf_[threadIdx.x >> 6][1]
 
 // sum over the wavefront
 __device__ __forceinline__ double wave_sum(double v) {
@@ -483,6 +488,21 @@ __global__ __launch_bounds__(256, 2) void k_phaseC_batch(const PhaseArgs* __rest
 #include "loma_phaseC_body.inc"
 }
 
+// phase C of the mixed form: RHS in FP32, state updates in FP64
+template <int M>
+__global__ __launch_bounds__(256, 3) void k_phaseC32(PhaseArgs a) {
+  const int bx = blockIdx.x;
+#include "loma_phaseC32_body.inc"
+}
+template <int M>
+__global__ __launch_bounds__(256, 3) void k_phaseC32_batch(const PhaseArgs* __restrict__ args,
+    const int* __restrict__ first, int nrun) {
+  const int r = wave_uniform(batch_run(first, nrun, blockIdx.x));
+  const PhaseArgs& a = args[r];
+  const int bx = blockIdx.x - first[r];
+#include "loma_phaseC32_body.inc"
+}
+
 // (8) psi from the low-Mach constraint and the 00 mode of rho*v
 // (cross.f:1027-1104).  A kernel of its own: it reads only drho and writes
 // psi, so splitting it from the rest of phase C adds no HBM traffic.
@@ -798,7 +818,7 @@ int launch_phase_m(int which, bool fp32, const PhaseArgs& a, hipStream_t st) {
     if (fp32) hipLaunchKernelGGL((k_phaseA<M, float>), g, b, 0, st, a);
     else hipLaunchKernelGGL((k_phaseA<M, double>), g, b, 0, st, a);
   } else if (which == 1) {
-    if (fp32) hipLaunchKernelGGL((k_phaseC<M, float>), g, b, 0, st, a);
+    if (fp32) hipLaunchKernelGGL((k_phaseC32<M>), g, b, 0, st, a);
     else hipLaunchKernelGGL((k_phaseC<M, double>), g, b, 0, st, a);
   } else {
     // one wavefront per Helmholtz class (the +-kz pair)
@@ -817,7 +837,7 @@ int launch_phase_batch_m(int which, bool fp32, const PhaseArgs* args, const int*
     if (fp32) hipLaunchKernelGGL((k_phaseA_batch<M, float>), g, b, 0, st, args, first, nrun);
     else hipLaunchKernelGGL((k_phaseA_batch<M, double>), g, b, 0, st, args, first, nrun);
   } else if (which == 1) {
-    if (fp32) hipLaunchKernelGGL((k_phaseC_batch<M, float>), g, b, 0, st, args, first, nrun);
+    if (fp32) hipLaunchKernelGGL((k_phaseC32_batch<M>), g, b, 0, st, args, first, nrun);
     else hipLaunchKernelGGL((k_phaseC_batch<M, double>), g, b, 0, st, args, first, nrun);
   } else {
     if (fp32) hipLaunchKernelGGL((k_phaseC2_batch<M, float>), g, b, 0, st, args, first, nrun);
