@@ -158,6 +158,23 @@ constexpr int zf_pad() { return sizeof(T) == 8 ? 1 : LOMA_ZF_PAD32; }
 #ifndef LOMA_TW_REC
 #define LOMA_TW_REC 1
 #endif
+// the powers of a loaded first twiddle p1 (LOMA_TW_REC's products)
+template <int R, typename T2>
+__device__ __forceinline__ void twiddle_pow(T2* w, T2 p1, int s) {
+  T2 p[R];
+  p[1] = p1;
+  if (s > 0) p[1].y = -p[1].y;
+  if constexpr (R >= 3) p[2] = cmul(p[1], p[1]);
+  if constexpr (R >= 4) p[3] = cmul(p[2], p[1]);
+  if constexpr (R >= 5) p[4] = cmul(p[2], p[2]);
+  if constexpr (R >= 8) {
+    p[5] = cmul(p[4], p[1]);
+    p[6] = cmul(p[3], p[3]);
+    p[7] = cmul(p[4], p[3]);
+  }
+#pragma unroll
+  for (int r = 1; r < R; ++r) w[r] = cmul(w[r], p[r]);
+}
 template <int R, typename T2>
 __device__ __forceinline__ void twiddle(T2* w, const T2* __restrict__ tw, int jm, int tstep,
                                         int s) {
@@ -194,6 +211,28 @@ __device__ __forceinline__ void stockham_item(const T2* __restrict__ src, T2* __
 #pragma unroll
   for (int r = 0; r < R; ++r) v[r] = src[lds_i<PAD>(rowbase + j + r * nb)];
   if (Ns > 1) twiddle<R>(v, tw, jm, tstep, s);
+  if (R == 2) dft2<T, T2>(v);
+  if (R == 3) dft3<T, T2>(v, s);
+  if (R == 4) dft4<T, T2>(v, s);
+  if (R == 5) dft5<T, T2>(v, s);
+  if (R == 8) dft8<T, T2>(v, s);
+  const int idxD = rowbase + (j / Ns) * Ns * R + jm;
+#pragma unroll
+  for (int r = 0; r < R; ++r) dst[lds_i<PAD>(idxD + r * Ns)] = v[r];
+}
+
+// The same butterfly with its first twiddle p1 = tw[(j % Ns) * tstep]
+// already in a register (no global load inside the pass)
+template <int R, typename T, typename T2, int PAD = LOMA_LDS_PAD>
+__device__ __forceinline__ void stockham_item_p1(const T2* __restrict__ src,
+                                                 T2* __restrict__ dst, int rowbase, int j,
+                                                 int nb, int Ns, T2 p1, int s) {
+  static_assert(LOMA_TW_REC, "register twiddles are the first power only");
+  T2 v[R];
+  const int jm = j % Ns;
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = src[lds_i<PAD>(rowbase + j + r * nb)];
+  if (Ns > 1) twiddle_pow<R>(v, p1, s);
   if (R == 2) dft2<T, T2>(v);
   if (R == 3) dft3<T, T2>(v, s);
   if (R == 4) dft4<T, T2>(v, s);
@@ -651,16 +690,11 @@ constexpr int xl_pad() {
   return sizeof(T) == 8 ? LOMA_XL_PAD64 : LOMA_XL_PAD32;
 }
 
-// lines of NF fields (W layout [i][z][j]; src1 unused for NF = 1) ->
-// Hermitian-packed complex rows in A (row f*pairs + p = lines j0+2p,
-// j0+2p+1 of field f); no barrier.
+// fetched lines of NF fields (xl_fetch) -> Hermitian-packed complex rows in
+// A (row f*pairs + p = lines j0+2p, j0+2p+1 of field f); no barrier.
 template <typename T, typename T2, int XR, int NN, int NF>
-__device__ __forceinline__ void xl_pack(const FftArgs& a, const T2* __restrict__ src0,
-                                        const T2* __restrict__ src1, T2* A, int z, int j0,
-                                        int pairs) {
-  XlIn<T2, XR> v[NF];
-  xl_fetch<T2, XR, NN>(a, src0, z, j0, pairs, v[0]);
-  if constexpr (NF > 1) xl_fetch<T2, XR, NN>(a, src1, z, j0, pairs, v[1]);
+__device__ __forceinline__ void xl_pack_from(const FftArgs& a, const XlIn<T2, XR>* v, T2* A,
+                                             int pairs) {
   constexpr int XP = xl_pad<T>();
   const int N = NN ? NN : a.rc.mgalx, np = NN ? NN / 3 : a.rc.nplanes;
   const int lp = __builtin_ctz(pairs);
@@ -690,6 +724,17 @@ __device__ __forceinline__ void xl_pack(const FftArgs& a, const T2* __restrict__
       }
     }
   }
+}
+// lines of NF fields (W layout [i][z][j]; src1 unused for NF = 1) -> the
+// packed rows in A; no barrier.
+template <typename T, typename T2, int XR, int NN, int NF>
+__device__ __forceinline__ void xl_pack(const FftArgs& a, const T2* __restrict__ src0,
+                                        const T2* __restrict__ src1, T2* A, int z, int j0,
+                                        int pairs) {
+  XlIn<T2, XR> v[NF];
+  xl_fetch<T2, XR, NN>(a, src0, z, j0, pairs, v[0]);
+  if constexpr (NF > 1) xl_fetch<T2, XR, NN>(a, src1, z, j0, pairs, v[1]);
+  xl_pack_from<T, T2, XR, NN, NF>(a, v, A, pairs);
 }
 
 // Hermitian unpacking of the transformed pairs in R into fields dst0 (rows
@@ -776,6 +821,53 @@ __device__ __forceinline__ void xl_sync_x() {
   else __syncthreads();
 }
 
+// LOMA_XL_PF = 1: the register-resident path takes the first twiddles of
+// its passes from registers (loaded once per workgroup) and fetches the next
+// input field while the current one is transformed.  vmcnt counts loads and
+// stores in order, so a twiddle load inside a pass would make the pass wait
+// for the prefetch (and for the previous field's stores).
+#ifndef LOMA_XL_PF
+#define LOMA_XL_PF 1
+#endif
+static_assert(!LOMA_XL_PF || LOMA_XL_WAVE == 2, "the prefetch feeds the workgroup pack");
+// first twiddles of the register-resident passes for lane j: tw[(j % 8) * 8]
+// (the Ns = 8 passes) and tw[j] (Ns = 64)
+template <typename T2>
+struct XlTw {
+  T2 w8, w64;
+};
+template <typename T2>
+__device__ __forceinline__ XlTw<T2> xl_tw(const FftArgs& a) {
+  const T2* tw = (const T2*)a.twx;
+  const int j = threadIdx.x & 63;
+  return XlTw<T2>{tw[(j & 7) * 8], tw[j]};
+}
+// inverse transform of a fetched field (LOMA_XL_PF)
+template <typename T, typename T2, int XR>
+__device__ __forceinline__ void xl_inverse_reg_in(const FftArgs& a, const XlIn<T2, XR>& in,
+                                                  T2* A, T2* B, int pairs, const XlTw<T2>& tw,
+                                                  T (&v)[8][2]) {
+  constexpr int RS = 512 + LOMA_XL_RS1;
+  const int row = threadIdx.x >> 6, j = threadIdx.x & 63;
+  xl_pack_from<T, T2, XR, 512, 1>(a, &in, A, pairs);
+  xl_sync_x<T, T2>();
+  stockham_item_p1<8, T, T2, 0>(A, B, row * RS, j, 64, 1, tw.w8, +1);  // Ns = 1: no twiddles
+  xl_sync<T, T2>();
+  stockham_item_p1<8, T, T2, 0>(B, A, row * RS, j, 64, 8, tw.w8, +1);
+  xl_sync<T, T2>();
+  T2 w[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) w[r] = A[row * RS + j + r * 64];
+  twiddle_pow<8>(w, tw.w64, +1);
+  dft8<T, T2>(w, +1);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    v[r][0] = w[r].x;
+    v[r][1] = w[r].y;
+  }
+  xl_sync_x<T, T2>();  // A is packed again by the next field
+}
+
 template <typename T, typename T2, int XR>
 __device__ __forceinline__ void xl_inverse_reg(const FftArgs& a, const T2* __restrict__ src,
                                                T2* A, T2* B, int z, int j0, int pairs,
@@ -835,10 +927,10 @@ __device__ __forceinline__ void xl_inverse_reg(const FftArgs& a, const T2* __res
   xl_sync_x<T, T2>();  // A is packed again by the next field
 }
 
-template <typename T, typename T2>
+template <typename T, typename T2, bool PF>
 __device__ __forceinline__ void xl_forward_reg(const FftArgs& a, T2* __restrict__ dst, T2* A,
                                                T2* B, int z, int j0, int pairs,
-                                               const T (&v)[8][2]) {
+                                               const T (&v)[8][2], const XlTw<T2>& twr) {
   static_assert(xl_pad<T>() == 0, "register ends assume unpadded rows");
   constexpr int RS = 512 + LOMA_XL_RS1;
   const T2* tw = (const T2*)a.twx;
@@ -850,9 +942,15 @@ __device__ __forceinline__ void xl_forward_reg(const FftArgs& a, T2* __restrict_
 #pragma unroll
   for (int r = 0; r < 8; ++r) A[row * RS + 8 * j + r] = w[r];
   xl_sync<T, T2>();
-  stockham_item<8, T, T2, 0>(A, B, row * RS, j, 64, 8, 8, tw, -1);
-  xl_sync<T, T2>();
-  stockham_item<8, T, T2, 0>(B, A, row * RS, j, 64, 64, 1, tw, -1);
+  if constexpr (PF) {
+    stockham_item_p1<8, T, T2, 0>(A, B, row * RS, j, 64, 8, twr.w8, -1);
+    xl_sync<T, T2>();
+    stockham_item_p1<8, T, T2, 0>(B, A, row * RS, j, 64, 64, twr.w64, -1);
+  } else {
+    stockham_item<8, T, T2, 0>(A, B, row * RS, j, 64, 8, 8, tw, -1);
+    xl_sync<T, T2>();
+    stockham_item<8, T, T2, 0>(B, A, row * RS, j, 64, 64, 1, tw, -1);
+  }
   xl_sync_x<T, T2>();
   if constexpr (LOMA_XL_WAVE == 1) {
     // wave `row` unpacks and stores line pair `row`
