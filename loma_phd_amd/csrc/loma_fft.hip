@@ -351,16 +351,21 @@ constexpr int ct_radix(int m) {
 #ifndef LOMA_XL_WR
 #define LOMA_XL_WR 1
 #endif
+// TWR (with WR, N = 512): the first twiddles of the passes come from a
+// copy in LDS, twl[j] = tw[j] (Ns = 64) and twl[64 + j % 8] = tw[(j % 8) * 8]
+// (Ns = 8); the passes then issue no global loads, so they do not wait
+// (vmcnt counts loads and stores in order) for the previous field's stores.
 template <typename T, typename T2, int N, int Ns, int NT, int RS, int PAD = LOMA_LDS_PAD,
-          bool WR = false>
+          bool WR = false, bool TWR = false>
 __device__ __forceinline__ T2* fft_rows_ct(T2* a, T2* b, int nrows, const T2* __restrict__ tw,
-                                           int s) {
+                                           int s, const T2* twl = nullptr) {
   if constexpr (Ns >= N) {
     return a;
   } else {
     constexpr int R = ct_radix(N / Ns);
     constexpr int nb = N / R;
     constexpr int tstep = N / (Ns * R);
+    static_assert(!TWR || (WR && N == 512 && R == 8), "register twiddles: 512 = 8 x 8 x 8");
     // the pass's index arithmetic starts here, not hoisted into earlier
     // passes where it would hold registers across them
     int t0 = threadIdx.x;
@@ -368,12 +373,16 @@ __device__ __forceinline__ T2* fft_rows_ct(T2* a, T2* b, int nrows, const T2* __
     const int total = nrows * nb;
     for (int it = t0; it < total; it += NT) {
       const int row = it / nb, j = it - row * nb;
-      stockham_item<R, T, T2, PAD>(a, b, row * RS, j, nb, Ns, tstep, tw, s);
+      if constexpr (TWR)
+        stockham_item_p1<R, T, T2, PAD>(a, b, row * RS, j, nb, Ns,
+                                        Ns == 8 ? twl[64 + (j & 7)] : twl[j], s);
+      else
+        stockham_item<R, T, T2, PAD>(a, b, row * RS, j, nb, Ns, tstep, tw, s);
     }
     static_assert(!WR || (nb == 64 && NT % 64 == 0), "wave-local rows need 64 items per row");
     if constexpr (WR && Ns * R < N) wave_sync();
     else __syncthreads();
-    return fft_rows_ct<T, T2, N, Ns * R, NT, RS, PAD, WR>(b, a, nrows, tw, s);
+    return fft_rows_ct<T, T2, N, Ns * R, NT, RS, PAD, WR, TWR>(b, a, nrows, tw, s, twl);
   }
 }
 
@@ -446,12 +455,15 @@ __device__ __forceinline__ void dft_r(T2* w, int s) {
 template <int NN>
 __device__ __forceinline__ int xl_row_stride(int N) { return (NN ? NN : N) + LOMA_XL_RS1; }
 
-template <typename T, typename T2, int NN, int NT, int PAD = LOMA_LDS_PAD>
+// TWR: the 512-point plan's first twiddles from the LDS copy twl
+// (fft_rows_ct)
+template <typename T, typename T2, int NN, int NT, int PAD = LOMA_LDS_PAD, bool TWR = false>
 __device__ __forceinline__ T2* fft_any(T2* a, T2* b, int N, int nrows, const FftPlan& pl,
-                                       const T2* __restrict__ tw, int s) {
+                                       const T2* __restrict__ tw, int s,
+                                       const T2* twl = nullptr) {
   if constexpr (NN > 0)
     return fft_rows_ct<T, T2, NN, 1, NT, NN + LOMA_XL_RS1, PAD,
-                       (NN == 512 && LOMA_XL_WR != 0)>(a, b, nrows, tw, s);
+                       (NN == 512 && LOMA_XL_WR != 0), TWR>(a, b, nrows, tw, s, twl);
   else
     return fft_rows<T, T2, PAD>(a, b, N, N + LOMA_XL_RS1, nrows, pl, tw, s);
 }
@@ -629,10 +641,10 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 __host__ inline int xcd_groups(int n) { return 8 * ((n + 7) / 8); }
 
 // Per-thread share of one input field's lines: XR (kx, line pair) items of
-// two complex coefficients, all loaded before the first is packed.
-// (Fetching the next field during the current field's FFT was measured:
-// no gain in FP64, and in FP32 the held registers cost more than the hidden
-// latency.)
+// two complex coefficients, all loaded before the first is packed.  The
+// FP32 512-point path fetches the next field's share while the current one
+// is transformed (LOMA_XL_PF); elsewhere the held registers cost more than
+// the hidden latency (FP32 at 3 waves per SIMD) or gained nothing (FP64).
 template <typename T2, int XR>
 struct XlIn {
   T2 a[XR], b[XR];
@@ -767,26 +779,46 @@ __device__ __forceinline__ void xl_unpack(const FftArgs& a, const T2* R, T2* __r
   }
 }
 
+// first twiddles of the 512-point passes (radix 8, 8, 8) for lane j:
+// tw[(j % 8) * 8] (the Ns = 8 pass) and tw[j] (Ns = 64)
+template <typename T2>
+struct XlTw {
+  T2 w8, w64;
+};
+template <typename T2>
+__device__ __forceinline__ XlTw<T2> xl_tw(const FftArgs& a) {
+  const T2* tw = (const T2*)a.twx;
+  const int j = threadIdx.x & 63;
+  return XlTw<T2>{tw[(j & 7) * 8], tw[j]};
+}
+// LOMA_XL_TWR = 1: the 512-point x lines outside the register-resident path
+// (FP64) take those twiddles from a 72-entry LDS copy (fft_rows_ct's TWR;
+// in registers they held 8 VGPRs across the kernel and made it spill)
+#ifndef LOMA_XL_TWR
+#define LOMA_XL_TWR 1
+#endif
+
 // pack, then the inverse FFT.  Returns the buffer holding the physical pairs.
-template <typename T, typename T2, int XR, int NN, int NF>
+template <typename T, typename T2, int XR, int NN, int NF, bool TWR = false>
 __device__ __forceinline__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src0,
-                          const T2* __restrict__ src1, T2* A, T2* B, int z, int j0, int pairs) {
+                          const T2* __restrict__ src1, T2* A, T2* B, int z, int j0, int pairs,
+                          const T2* twl = nullptr) {
   xl_pack<T, T2, XR, NN, NF>(a, src0, src1, A, z, j0, pairs);
   __syncthreads();
   const int N = NN ? NN : a.rc.mgalx;
-  return fft_any<T, T2, NN, XL_THREADS, xl_pad<T>()>(A, B, N, NF * pairs, a.plx,
-                                                      (const T2*)a.twx, +1);
+  return fft_any<T, T2, NN, XL_THREADS, xl_pad<T>(), TWR>(A, B, N, NF * pairs, a.plx,
+                                                           (const T2*)a.twx, +1, twl);
 }
 
 // forward FFT of A (rows of NF outputs, filled by the caller, then
 // synchronised here) and the unpacking
-template <typename T, typename T2, int NN, int NF>
+template <typename T, typename T2, int NN, int NF, bool TWR = false>
 __device__ __forceinline__ void xl_forward(const FftArgs& a, T2* __restrict__ dst0, T2* __restrict__ dst1,
-                           T2* A, T2* B, int z, int j0, int pairs) {
+                           T2* A, T2* B, int z, int j0, int pairs, const T2* twl = nullptr) {
   const int N = NN ? NN : a.rc.mgalx;
   __syncthreads();
-  const T2* R = fft_any<T, T2, NN, XL_THREADS, xl_pad<T>()>(A, B, N, NF * pairs, a.plx,
-                                                             (const T2*)a.twx, -1);
+  const T2* R = fft_any<T, T2, NN, XL_THREADS, xl_pad<T>(), TWR>(A, B, N, NF * pairs, a.plx,
+                                                                  (const T2*)a.twx, -1, twl);
   xl_unpack<T, T2, NN, NF>(a, R, dst0, dst1, z, j0, pairs);
   __syncthreads();
 }
@@ -830,18 +862,6 @@ __device__ __forceinline__ void xl_sync_x() {
 #define LOMA_XL_PF 1
 #endif
 static_assert(!LOMA_XL_PF || LOMA_XL_WAVE == 2, "the prefetch feeds the workgroup pack");
-// first twiddles of the register-resident passes for lane j: tw[(j % 8) * 8]
-// (the Ns = 8 passes) and tw[j] (Ns = 64)
-template <typename T2>
-struct XlTw {
-  T2 w8, w64;
-};
-template <typename T2>
-__device__ __forceinline__ XlTw<T2> xl_tw(const FftArgs& a) {
-  const T2* tw = (const T2*)a.twx;
-  const int j = threadIdx.x & 63;
-  return XlTw<T2>{tw[(j & 7) * 8], tw[j]};
-}
 // inverse transform of a fetched field (LOMA_XL_PF)
 template <typename T, typename T2, int XR>
 __device__ __forceinline__ void xl_inverse_reg_in(const FftArgs& a, const XlIn<T2, XR>& in,
