@@ -546,11 +546,27 @@ __device__ __forceinline__ void zfft_reg(const T2* __restrict__ src, T2* __restr
   __syncthreads();
   // middle passes (Ns = R1 .. NSL)
   fft_rows_ip_ct<T, T2, E, N, R1, 256, PAD, NSL>(A, J, tw, sgn, v);
-  // last pass
+  // last pass.  FP64: every item's first twiddle is loaded before the first
+  // store (a load after a store waits for it: vmcnt counts both, in order)
+  // and pinned there so that the compiler does not sink the loads below the
+  // earlier items' stores: 0.949 -> 0.920 ms (inverse) and 1.224 -> 1.214 ms
+  // (forward) per launch at 512x513x256.  FP32 lost with it (inverse 0.458
+  // -> 0.478 ms) and loads each item's twiddle in the item.
   {
     constexpr int nit = (E + RL - 1) / RL;
     const int total = J * nbL;
     const T sc = INV ? (T)1 : (T)(1.0 / (double)N);
+    constexpr bool TWA = sizeof(T) == 8 && LOMA_TW_REC;
+    T2 tw1[nit];
+    if constexpr (TWA) {
+#pragma unroll
+      for (int q = 0; q < nit; ++q) {
+        const int it = threadIdx.x + q * 256;
+        tw1[q] = it < total ? tw[it / J] : T2{1, 0};
+      }
+#pragma unroll
+      for (int q = 0; q < nit; ++q) asm volatile("" : "+v"(tw1[q].x), "+v"(tw1[q].y));
+    }
 #pragma unroll
     for (int q = 0; q < nit; ++q) {
       const int it = threadIdx.x + q * 256;
@@ -559,7 +575,8 @@ __device__ __forceinline__ void zfft_reg(const T2* __restrict__ src, T2* __restr
         T2* w = v + q * RL;
 #pragma unroll
         for (int r = 0; r < RL; ++r) w[r] = A[jj * RS + lds_i<PAD>(j + r * nbL)];
-        twiddle<RL>(w, tw, j, 1, sgn);
+        if constexpr (TWA) twiddle_pow<RL>(w, tw1[q], sgn);
+        else twiddle<RL>(w, tw, j, 1, sgn);
         dft_r<RL, T, T2>(w, sgn);
         if (jy < my) {
 #pragma unroll
