@@ -816,26 +816,30 @@ __device__ __forceinline__ XlTw<T2> xl_tw(const FftArgs& a) {
 #endif
 
 // pack, then the inverse FFT.  Returns the buffer holding the physical pairs.
-template <typename T, typename T2, int XR, int NN, int NF, bool TWR = false>
+// TWR: the 512-point plan's first twiddles from the 72-entry LDS copy twl;
+// TWF: every twiddle from twl holding the whole table (other lengths)
+template <typename T, typename T2, int XR, int NN, int NF, bool TWR = false, bool TWF = false>
 __device__ __forceinline__ T2* xl_inverse(const FftArgs& a, const T2* __restrict__ src0,
                           const T2* __restrict__ src1, T2* A, T2* B, int z, int j0, int pairs,
                           const T2* twl = nullptr) {
   xl_pack<T, T2, XR, NN, NF>(a, src0, src1, A, z, j0, pairs);
   __syncthreads();
   const int N = NN ? NN : a.rc.mgalx;
-  return fft_any<T, T2, NN, XL_THREADS, xl_pad<T>(), TWR>(A, B, N, NF * pairs, a.plx,
-                                                           (const T2*)a.twx, +1, twl);
+  const T2* tw = TWF ? twl : (const T2*)a.twx;
+  return fft_any<T, T2, NN, XL_THREADS, xl_pad<T>(), TWR>(A, B, N, NF * pairs, a.plx, tw, +1,
+                                                           twl);
 }
 
 // forward FFT of A (rows of NF outputs, filled by the caller, then
 // synchronised here) and the unpacking
-template <typename T, typename T2, int NN, int NF, bool TWR = false>
+template <typename T, typename T2, int NN, int NF, bool TWR = false, bool TWF = false>
 __device__ __forceinline__ void xl_forward(const FftArgs& a, T2* __restrict__ dst0, T2* __restrict__ dst1,
                            T2* A, T2* B, int z, int j0, int pairs, const T2* twl = nullptr) {
   const int N = NN ? NN : a.rc.mgalx;
   __syncthreads();
+  const T2* tw = TWF ? twl : (const T2*)a.twx;
   const T2* R = fft_any<T, T2, NN, XL_THREADS, xl_pad<T>(), TWR>(A, B, N, NF * pairs, a.plx,
-                                                                  (const T2*)a.twx, -1, twl);
+                                                                  tw, -1, twl);
   xl_unpack<T, T2, NN, NF>(a, R, dst0, dst1, z, j0, pairs);
   __syncthreads();
 }
