@@ -33,10 +33,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # writes 1 GiB by these sums (python3 -m loma_phd_amd.profile_step; summary
 # in profiles/rk3_step_512x513x256_hbm_traffic.md).
 MEASURED_TRAFFIC = {
-    "fp64": {"phaseA": 4.266e9, "zfft_inv": 4.776e9, "xline": 8.020e9,
-             "zfft_fwd": 6.582e9, "phaseC": 4.319e9, "phaseC2": 1.030e9},
-    "mixed": {"phaseA": 3.310e9, "zfft_inv": 2.408e9, "xline": 4.564e9,
-              "zfft_fwd": 3.327e9, "phaseC": 3.130e9, "phaseC2": 1.009e9},
+    "fp64": {"phaseA": 4.262e9, "zfft_inv": 4.775e9, "xline": 8.034e9,
+             "zfft_fwd": 6.582e9, "phaseC": 4.328e9, "phaseC2": 0.978e9},
+    "mixed": {"phaseA": 3.309e9, "zfft_inv": 2.408e9, "xline": 4.553e9,
+              "zfft_fwd": 3.327e9, "phaseC": 3.112e9, "phaseC2": 0.853e9},
 }
 
 
