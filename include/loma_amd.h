@@ -33,8 +33,10 @@ extern "C" {
 
 /* precision modes */
 #define LOMA_FP64 0   /* everything in double */
-#define LOMA_MIXED 1  /* FP32 transforms + nonlinear products, FP64 y-solves,
-                         00 modes and RK accumulation */
+#define LOMA_MIXED 1  /* FP32 transforms, nonlinear products and the
+                         y-derivatives of the vorticity/phi right-hand sides;
+                         FP64 Helmholtz/Poisson solves, 00 modes and RK
+                         accumulation */
 
 /* flags */
 #define LOMA_FLAG_DT_FP32 1   /* round dt to float, as the tem module stores it
