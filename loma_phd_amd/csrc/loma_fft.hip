@@ -179,19 +179,7 @@ template <int R, typename T2>
 __device__ __forceinline__ void twiddle(T2* w, const T2* __restrict__ tw, int jm, int tstep,
                                         int s) {
   if constexpr (LOMA_TW_REC) {
-    T2 p[R];
-    p[1] = tw[jm * tstep];
-    if (s > 0) p[1].y = -p[1].y;
-    if constexpr (R >= 3) p[2] = cmul(p[1], p[1]);
-    if constexpr (R >= 4) p[3] = cmul(p[2], p[1]);
-    if constexpr (R >= 5) p[4] = cmul(p[2], p[2]);
-    if constexpr (R >= 8) {
-      p[5] = cmul(p[4], p[1]);
-      p[6] = cmul(p[3], p[3]);
-      p[7] = cmul(p[4], p[3]);
-    }
-#pragma unroll
-    for (int r = 1; r < R; ++r) w[r] = cmul(w[r], p[r]);
+    twiddle_pow<R>(w, tw[jm * tstep], s);
   } else {
 #pragma unroll
     for (int r = 1; r < R; ++r) {
