@@ -858,27 +858,57 @@ __device__ __forceinline__ void update_col32(double2* __restrict__ state,
     buf_st(rh, col_voff<double2>(lane, t), a, (double2*)nullptr);
   }
 }
-// FP64 state column += an FP32 spectral increment (plus, for the 00 mode,
-// its FP64 mean profile `mean`; null otherwise), in the coalesced order
-template <int M>
-__device__ __forceinline__ void add_col32(double2* __restrict__ state,
-                                          const float2* __restrict__ inc, int my, int lane,
-                                          const double* __restrict__ mean) {
-  const auto rs = make_rsrc(state, my * 16), ri = make_rsrc(inc, my * 8);
-  double2 h[M];
-  float2 d[M];
+// FP64 state column += a spectral increment (T2 = float2 or double2; plus,
+// for the 00 mode, its FP64 mean profile `mean`; null otherwise), in the
+// coalesced order
+template <int M, typename T2>
+__device__ __forceinline__ void add_col(double2* __restrict__ state, const T2* __restrict__ inc,
+                                        int my, int lane, const double* __restrict__ mean) {
+  const auto rs = make_rsrc(state, my * 16), ri = make_rsrc(inc, my * (int)sizeof(T2));
+  double2 h[M], d[M];
 #pragma unroll
   for (int t = 0; t < M; ++t) {
     h[t] = buf_ld(rs, col_voff<double2>(lane, t), (double2*)nullptr);
-    d[t] = buf_ld32(ri, col_voff<float2>(lane, t));
+    d[t] = buf_ld(ri, col_voff<T2>(lane, t), (T2*)nullptr);
   }
 #pragma unroll
   for (int t = 0; t < M; ++t) {
     const int p = t * WAVE + lane;
-    double dr = (double)d[t].x;
+    double dr = d[t].x;
     if (mean != nullptr && p < my) dr += mean[p];
-    buf_st(rs, col_voff<double2>(lane, t), make_double2(h[t].x + dr, h[t].y + (double)d[t].y),
+    buf_st(rs, col_voff<double2>(lane, t), make_double2(h[t].x + dr, h[t].y + d[t].y),
            (double2*)nullptr);
+  }
+}
+template <int M>
+__device__ __forceinline__ void add_col32(double2* __restrict__ state,
+                                          const float2* __restrict__ inc, int my, int lane,
+                                          const double* __restrict__ mean) {
+  add_col<M, float2>(state, inc, my, lane, mean);
+}
+// FP64 state column += s * x and the history column = x, for an FP64 column
+// x (own chunk), through the stage in the coalesced order (update_col32's
+// FP64 form)
+template <int M>
+__device__ __forceinline__ void update_col(double2* __restrict__ state,
+                                           double2* __restrict__ hist, int my, int lane,
+                                           double2* st, const double (&x)[2][M], double s) {
+  wave_sync();
+  stage_put<M>(st, lane, x);
+  wave_sync();
+  double2 v[M];
+#pragma unroll
+  for (int t = 0; t < M; ++t) v[t] = st[t * WAVE + lane];
+  wave_sync();
+  const auto rs = make_rsrc(state, my * 16), rh = make_rsrc(hist, my * 16);
+  double2 h[M];
+#pragma unroll
+  for (int t = 0; t < M; ++t) h[t] = buf_ld(rs, col_voff<double2>(lane, t), (double2*)nullptr);
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    buf_st(rs, col_voff<double2>(lane, t), make_double2(h[t].x + s * v[t].x, h[t].y + s * v[t].y),
+           (double2*)nullptr);
+    buf_st(rh, col_voff<double2>(lane, t), v[t], (double2*)nullptr);
   }
 }
 

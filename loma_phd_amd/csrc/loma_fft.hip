@@ -1026,8 +1026,11 @@ __device__ __forceinline__ void opaque_regs(T (&v)[E][2]) {
     for (int h = 0; h < 2; ++h) asm volatile("" : "+v"(v[m][h]));
 }
 
+// RK3 coefficients of a substep times dt; dtalpha = ab * dtbeta, so that
+// the constraint source drho = d/(T T+ dtbeta) - ab r/T^2 takes one division:
+// (d T - dtalpha r T+) / (T^2 T+ dtbeta)
 struct XlRk {
-  double dtxi, dtgamma, dtbeta, ab;
+  double dtxi, dtgamma, dtbeta, ab, dtalpha;
 };
 __device__ __forceinline__ XlRk xl_rk(double dt, int rkstep) {
   const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
@@ -1039,7 +1042,8 @@ __device__ __forceinline__ XlRk xl_rk(double dt, int rkstep) {
   const double betas[3] = {gam[0] - p0, p1, b3};
   const double alphas[3] = {p0, gam[1] + xis[1] - p1, gam[2] + xis[2] - b3};
   const int rk = rkstep - 1;
-  return XlRk{dt * xis[rk], dt * gam[rk], dt * betas[rk], alphas[rk] / betas[rk]};
+  return XlRk{dt * xis[rk], dt * gam[rk], dt * betas[rk], alphas[rk] / betas[rk],
+              dt * alphas[rk]};
 }
 
 // Compiled for two waves per SIMD (LOMA_XL_WPE; FP64 always): at three,
