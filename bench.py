@@ -40,6 +40,88 @@ MEASURED_TRAFFIC = {
 }
 
 
+_TRAFFIC_KEYS = {"k_phaseA": "phaseA", "k_zfft_inv": "zfft_inv", "k_xline": "xline",
+                 "k_zfft_fwd": "zfft_fwd", "k_phaseC": "phaseC", "k_phaseC32": "phaseC",
+                 "k_phaseC2": "phaseC2"}
+
+
+def _kernel_key(name):
+    """Kernel class of a rocprofv3 kernel name (None for other kernels)."""
+    base = name.replace("void ", "").replace("loma::", "").split("(")[0].split("<")[0]
+    return _TRAFFIC_KEYS.get(base.strip())
+
+
+def _pmc_bytes(path, counters):
+    """Per-dispatch bytes of one counter pass (csv of rocprofv3 --pmc)."""
+    import collections
+    import csv
+    per, names = collections.defaultdict(float), {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            k = int(r["Dispatch_Id"])
+            names[k] = r["Kernel_Name"]
+            per[k] += counters.get(r["Counter_Name"], 0.0) * float(r["Counter_Value"])
+    return per, names
+
+
+def measure_traffic(precision_name, timeout=240):
+    """HBM bytes per launch of each kernel class, measured in this run: two
+    rocprofv3 counter passes over one RK3 step of the 512x513x256 case
+    (python3 -m loma_phd_amd.profile_step, child processes).  Read bytes =
+    32/64/128 x the TCC read requests of each size, write bytes = WRITE_SIZE
+    x 1024, each divided by what the same sum reads for the step's trailing
+    1 GiB device copy (the calibration factor, 1.0 on MI355X).  Returns
+    (bytes per launch by kernel class, calibration) or None when rocprofv3 is
+    missing or a pass gives no counters."""
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None
+    passes = (("rd", {"TCC_EA0_RDREQ_32B_sum": 32.0, "TCC_EA0_RDREQ_64B_sum": 64.0,
+                      "TCC_EA0_RDREQ_128B_sum": 128.0, "TCC_EA0_RDREQ_sum": 0.0}),
+              ("wr", {"WRITE_SIZE": 1024.0}))
+    env = dict(os.environ, TMPDIR="/tmp",
+               PYTHONPATH=HERE + os.pathsep + os.environ.get("PYTHONPATH", ""))
+    got = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        for name, counters in passes:
+            d = os.path.join(tmp, name)
+            cmd = [exe, "--pmc", *counters, "--output-format", "csv", "-d", d, "-o", "p", "--",
+                   sys.executable, "-m", "loma_phd_amd.profile_step", precision_name]
+            proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                                    stderr=subprocess.DEVNULL, start_new_session=True)
+            try:
+                proc.wait(timeout=timeout)
+            except subprocess.TimeoutExpired:
+                os.killpg(proc.pid, signal.SIGKILL)  # the group this call started
+                proc.wait()
+                return None
+            # rocprofv3 may end non-zero after the program and its files are
+            # complete; the counters are what count
+            path = next((os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs
+                         if f.endswith("counter_collection.csv")), None)
+            if path is None:
+                return None
+            got[name] = _pmc_bytes(path, counters)
+    calib, out = {}, {}
+    for name, (per, names) in got.items():
+        copies = [v for k, v in per.items() if "copyBuffer" in names[k]]
+        if not copies or max(copies) <= 0:
+            return None
+        calib[name] = float(1 << 30) / max(copies)
+        by = {}
+        for k, v in per.items():
+            key = _kernel_key(names[k])
+            if key is not None:
+                by.setdefault(key, []).append(v)
+        for key, vals in by.items():
+            out[key] = out.get(key, 0.0) + calib[name] * sum(vals) / len(vals)
+    return out, {k: round(v, 4) for k, v in calib.items()}
+
+
 def algorithmic_bytes(run, with_z: bool):
     """Compulsory HBM bytes per launch of each kernel class (one substep)
     and per RK3 step: one read of every field a kernel consumes and one write
@@ -378,6 +460,9 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=30)
     ap.add_argument("--stats-steps", type=int, default=3,
                     help="steps timed with the statistics window closed and open (0: skip)")
+    ap.add_argument("--traffic", default="live", choices=["live", "const"],
+                    help="roofline.traffic from rocprofv3 counter passes of this run (live) "
+                         "or the published per-kernel constants")
     ap.add_argument("--served-window", type=float, default=30.0,
                     help="seconds of Poisson arrivals in the served leg (0 skips it)")
     args = ap.parse_args()
@@ -408,7 +493,24 @@ def main():
     mixed = "mixed" in res
     measured = (args.config == "B" and not cfg.scalar_z)
     tr = {p: (MEASURED_TRAFFIC[p] if measured else None) for p in ("fp64", "mixed")}
+    tr_source, tr_calib = "profiles/rk3_step_512x513x256_hbm_traffic.md (earlier run)", None
+    if measured and args.traffic == "live":
+        live = {}
+        for prec in res:
+            print(f"[bench] HBM traffic counters ({prec})", file=sys.stderr, flush=True)
+            m = measure_traffic(prec)
+            if m is None:
+                live = None
+                break
+            live[prec], tr_calib = m[0], m[1]
+        if live:
+            tr.update(live)
+            tr_source = "rocprofv3 counters of this run (loma_phd_amd.profile_step)"
     dom, roof = roofline_of(head, tr["mixed" if mixed else "fp64"])
+    if measured:
+        roof["traffic_source"] = tr_source
+        if tr_calib:
+            roof["traffic_calibration"] = tr_calib
     S, P = head["S"], head["P"]
     line = {
         "metric": "grid-pt*RK3-steps/s (mixed vs FP64, 1 GPU)",
