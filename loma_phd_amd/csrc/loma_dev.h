@@ -96,6 +96,23 @@ __device__ __forceinline__ TabRef relaunder_after(const TabRef& b, V dep) {
   asm volatile("" : "+s"(z) : "v"(dep));
   return TabRef{b.r, b.ncol, b.off, b.lane_off, z};
 }
+// Entries [0, K) of row t of a table view, issued together, and the wait
+// for a row loaded ahead, taken once for the whole row: left to itself the
+// scheduler issues each coefficient load next to its product, one memory
+// round trip per coefficient.
+template <int K, typename V>
+__device__ __forceinline__ void row_ld(const TabRef& b, int t, V (&c)[K]) {
+#pragma unroll
+  for (int i = 0; i < K; ++i) c[i] = tget_v<V>(b, t, i);
+}
+template <int K, typename V>
+__device__ __forceinline__ void row_take(const V (&n)[K], V (&c)[K]) {
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    c[i] = n[i];
+    asm volatile("" : "+v"(c[i]));
+  }
+}
 // index of entry `idx` of grid row `row` in a lane-major table
 template <int M>
 __host__ __device__ inline int lane_major_index(int row, int ncol, int idx) {
@@ -525,15 +542,22 @@ __device__ __forceinline__ void bmatvec(const TabRef B, int lane, const V (&u)[N
       hl[n][q] = lane == 0 ? V(0) : a;
       hr[n][q] = lane == WAVE - 1 ? V(0) : b;
     }
+  // each row's coefficients are loaded one row ahead (row_ld / row_take)
+  V bn[2 * P + 1];
+  row_ld<2 * P + 1>(B, 0, bn);
 #pragma unroll
   for (int t = 0; t < M; ++t) {
+    V bc[2 * P + 1];
+    row_take<2 * P + 1>(bn, bc);
+    if (t + 1 < M) row_ld<2 * P + 1>(B, t + 1, bn);
+    op_fence();
     V acc[NC];
 #pragma unroll
     for (int n = 0; n < NC; ++n) acc[n] = V(0);
 #pragma unroll
     for (int d = 0; d <= 2 * P; ++d) {
       const int col = t - P + d;
-      const V bv = tget_v<V>(B, t, d);
+      const V bv = bc[d];
 #pragma unroll
       for (int n = 0; n < NC; ++n) {
         const V v = col < 0 ? hl[n][col + P] : (col >= M ? hr[n][col - M] : u[n][col]);
