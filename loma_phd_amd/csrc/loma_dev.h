@@ -529,7 +529,9 @@ __device__ __forceinline__ void usolve_tab(const TabRef up, const TabRef inv, co
 }
 
 // ---- banded matvec y_r = sum_d B(r,d) u_{r-P+d} ---------------------------
-template <int M, int P, int NC, typename V>
+// ROWS: each row's coefficients are loaded together one row ahead
+// (row_ld / row_take); otherwise the scheduler places the loads.
+template <int M, int P, int NC, typename V, bool ROWS = true>
 __device__ __forceinline__ void bmatvec(const TabRef B, int lane, const V (&u)[NC][M],
                                         V (&y)[NC][M]) {
   V hl[NC][P], hr[NC][P];
@@ -542,15 +544,18 @@ __device__ __forceinline__ void bmatvec(const TabRef B, int lane, const V (&u)[N
       hl[n][q] = lane == 0 ? V(0) : a;
       hr[n][q] = lane == WAVE - 1 ? V(0) : b;
     }
-  // each row's coefficients are loaded one row ahead (row_ld / row_take)
   V bn[2 * P + 1];
-  row_ld<2 * P + 1>(B, 0, bn);
+  if constexpr (ROWS) row_ld<2 * P + 1>(B, 0, bn);
 #pragma unroll
   for (int t = 0; t < M; ++t) {
     V bc[2 * P + 1];
-    row_take<2 * P + 1>(bn, bc);
-    if (t + 1 < M) row_ld<2 * P + 1>(B, t + 1, bn);
-    op_fence();
+    if constexpr (ROWS) {
+      row_take<2 * P + 1>(bn, bc);
+      if (t + 1 < M) row_ld<2 * P + 1>(B, t + 1, bn);
+      op_fence();
+    } else {
+      row_ld<2 * P + 1>(B, t, bc);
+    }
     V acc[NC];
 #pragma unroll
     for (int n = 0; n < NC; ++n) acc[n] = V(0);
@@ -680,7 +685,7 @@ template <int M, int NC>
 __device__ __forceinline__ void op_helm(const YTab& T, const double* __restrict__ lu,
                                         int lane, const double (&f)[NC][M],
                                         double (&v)[NC][M]) {
-  bmatvec<M, 2, NC>(ytab<M>(T, YT_A2, 5, lane), lane, f, v);
+  bmatvec<M, 2, NC, double, false>(ytab<M>(T, YT_A2, 5, lane), lane, f, v);
   const int r0 = lane * M;
 #pragma unroll
   for (int t = 0; t < M; ++t) {
