@@ -642,10 +642,10 @@ __device__ __forceinline__ TabRef ytab_v(const YTab& T, int id, int ncol, int la
 }
 
 // first derivative du = fmap * A1^{-1} B1 u   (deryr2, cfdiff.f:518-601)
-template <int M, int NC, typename V>
+template <int M, int NC, typename V, bool ROWS = true>
 __device__ __forceinline__ void op_dy1(const YTab& T, int lane, const V (&u)[NC][M],
                                        V (&du)[NC][M], bool apply_fmap = true) {
-  bmatvec<M, 3, NC>(ytab_v<M, V>(T, YT_B1, 7, lane), lane, u, du);
+  bmatvec<M, 3, NC, V, ROWS>(ytab_v<M, V>(T, YT_B1, 7, lane), lane, u, du);
   op_fence();
   lsolve_tab<M, 3, NC>(ytab_v<M, V>(T, YT_LO1, 3, lane), ytab_v<M, V>(T, YT_HL1, 3, lane), T.kL1, lane, du);
   op_fence();
@@ -665,10 +665,10 @@ __device__ __forceinline__ void op_dy1(const YTab& T, int lane, const V (&u)[NC]
 }
 
 // second derivative d2u = A2^{-1} B2 u   (deryyr2, cfdiff.f:735-797)
-template <int M, int NC, typename V>
+template <int M, int NC, typename V, bool ROWS = true>
 __device__ __forceinline__ void op_dy2(const YTab& T, int lane, const V (&u)[NC][M],
                                        V (&d2u)[NC][M]) {
-  bmatvec<M, 2, NC>(ytab_v<M, V>(T, YT_B2, 5, lane), lane, u, d2u);
+  bmatvec<M, 2, NC, V, ROWS>(ytab_v<M, V>(T, YT_B2, 5, lane), lane, u, d2u);
   op_fence();
   lsolve_tab<M, 2, NC>(ytab_v<M, V>(T, YT_LO2, 2, lane), ytab_v<M, V>(T, YT_HL2, 2, lane), T.kL2, lane, d2u);
   op_fence();
